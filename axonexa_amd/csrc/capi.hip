@@ -1,0 +1,725 @@
+// capi.hip — host side of the C ABI declared in include/axonexa.h.
+//
+// Replaces the reference's host orchestration of the lattice walk,
+// phaseAcquisitionStream<F>::runAcquisitionLattice
+// (src/acquisition/phaseAcquisitionStream.cuh:126-190): validation, the
+// periodic images and cell bins of the substrate, per-compartment tables,
+// device allocation, kernel launches on the context stream and the
+// device-to-host copy of the signal sums.  Errors are returned as codes, never
+// aborts (the reference's safe_cuda resets the device and asserts,
+// src/util/misc.cuh:5-11).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/axonexa.h"
+#include "reduce_kernel.h"
+#include "walk_kernel.h"
+
+using namespace axx;
+
+struct axx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t lds_max = 65536;
+    std::string err;
+};
+
+struct axx_plan {
+    axx_ctx* ctx = nullptr;
+    // host copies
+    int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7;
+    bool sph = false;
+    long long N = 0;
+    axx_run run{};
+    double dt = 0;
+    std::vector<double> S, wf_scale, wf_dir;
+    std::vector<int> wf_p;
+    // geometry sizes
+    int n_img = 0, n_bins = 0, n_cand = 0, nb[3] = {1, 1, 1};
+    float ib[3] = {1, 1, 1};
+    double bin_width = 0;
+    size_t lds_bytes = 0;
+    int init_img = 0;
+    int block = 256, wpt = 1;
+    int next_step = 0;
+    bool reduced = false;
+    // device buffers
+    float4* d_geom = nullptr;
+    int* d_meta = nullptr;
+    uint32_t* d_bins = nullptr;
+    uint16_t* d_cand = nullptr;
+    float* d_L = nullptr;
+    float* d_rate = nullptr;
+    float* d_thr_out = nullptr;
+    float* d_thr_in = nullptr;
+    float* d_prof = nullptr;
+    uint32_t* d_state = nullptr;
+    int* d_err = nullptr;
+    int* d_wfp = nullptr;
+    double* d_wfdir = nullptr;
+    double* d_wfscale = nullptr;
+    double* d_S = nullptr;
+    double* d_partial = nullptr;
+    double* d_out = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    int n_chunks = 0, chunk = 0;
+    float cell[3] = {0, 0, 0};
+    float H2c = 0, H2s = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static int fail(axx_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), AXX_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char* axx_version(void) { return "axonexa_amd 0.1.0 (gfx950)"; }
+
+extern "C" const char* axx_strerror(int code) {
+    switch (code) {
+    case AXX_OK: return "ok";
+    case AXX_ERR_INVALID: return "invalid argument";
+    case AXX_ERR_DEVICE: return "HIP runtime error";
+    case AXX_ERR_NOMEM: return "out of memory";
+    case AXX_ERR_LIMIT: return "size beyond a compiled limit";
+    case AXX_ERR_GEOMETRY: return "invalid geometry (overlap or step >= half cell)";
+    case AXX_ERR_INIT: return "rejection initialisation did not finish";
+    case AXX_ERR_NODEVICE: return "no HIP device";
+    case AXX_ERR_STATE: return "call out of order";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char* axx_last_error(const axx_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+extern "C" int axx_create(axx_ctx** out, int device) {
+    if (!out) return AXX_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return AXX_ERR_NODEVICE;
+    if (device < 0 || device >= n) return AXX_ERR_INVALID;
+    axx_ctx* c = new (std::nothrow) axx_ctx();
+    if (!c) return AXX_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return AXX_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->n_cu = prop.multiProcessorCount;
+        c->lds_max = prop.sharedMemPerBlock;
+    }
+    *out = c;
+    return AXX_OK;
+}
+
+extern "C" void axx_destroy(axx_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+// ---------------------------------------------------------------------------
+// Substrate preparation
+
+struct HostPrep {
+    std::vector<float4> geom;
+    std::vector<int> meta;
+    std::vector<int> orig;
+    std::vector<float> L, rate, thr_out, thr_in;
+    std::vector<uint32_t> bins;
+    std::vector<uint16_t> cand;
+    int nb[3] = {1, 1, 1};
+    float ib[3] = {1, 1, 1};
+    double bin_width = 0;
+    double Lmax = 0;
+    bool sph = false;
+};
+
+static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
+    if (!(s->a > 0 && s->b > 0 && s->c > 0)) return fail(ctx, AXX_ERR_INVALID, "cell lengths must be > 0");
+    if (s->n_prims < 0) return fail(ctx, AXX_ERR_INVALID, "n_prims < 0");
+    if (s->n_prims > AXX_MAX_PRIMS) return fail(ctx, AXX_ERR_LIMIT, "n_prims > %d", AXX_MAX_PRIMS);
+    if (s->n_prims > 0 && (!s->type || !s->center || !s->radius || !s->D || !s->T2))
+        return fail(ctx, AXX_ERR_INVALID, "primitive arrays missing");
+    if (!(q->dt > 0) || q->n_steps < 1) return fail(ctx, AXX_ERR_INVALID, "dt and n_steps must be > 0");
+    if (q->n_profiles < 1 || q->n_profiles > AXX_MAX_PROFILES)
+        return fail(ctx, AXX_ERR_LIMIT, "n_profiles must be in [1, %d]", AXX_MAX_PROFILES);
+    if (!q->profiles) return fail(ctx, AXX_ERR_INVALID, "profiles missing");
+    if (q->n_waveforms < 1 || !q->wf_profile || !q->wf_amplitude || !q->wf_dir)
+        return fail(ctx, AXX_ERR_INVALID, "waveforms missing");
+    for (int j = 0; j < q->n_waveforms; ++j)
+        if (q->wf_profile[j] < 0 || q->wf_profile[j] >= q->n_profiles)
+            return fail(ctx, AXX_ERR_INVALID, "waveform %d: profile index out of range", j);
+    for (long long i = 0; i < (long long)q->n_steps * q->n_profiles; ++i)
+        if (!std::isfinite(q->profiles[i])) return fail(ctx, AXX_ERR_INVALID, "non-finite profile sample");
+    if (!(s->D_lat >= 0)) return fail(ctx, AXX_ERR_INVALID, "D_lat < 0");
+    double Lmax = std::sqrt(6.0 * s->D_lat * q->dt);
+    for (int k = 0; k < s->n_prims; ++k) {
+        if (s->type[k] != AXX_PRIM_CYLINDER_XY && s->type[k] != AXX_PRIM_SPHERE)
+            return fail(ctx, AXX_ERR_INVALID, "primitive %d: unknown type", k);
+        if (!(s->radius[k] > 0) || !(s->D[k] >= 0))
+            return fail(ctx, AXX_ERR_INVALID, "primitive %d: radius/D", k);
+        const double R = s->radius[k];
+        if (2 * R >= s->a || 2 * R >= s->b || (s->type[k] == AXX_PRIM_SPHERE && 2 * R >= s->c))
+            return fail(ctx, AXX_ERR_GEOMETRY, "primitive %d larger than half the cell", k);
+        Lmax = std::max(Lmax, std::sqrt(6.0 * s->D[k] * q->dt));
+    }
+    if (2 * Lmax >= s->a || 2 * Lmax >= s->b || 2 * Lmax >= s->c)
+        return fail(ctx, AXX_ERR_GEOMETRY, "step length %.3g >= half the cell", Lmax);
+    // non-overlap under the minimum image (RPSinitializer.h:191-219, with
+    // every image shift)
+    const double Ls[3] = {s->a, s->b, s->c};
+    for (int i = 0; i < s->n_prims; ++i)
+        for (int j = i + 1; j < s->n_prims; ++j) {
+            const int dims = (s->type[i] == AXX_PRIM_SPHERE && s->type[j] == AXX_PRIM_SPHERE) ? 3 : 2;
+            double d2 = 0;
+            for (int a = 0; a < dims; ++a) {
+                double d = s->center[3 * i + a] - s->center[3 * j + a];
+                d -= Ls[a] * std::nearbyint(d / Ls[a]);
+                d2 += d * d;
+            }
+            const double rr = s->radius[i] + s->radius[j];
+            if (d2 < rr * rr) return fail(ctx, AXX_ERR_GEOMETRY, "primitives %d and %d overlap", i, j);
+        }
+    return AXX_OK;
+}
+
+static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], double reach) {
+    const double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
+    const int dims = s->type[k] == AXX_PRIM_SPHERE ? 3 : 2;
+    const double R = s->radius[k] + reach;
+    double d2 = 0;
+    for (int a = 0; a < dims; ++a) {
+        const double c = s->center[3 * k + a] + sh[a];
+        double d = 0;
+        if (c < lo[a]) d = lo[a] - c;
+        else if (c > hi[a]) d = c - hi[a];
+        d2 += d * d;
+    }
+    return d2 <= R * R;
+}
+
+static bool surface_near_box(const float4& g, bool sph, const double lo[3], const double hi[3], double reach) {
+    const double c[3] = {g.x, g.y, g.z};
+    const double R = std::sqrt((double)g.w);
+    const int dims = sph ? 3 : 2;
+    double dmin2 = 0, dmax2 = 0;
+    for (int a = 0; a < dims; ++a) {
+        double d = 0;
+        if (c[a] < lo[a]) d = lo[a] - c[a];
+        else if (c[a] > hi[a]) d = c[a] - hi[a];
+        dmin2 += d * d;
+        const double e = std::max(std::fabs(c[a] - lo[a]), std::fabs(c[a] - hi[a]));
+        dmax2 += e * e;
+    }
+    return std::sqrt(dmin2) <= R + reach && std::sqrt(dmax2) >= R - reach;
+}
+
+static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand) {
+    return n_img * 16 + n_img * 4 + n_bins * 4 + n_cand * 2;
+}
+
+// Images, compartments and bins.  The float32 image centres are
+// (float)(centre + shift) computed in double; compartments use
+// L = (float)sqrt(6 D dt), rate = (float)(dt / T2), thresholds
+// (float)(4 kappa / v).
+static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_t lds_budget,
+                   double bin_width_req, HostPrep& H) {
+    const int n = s->n_prims;
+    const double dt = q->dt;
+    H.L.resize(n + 1);
+    H.rate.resize(n + 1);
+    H.thr_out.assign(std::max(n, 1), 0.0f);
+    H.thr_in.assign(std::max(n, 1), 0.0f);
+    double Lmax = std::sqrt(6.0 * s->D_lat * dt);
+    H.L[0] = (float)std::sqrt(6.0 * s->D_lat * dt);
+    H.rate[0] = (s->T2_lat > 0 && std::isfinite(s->T2_lat)) ? (float)(dt / s->T2_lat) : 0.0f;
+    const double v_lat = std::sqrt(6.0 * s->D_lat / dt);
+    for (int k = 0; k < n; ++k) {
+        const double L = std::sqrt(6.0 * s->D[k] * dt);
+        Lmax = std::max(Lmax, L);
+        H.L[k + 1] = (float)L;
+        H.rate[k + 1] = (s->T2[k] > 0 && std::isfinite(s->T2[k])) ? (float)(dt / s->T2[k]) : 0.0f;
+        const double kap = s->kappa ? s->kappa[k] : 0.0;
+        const double v_in = std::sqrt(6.0 * s->D[k] / dt);
+        H.thr_out[k] = kap > 0 ? (float)(4.0 * kap / v_in) : 0.0f;
+        H.thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
+        if (s->type[k] == AXX_PRIM_SPHERE) H.sph = true;
+    }
+    H.Lmax = Lmax;
+    const double reach = Lmax * 1.001 + 1e-6 * std::max(s->a, s->b);
+    for (int k = 0; k < n; ++k) {
+        const int zr = s->type[k] == AXX_PRIM_SPHERE ? 1 : 0;
+        for (int kz = -zr; kz <= zr; ++kz)
+            for (int ky = -1; ky <= 1; ++ky)
+                for (int kx = -1; kx <= 1; ++kx) {
+                    const double sh[3] = {kx * s->a, ky * s->b, kz * s->c};
+                    if (!reach_hits_cell(s, k, sh, reach)) continue;
+                    const float Rf = (float)s->radius[k];
+                    H.geom.push_back(make_float4((float)(s->center[3 * k] + sh[0]),
+                                                 (float)(s->center[3 * k + 1] + sh[1]),
+                                                 (float)(s->center[3 * k + 2] + sh[2]), Rf * Rf));
+                    H.meta.push_back((k + 1) | (s->type[k] == AXX_PRIM_SPHERE ? kSphereBit : 0));
+                    H.orig.push_back(k);
+                }
+    }
+    const int n_img = (int)H.geom.size();
+    if (n_img > 65535) return fail(ctx, AXX_ERR_LIMIT, "too many primitive images (%d)", n_img);
+    // bin width: at least 4 step lengths; widen until the tables fit in LDS
+    const double Ls[3] = {s->a, s->b, s->c};
+    double w = bin_width_req > 0 ? bin_width_req : std::max(4.0 * Lmax, 1e-9);
+    for (int attempt = 0; attempt < 64; ++attempt) {
+        int nb[3];
+        for (int a = 0; a < 3; ++a) {
+            int v = (int)std::floor(Ls[a] / w);
+            nb[a] = std::min(std::max(v, 1), 2048);
+        }
+        if (!H.sph) nb[2] = 1;
+        const long long nbins = (long long)nb[0] * nb[1] * nb[2];
+        if (nbins > (1 << 22) || lds_size(n_img, nbins, 0) > lds_budget) {
+            if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
+            w *= 1.25;
+            continue;
+        }
+        // per-bin lists, filled image by image (ascending image index) over
+        // the bins that the image's grown bounding box touches
+        std::vector<std::vector<uint16_t>> lists(nbins);
+        for (int i = 0; i < n_img; ++i) {
+            const bool sk = (H.meta[i] & kSphereBit) != 0;
+            const double R = std::sqrt((double)H.geom[i].w) + reach;
+            const double c3[3] = {H.geom[i].x, H.geom[i].y, H.geom[i].z};
+            int lo_b[3], hi_b[3];
+            for (int a = 0; a < 3; ++a) {
+                if (a == 2 && !sk) { lo_b[a] = 0; hi_b[a] = nb[a] - 1; continue; }
+                const double bw = Ls[a] / nb[a];
+                lo_b[a] = std::max(0, (int)std::floor((c3[a] - R) / bw) - 1);
+                hi_b[a] = std::min(nb[a] - 1, (int)std::floor((c3[a] + R) / bw) + 1);
+            }
+            for (int bz = lo_b[2]; bz <= hi_b[2]; ++bz)
+                for (int by = lo_b[1]; by <= hi_b[1]; ++by)
+                    for (int bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
+                        const double lo[3] = {bx * s->a / nb[0], by * s->b / nb[1], bz * s->c / nb[2]};
+                        const double hi[3] = {(bx + 1) * s->a / nb[0], (by + 1) * s->b / nb[1],
+                                              (bz + 1) * s->c / nb[2]};
+                        if (surface_near_box(H.geom[i], sk, lo, hi, reach))
+                            lists[(bz * nb[1] + by) * nb[0] + bx].push_back((uint16_t)i);
+                    }
+        }
+        std::vector<uint32_t> bins(nbins);
+        std::vector<uint16_t> cand;
+        bool ok = true;
+        for (long long bi = 0; bi < nbins; ++bi) {
+            const size_t off = cand.size();
+            const size_t cnt = lists[bi].size();
+            if (cnt > 255 || off >= (1u << 24)) { ok = false; break; }
+            cand.insert(cand.end(), lists[bi].begin(), lists[bi].end());
+            bins[bi] = (uint32_t)(off << 8) | (uint32_t)cnt;
+        }
+        if (!ok || lds_size(n_img, nbins, cand.size()) > lds_budget) {
+            if (bin_width_req > 0 && !ok) return fail(ctx, AXX_ERR_LIMIT, "bin with > 255 candidates");
+            if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
+            if (!ok) { w *= 0.8; if (w < Lmax) return fail(ctx, AXX_ERR_LIMIT, "cannot bin substrate"); continue; }
+            w *= 1.25;
+            continue;
+        }
+        for (int a = 0; a < 3; ++a) {
+            H.nb[a] = nb[a];
+            H.ib[a] = (float)(nb[a] / Ls[a]);
+        }
+        H.bins.swap(bins);
+        H.cand.swap(cand);
+        if (H.cand.empty()) H.cand.push_back(0);
+        H.bin_width = w;
+        return AXX_OK;
+    }
+    return fail(ctx, AXX_ERR_LIMIT, "substrate tables do not fit in LDS");
+}
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch over (profile width, axes, sphere) instantiations
+
+typedef void (*walk_fn)(WalkArgs);
+
+template <int P, int AX>
+static walk_fn pick_sph(bool sph) {
+    return sph ? (walk_fn)walk_kernel<P, AX, true> : (walk_fn)walk_kernel<P, AX, false>;
+}
+template <int P>
+static walk_fn pick_ax(int ax, bool sph) {
+    switch (ax) {
+    case 1: return pick_sph<P, 1>(sph);
+    case 3: return pick_sph<P, 3>(sph);
+    case 5: return pick_sph<P, 5>(sph);
+    default: return pick_sph<P, 7>(sph);
+    }
+}
+static walk_fn pick_walk(int Pt, int ax, bool sph) {
+    switch (Pt) {
+    case 1: return pick_ax<1>(ax, sph);
+    case 2: return pick_ax<2>(ax, sph);
+    case 4: return pick_ax<4>(ax, sph);
+    case 8: return pick_ax<8>(ax, sph);
+    case 16: return pick_ax<16>(ax, sph);
+    case 32: return pick_ax<32>(ax, sph);
+    default: return pick_ax<64>(ax, sph);
+    }
+}
+
+static int axes_mask(const axx_acq* q) {
+    int mask = 0;
+    for (int j = 0; j < q->n_waveforms; ++j)
+        for (int a = 0; a < 3; ++a)
+            if (q->wf_dir[3 * j + a] != 0.0 && q->wf_amplitude[j] != 0.0) mask |= 1 << a;
+    static const int widen[8] = {1, 1, 3, 3, 5, 5, 7, 7};
+    return widen[mask];
+}
+
+template <typename T>
+static int dupload(axx_ctx* ctx, T** dst, const T* src, size_t n) {
+    if (n == 0) n = 1;
+    HIP_TRY(ctx, hipMalloc((void**)dst, n * sizeof(T)));
+    if (src) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return AXX_OK;
+}
+
+static void plan_free(axx_plan* p) {
+    if (!p) return;
+    void* bufs[] = {p->d_geom, p->d_meta, p->d_bins, p->d_cand, p->d_L, p->d_rate,
+                    p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
+                    p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    delete p;
+}
+
+extern "C" void axx_plan_destroy(axx_plan* plan) {
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    plan_free(plan);
+}
+
+static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
+                            const axx_run* r, axx_plan* p) {
+    int rc = validate(ctx, s, q);
+    if (rc) return rc;
+    if (r->n_walkers < 1) return fail(ctx, AXX_ERR_INVALID, "n_walkers must be >= 1");
+    if (r->n_walkers > (1ll << 32)) return fail(ctx, AXX_ERR_LIMIT, "n_walkers > 2^32");
+    if (r->boundary_mode < 0 || r->boundary_mode > 2) return fail(ctx, AXX_ERR_INVALID, "boundary_mode");
+    if (r->init_mode < 0 || r->init_mode > 2) return fail(ctx, AXX_ERR_INVALID, "init_mode");
+    if (r->init_mode == AXX_INIT_INSIDE_PRIM && (r->init_arg < 0 || r->init_arg >= s->n_prims))
+        return fail(ctx, AXX_ERR_INVALID, "init_arg: no such primitive");
+    if (r->init_mode == AXX_INIT_COMPARTMENT && (r->init_arg < 0 || r->init_arg > s->n_prims))
+        return fail(ctx, AXX_ERR_INVALID, "init_arg: no such compartment");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HostPrep H;
+    const size_t budget = std::min<size_t>(ctx->lds_max, 96 * 1024);
+    rc = prepare(ctx, s, q, budget, 0.0, H);
+    if (rc) return rc;
+    p->ctx = ctx;
+    p->run = *r;
+    p->N = r->n_walkers;
+    p->n_steps = q->n_steps;
+    p->dt = q->dt;
+    p->P = q->n_profiles;
+    p->Pt = 1;
+    while (p->Pt < p->P) p->Pt *= 2;
+    p->M = q->n_waveforms;
+    p->ax = axes_mask(q);
+    p->sph = H.sph;
+    p->cell[0] = (float)s->a; p->cell[1] = (float)s->b; p->cell[2] = (float)s->c;
+    {
+        const double hc = 0.5 * std::min(s->a, s->b), hs = 0.5 * std::min(std::min(s->a, s->b), s->c);
+        p->H2c = (float)(hc * hc);
+        p->H2s = (float)(hs * hs);
+    }
+    p->n_img = (int)H.geom.size();
+    p->n_bins = (int)H.bins.size();
+    p->n_cand = (int)H.cand.size();
+    for (int a = 0; a < 3; ++a) { p->nb[a] = H.nb[a]; p->ib[a] = H.ib[a]; }
+    p->bin_width = H.bin_width;
+    p->lds_bytes = lds_size(p->n_img, p->n_bins, p->n_cand);
+    p->lds_bytes = (p->lds_bytes + 15) & ~(size_t)15;
+    p->init_img = 0;
+    if (r->init_mode == AXX_INIT_INSIDE_PRIM) {
+        p->init_img = -1;
+        for (int i = 0; i < p->n_img; ++i)
+            if (H.orig[i] == r->init_arg) { p->init_img = i; break; }
+        if (p->init_img < 0) return fail(ctx, AXX_ERR_INVALID, "init primitive has no image in the cell");
+    }
+    // profiles padded to the template width
+    std::vector<float> prof((size_t)q->n_steps * p->Pt, 0.0f);
+    for (int i = 0; i < q->n_steps; ++i)
+        for (int k = 0; k < p->P; ++k) prof[(size_t)i * p->Pt + k] = q->profiles[(size_t)i * p->P + k];
+    p->S.assign(p->Pt, 0.0);
+    for (int i = 0; i < q->n_steps; ++i)
+        for (int k = 0; k < p->P; ++k) p->S[k] += (double)q->profiles[(size_t)i * p->P + k];
+    p->wf_p.assign(q->wf_profile, q->wf_profile + p->M);
+    p->wf_dir.assign(q->wf_dir, q->wf_dir + 3 * p->M);
+    p->wf_scale.resize(p->M);
+    for (int j = 0; j < p->M; ++j) p->wf_scale[j] = q->gamma * q->dt * q->wf_amplitude[j];
+
+    if ((rc = dupload(ctx, &p->d_geom, H.geom.data(), H.geom.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_L, H.L.data(), H.L.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_rate, H.rate.data(), H.rate.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_thr_out, H.thr_out.data(), H.thr_out.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_thr_in, H.thr_in.data(), H.thr_in.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_prof, prof.data(), prof.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_wfp, p->wf_p.data(), p->wf_p.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_wfdir, p->wf_dir.data(), p->wf_dir.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_wfscale, p->wf_scale.data(), p->wf_scale.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_S, p->S.data(), p->S.size()))) return rc;
+    const size_t nst = (size_t)AXX_STATE_FIELDS(p->Pt) * (size_t)p->N;
+    if (hipMalloc((void**)&p->d_state, nst * sizeof(uint32_t)) != hipSuccess)
+        return fail(ctx, AXX_ERR_NOMEM, "walker state (%zu bytes)", nst * 4);
+    HIP_TRY(ctx, hipMemset(p->d_state, 0, nst * sizeof(uint32_t)));
+    if ((rc = dupload<int>(ctx, &p->d_err, nullptr, 1))) return rc;
+    HIP_TRY(ctx, hipMemset(p->d_err, 0, sizeof(int)));
+    p->chunk = kRedThreads * 16;
+    p->n_chunks = (int)((p->N + p->chunk - 1) / p->chunk);
+    if ((rc = dupload<double>(ctx, &p->d_partial, nullptr, (size_t)p->M * p->n_chunks * kRedVals))) return rc;
+    if ((rc = dupload<double>(ctx, &p->d_out, nullptr, (size_t)p->M * kRedVals))) return rc;
+    if ((rc = dupload<unsigned long long>(ctx, &p->d_cnt, nullptr, 4))) return rc;
+    HIP_TRY(ctx, hipEventCreate(&p->ev0));
+    HIP_TRY(ctx, hipEventCreate(&p->ev1));
+    p->block = 256;
+    p->wpt = 1;
+    HIP_TRY(ctx, hipDeviceSynchronize());
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
+                               const axx_run* r, axx_plan** out) {
+    if (!ctx || !s || !q || !r || !out) return fail(ctx, AXX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    axx_plan* p = new (std::nothrow) axx_plan();
+    if (!p) return AXX_ERR_NOMEM;
+    p->ctx = ctx;
+    int rc = plan_create_impl(ctx, s, q, r, p);
+    if (rc) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return AXX_OK;
+}
+
+static int launch_walk(axx_plan* p, int i0, int i1) {
+    axx_ctx* ctx = p->ctx;
+    WalkArgs A{};
+    A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
+    A.H2c = p->H2c; A.H2s = p->H2s;
+    A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
+    A.ibx = p->ib[0]; A.iby = p->ib[1]; A.ibz = p->ib[2];
+    A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
+    A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
+    A.compL = p->d_L; A.rate = p->d_rate; A.thr_out = p->d_thr_out; A.thr_in = p->d_thr_in;
+    A.prof = p->d_prof;
+    A.state = p->d_state;
+    A.N = p->N;
+    A.k0 = (uint32_t)p->run.seed; A.k1 = (uint32_t)(p->run.seed >> 32);
+    A.inst = p->run.instance; A.woff = p->run.walker_offset;
+    A.mode = p->run.boundary_mode; A.init_mode = p->run.init_mode; A.init_arg = p->run.init_arg;
+    A.init_img = p->init_img;
+    A.i0 = i0; A.i1 = i1;
+    A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
+    A.err = p->d_err;
+    walk_fn fn = pick_walk(p->Pt, p->ax, p->sph);
+    const int block = p->block;
+    const long long grid = (p->N + block - 1) / block;
+    if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), p->lds_bytes, ctx->stream, A);
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_walk(axx_plan* p, int step_begin, int step_end) {
+    if (!p) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    if (step_begin < 0 || step_end > p->n_steps || step_begin > step_end)
+        return fail(ctx, AXX_ERR_INVALID, "step range [%d, %d) outside [0, %d)", step_begin, step_end,
+                    p->n_steps);
+    if (step_begin != p->next_step)
+        return fail(ctx, AXX_ERR_STATE, "walk must continue at step %d", p->next_step);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = launch_walk(p, step_begin, step_end);
+    if (rc) return rc;
+    p->next_step = step_end;
+    p->reduced = false;
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_time_walk(axx_plan* p, int step_begin, int step_end, float* ms) {
+    if (!p || !ms) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipEventRecord(p->ev0, ctx->stream));
+    int rc = axx_plan_walk(p, step_begin, step_end);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(p->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(p->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, p->ev0, p->ev1));
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_reduce(axx_plan* p) {
+    if (!p) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    if (p->next_step != p->n_steps)
+        return fail(ctx, AXX_ERR_STATE, "walk not finished (at step %d of %d)", p->next_step, p->n_steps);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    RedArgs R{};
+    R.state = p->d_state;
+    R.N = p->N;
+    R.P = p->Pt;
+    R.M = p->M;
+    R.n_steps = p->n_steps;
+    R.relaxation = p->run.relaxation;
+    R.chunk = p->chunk;
+    R.n_chunks = p->n_chunks;
+    R.rate = p->d_rate;
+    R.wf_p = p->d_wfp;
+    R.wf_dir = p->d_wfdir;
+    R.wf_scale = p->d_wfscale;
+    R.S = p->d_S;
+    R.partial = p->d_partial;
+    R.counters = p->d_cnt;
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_cnt, 4);
+    const int groups = (p->M + kRedJB - 1) / kRedJB;
+    hipLaunchKernelGGL(reduce_kernel, dim3(p->n_chunks, groups), dim3(kRedThreads), 0, ctx->stream, R);
+    HIP_TRY(ctx, hipGetLastError());
+    const int tot = p->M * kRedVals;
+    hipLaunchKernelGGL(finalize_kernel, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double*)p->d_partial, p->M, p->n_chunks, p->d_out);
+    HIP_TRY(ctx, hipGetLastError());
+    p->reduced = true;
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_sync(axx_plan* p) {
+    if (!p) return AXX_ERR_INVALID;
+    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_results(axx_plan* p, axx_result* out) {
+    if (!p || !out) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    if (!p->reduced) return fail(ctx, AXX_ERR_STATE, "results requested before reduce");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<double> h((size_t)p->M * kRedVals);
+    unsigned long long cnt[4];
+    int err = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), p->d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(cnt, p->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+    for (int j = 0; j < p->M; ++j) {
+        const double* v = &h[(size_t)j * kRedVals];
+        if (out->sum_cos) out->sum_cos[j] = v[0];
+        if (out->sum_sin) out->sum_sin[j] = v[1];
+        if (out->sum_cos2) out->sum_cos2[j] = v[2];
+        if (out->sum_sin2) out->sum_sin2[j] = v[3];
+        if (out->intra_cos) out->intra_cos[j] = v[4];
+        if (out->intra_sin) out->intra_sin[j] = v[5];
+    }
+    out->n_walkers = p->N;
+    out->collisions = cnt[0];
+    out->transmissions = cnt[1];
+    out->cap_hits = cnt[2];
+    out->n_intra = (int64_t)cnt[3];
+    return AXX_OK;
+}
+
+// State is exchanged in the public layout (AXX_STATE_FIELDS(n_profiles));
+// internally moments are kept for the padded profile width.
+extern "C" int axx_plan_get_state(axx_plan* p, uint32_t* host) {
+    if (!p || !host) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t nf = AXX_STATE_FIELDS(p->P);
+    HIP_TRY(ctx, hipMemcpy(host, p->d_state, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int err = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_set_state(axx_plan* p, const uint32_t* host) {
+    if (!p || !host) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t nf = AXX_STATE_FIELDS(p->P);
+    HIP_TRY(ctx, hipMemcpy(p->d_state, host, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    const int* is = reinterpret_cast<const int*>(host);
+    p->next_step = is[(size_t)F_STEP * p->N];
+    p->reduced = false;
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_get_config(const axx_plan* p, int* block, int* wpt, double* bin_width, int* lds) {
+    if (!p) return AXX_ERR_INVALID;
+    if (block) *block = p->block;
+    if (wpt) *wpt = p->wpt;
+    if (bin_width) *bin_width = p->bin_width;
+    if (lds) *lds = (int)p->lds_bytes;
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_set_config(axx_plan* p, int block, int wpt) {
+    if (!p) return AXX_ERR_INVALID;
+    if (block < 64 || block > 1024 || (block % 64) != 0) return fail(p->ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+    if (wpt != 1) return fail(p->ctx, AXX_ERR_INVALID, "walkers per thread must be 1");
+    p->block = block;
+    p->wpt = wpt;
+    return AXX_OK;
+}
+
+extern "C" int axx_plan_profile_count(const axx_plan* p) { return p ? p->P : AXX_ERR_INVALID; }
+
+extern "C" int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
+                             axx_result* out) {
+    if (!ctx || !out) return AXX_ERR_INVALID;
+    axx_plan* p = nullptr;
+    int rc = axx_plan_create(ctx, s, q, r, &p);
+    if (rc) return rc;
+    rc = axx_plan_walk(p, 0, q->n_steps);
+    if (!rc) rc = axx_plan_reduce(p);
+    if (!rc) rc = axx_plan_results(p, out);
+    axx_plan_destroy(p);
+    return rc;
+}

@@ -1,0 +1,139 @@
+// device_math.h — random numbers and geometry predicates of the walk.
+//
+// Every operation is written out explicitly (fmaf where a fused multiply-add
+// is intended; the library is compiled with -ffp-contract=off) so that the
+// arithmetic is fully specified; sqrtf and '/' are the correctly rounded IEEE
+// operations.
+//
+// Reference counterparts: curand_uniform / acos / sin / cos of the step
+// direction (src/kernels/kernelLattice.cuh:52-57), Cylinder_XY::intersect
+// (src/primitives/cylinderXY.cuh:82-122), Sphere::intersect
+// (src/primitives/Sphere.cuh:102-139), Lattice::lmod (src/substrate/
+// lattice.cuh:66-88).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace axx {
+
+// Philox4x32-10 (Salmon et al., SC'11).  Replaces cuRAND XORWOW
+// (src/kernels/kernelSetup.cuh:1-5): stateless, keyed by (seed), counter =
+// (walker, index, stream<<24 | sub, instance).
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                          uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+constexpr uint32_t kStreamStep = 0u;
+constexpr uint32_t kStreamColl = 1u << 24;
+constexpr uint32_t kStreamInit = 2u << 24;
+
+// Uniform in (0,1): odd multiples of 2^-24 (exactly representable).
+__device__ __forceinline__ float unif(uint32_t w) {
+    return (float)((w >> 8) | 1u) * 0x1p-24f;
+}
+
+// sin and cos of 2*pi*u(w): integer quarter-turn reduction, Taylor
+// polynomials of sin(pi/2 r), cos(pi/2 r) for r in [-1/2, 1/2].
+__device__ __forceinline__ void sincos_turn(uint32_t w, float& s, float& c) {
+    uint32_t v = (w >> 8) | 1u;
+    uint32_t q = (v + (1u << 21)) >> 22;
+    int32_t ri = (int32_t)v - (int32_t)(q << 22);
+    float r = (float)ri * 0x1p-22f;
+    float y = r * r;
+    float ps = fmaf(y, 0.00016044118478735982f, -0.004681754135318687f);
+    ps = fmaf(y, ps, 0.07969262624616704f);
+    ps = fmaf(y, ps, -0.6459640975062462f);
+    ps = fmaf(y, ps, 1.5707963267948966f);
+    float sp = r * ps;
+    float pc = fmaf(y, -0.000025202042373060605f, 0.00091926027483942659f);
+    pc = fmaf(y, pc, -0.020863480763352960f);
+    pc = fmaf(y, pc, 0.25366950790104802f);
+    pc = fmaf(y, pc, -1.2337005501361697f);
+    float cp = fmaf(y, pc, 1.0f);
+    uint32_t qq = q & 3u;
+    float a0 = (qq & 1u) ? cp : sp;   // sin magnitude source
+    float b0 = (qq & 1u) ? sp : cp;   // cos magnitude source
+    s = (qq >= 2u) ? -a0 : a0;
+    c = (qq == 1u || qq == 2u) ? -b0 : b0;
+}
+
+// Periodic wrap of one coordinate (Lattice::lmod, strict comparisons).
+__device__ __forceinline__ float wrap1(float v, float a) {
+    if (v > a) v = v - a;
+    if (v < 0.0f) v = v + a;
+    return v;
+}
+
+// Crossing of the segment s + t*u, t in [0, ell], with a circle (cylinder,
+// xy only) or sphere surface centred at o-offsets (ox, oy, oz) = s - centre.
+// The walker's side comes from its tracked compartment; the start point is
+// clamped onto that side, which replaces the reference's epsilon exclusion
+// of the root at the reflection point.
+// The walker is inside this image iff it is in the image's compartment and
+// within half a cell (H2 = squared half of the smallest cell side) of its
+// centre: other images of the same primitive are at least that far away.
+template <bool SPH>
+__device__ __forceinline__ bool hit_test(float ox, float oy, float oz, float ux, float uy,
+                                         float uz, float A, float R2, bool same_comp, float H2,
+                                         float ell, float& t_out, bool& ins_out) {
+    float B, q;
+    if (SPH) {
+        B = fmaf(oz, uz, fmaf(oy, uy, ox * ux));
+        q = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+    } else {
+        B = fmaf(oy, uy, ox * ux);
+        q = fmaf(oy, oy, ox * ox);
+    }
+    const float C = q - R2;
+    const bool inside = same_comp && q < H2;
+    ins_out = inside;
+    if (!(A > 0.0f)) return false;
+    if (inside) {
+        float Cp = C < 0.0f ? C : 0.0f;
+        float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
+        if (!(fe > 0.0f)) return false;
+        float disc = fmaf(B, B, -(A * Cp));
+        float sq = sqrtf(disc);
+        float t;
+        if (B < 0.0f) {
+            t = (sq - B) / A;
+        } else {
+            float den = B + sq;
+            t = den > 0.0f ? (-Cp) / den : 0.0f;
+        }
+        t_out = t < ell ? t : ell;
+        return true;
+    }
+    if (!(B < 0.0f)) return false;
+    float Cp = C > 0.0f ? C : 0.0f;
+    if (Cp == 0.0f) {
+        t_out = 0.0f;
+        return true;
+    }
+    float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
+    float disc = fmaf(B, B, -(A * Cp));
+    if (!(fe < 0.0f) && !((-B) < A * ell && disc > 0.0f)) return false;
+    float sq = sqrtf(disc > 0.0f ? disc : 0.0f);
+    float t = Cp / (sq - B);
+    t_out = t < ell ? t : ell;
+    return true;
+}
+
+}  // namespace axx

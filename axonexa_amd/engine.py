@@ -1,0 +1,203 @@
+"""ctypes binding of the HIP library ``libaxonexa.so`` (C ABI of
+``include/axonexa.h``).
+
+This is the product path: every call goes to the gfx950 kernels.  There is
+no CPU fallback — if the library is missing or no GPU is present, the calls
+raise.
+
+Reference interface mirrored: ``phaseAcquisitionStream<F>::
+runAcquisitionLattice`` (src/acquisition/phaseAcquisitionStream.cuh:126-190)
+as :func:`run_phase`; its parts (allocation/upload :134-158, walk kernel
+:163-169, reductions :171-183, download :185-189) as :class:`Plan`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaxonexa.so")
+_LIB = None
+
+
+class AxxError(RuntimeError):
+    def __init__(self, code: int, what: str, detail: str = ""):
+        self.code = code
+        msg = f"{what}: {abi.ERRORS.get(code, 'error')} ({code})"
+        if detail:
+            msg += f" — {detail}"
+        super().__init__(msg)
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.axx_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.axx_destroy.argtypes = [vp]
+    L.axx_strerror.argtypes = [C.c_int]
+    L.axx_strerror.restype = C.c_char_p
+    L.axx_last_error.argtypes = [vp]
+    L.axx_last_error.restype = C.c_char_p
+    L.axx_version.restype = C.c_char_p
+    S, Q, R, Res = (C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                    C.POINTER(abi.Result))
+    L.axx_run_phase.argtypes = [vp, S, Q, R, Res]
+    L.axx_plan_create.argtypes = [vp, S, Q, R, C.POINTER(vp)]
+    L.axx_plan_walk.argtypes = [vp, C.c_int, C.c_int]
+    L.axx_plan_reduce.argtypes = [vp]
+    L.axx_plan_results.argtypes = [vp, Res]
+    L.axx_plan_sync.argtypes = [vp]
+    L.axx_plan_get_state.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.axx_plan_set_state.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.axx_plan_time_walk.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.axx_plan_get_config.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.axx_plan_set_config.argtypes = [vp, C.c_int, C.c_int]
+    L.axx_plan_profile_count.argtypes = [vp]
+    L.axx_plan_destroy.argtypes = [vp]
+    _LIB = L
+    return L
+
+
+class Context:
+    """One device + stream (axx_create)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().axx_create(C.byref(self._h), int(device))
+        if rc != 0:
+            raise AxxError(rc, "axx_create")
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            detail = lib().axx_last_error(self._h).decode(errors="replace")
+            raise AxxError(rc, what, detail)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().axx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+_DEFAULT_CTX: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class Plan:
+    """Device-resident run: create -> walk (chunks) -> reduce -> results."""
+
+    def __init__(self, sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.sub, self.acq, self.run = sub, acq, run
+        self._s, self._q, self._r = sub.struct(), acq.struct(), run.struct()
+        self._h = C.c_void_p()
+        rc = lib().axx_plan_create(self.ctx.handle, C.byref(self._s), C.byref(self._q),
+                                   C.byref(self._r), C.byref(self._h))
+        self.ctx.check(rc, "axx_plan_create")
+
+    def walk(self, step_begin: int = 0, step_end: Optional[int] = None):
+        if step_end is None:
+            step_end = self.acq.n_steps
+        self.ctx.check(lib().axx_plan_walk(self._h, int(step_begin), int(step_end)), "axx_plan_walk")
+        return self
+
+    def time_walk(self, step_begin: int, step_end: int) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().axx_plan_time_walk(self._h, int(step_begin), int(step_end),
+                                                C.byref(ms)), "axx_plan_time_walk")
+        return ms.value
+
+    def reduce(self):
+        self.ctx.check(lib().axx_plan_reduce(self._h), "axx_plan_reduce")
+        return self
+
+    def sync(self):
+        self.ctx.check(lib().axx_plan_sync(self._h), "axx_plan_sync")
+
+    def results(self) -> abi.Signals:
+        res, bufs = abi.alloc_result(self.acq.n_waveforms)
+        self.ctx.check(lib().axx_plan_results(self._h, C.byref(res)), "axx_plan_results")
+        return abi.signals_from(res, bufs)
+
+    def get_state(self) -> np.ndarray:
+        st = np.zeros(abi.state_fields(self.acq.n_profiles) * self.run.n_walkers, np.uint32)
+        self.ctx.check(lib().axx_plan_get_state(self._h, st.ctypes.data_as(C.POINTER(C.c_uint32))),
+                       "axx_plan_get_state")
+        return st
+
+    def set_state(self, st: np.ndarray):
+        st = np.ascontiguousarray(st, dtype=np.uint32)
+        self.ctx.check(lib().axx_plan_set_state(self._h, st.ctypes.data_as(C.POINTER(C.c_uint32))),
+                       "axx_plan_set_state")
+
+    def config(self):
+        b, w, l = C.c_int(), C.c_int(), C.c_int()
+        bw = C.c_double()
+        self.ctx.check(lib().axx_plan_get_config(self._h, C.byref(b), C.byref(w), C.byref(bw),
+                                                 C.byref(l)), "axx_plan_get_config")
+        return {"block": b.value, "wpt": w.value, "bin_width": bw.value, "lds_bytes": l.value}
+
+    def set_config(self, block: int, wpt: int = 1):
+        self.ctx.check(lib().axx_plan_set_config(self._h, int(block), int(wpt)), "axx_plan_set_config")
+
+    def close(self):
+        if self._h:
+            lib().axx_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def run_phase(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
+              ctx: Optional[Context] = None) -> abi.Signals:
+    """One-shot run (axx_run_phase)."""
+    ctx = ctx or default_context()
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    res, bufs = abi.alloc_result(acq.n_waveforms)
+    ctx.check(lib().axx_run_phase(ctx.handle, C.byref(s), C.byref(q), C.byref(r), C.byref(res)),
+              "axx_run_phase")
+    return abi.signals_from(res, bufs)

@@ -1,0 +1,221 @@
+/*
+ * axonexa.h — C ABI of the MI355X random-walk diffusion-MRI engine.
+ *
+ * The reference exposes this path as C++ templates that a driver instantiates
+ * and runs in one call:
+ *   phaseAcquisitionStream<F>::runAcquisitionLattice(num, basis*, Lattice&, dt,
+ *                                                    blocks, threads, numOfSM)
+ *   (src/acquisition/phaseAcquisitionStream.cuh:126-190)
+ * which allocates device buffers, seeds the RNG (src/kernels/kernelSetup.cuh:1-5),
+ * runs updateWalkersLatticePhase (src/kernels/kernelLattice.cuh:1-104), reduces
+ * cos/sin of the phases (src/util/reduction.cuh:76-195) and copies Mx/My back
+ * into the acquisition (phaseAcquisitionStream.cuh:185-189).
+ *
+ * This header is that boundary restated as a C ABI: plain structs, plain
+ * pointers, negative error codes, no C++ or HIP types.  Every entry point
+ * below names the reference routine it replaces.  Host buffers are owned by
+ * the caller; device memory is owned by the context/plan.
+ *
+ * Units are the caller's (the reference uses mm, ms, T with GAMMA in
+ * rad/(ms*T)); the engine only needs them to be consistent.
+ */
+#ifndef AXONEXA_H
+#define AXONEXA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (the reference aborts via safe_cuda, src/util/misc.cuh:5-11;
+ *      this ABI returns instead) ---- */
+#define AXX_OK               0
+#define AXX_ERR_INVALID     -1  /* bad argument / descriptor                 */
+#define AXX_ERR_DEVICE      -2  /* HIP runtime error                         */
+#define AXX_ERR_NOMEM       -3  /* device or host allocation failed          */
+#define AXX_ERR_LIMIT       -4  /* size beyond a compiled limit              */
+#define AXX_ERR_GEOMETRY    -5  /* overlapping primitives / step >= half cell */
+#define AXX_ERR_INIT        -6  /* rejection initialisation did not finish   */
+#define AXX_ERR_NODEVICE    -7  /* no HIP device                              */
+#define AXX_ERR_STATE       -8  /* call out of order (e.g. results before run) */
+
+/* ---- primitive types (src/primitives/cylinderXY.cuh, Sphere.cuh) ---- */
+#define AXX_PRIM_CYLINDER_XY 0  /* infinite cylinder along z */
+#define AXX_PRIM_SPHERE      1
+
+/* ---- boundary policies (kernelLattice.cuh:64-79, boundaryCheck.cuh:34-113) ---- */
+#define AXX_BOUNDARY_SPECULAR  0  /* multi-reflection, impermeable (SPECULAR_REFLECTION)   */
+#define AXX_BOUNDARY_PERMEABLE 1  /* reflect or transmit with p = 4*kappa/v (USE_PERMEABLE) */
+#define AXX_BOUNDARY_REJECTION 2  /* stay put if the region changes (simple rejection)    */
+
+/* ---- initial placement (kernelLattice.cuh:29-37) ---- */
+#define AXX_INIT_UNIFORM      0  /* uniform in the cell (Lattice::initializeUniformly)          */
+#define AXX_INIT_COMPARTMENT  1  /* rejection until compartment == init_arg (initializeInRegion) */
+#define AXX_INIT_INSIDE_PRIM  2  /* rejection inside the bounding cube of primitive init_arg
+                                    (basis[k].randUnif, USE_INITIALIZE_IN_REGION = k+1)           */
+
+/* Limits compiled into the kernels. */
+#define AXX_MAX_PROFILES   64   /* distinct waveform time profiles per run      */
+#define AXX_MAX_PRIMS      32768
+
+/*
+ * Substrate: a periodic cell a x b x c (src/substrate/lattice.cuh:3-23) holding
+ * n_prims primitives (the reference's basis[] array).  Primitive k is the
+ * reference's Cylinder_XY(cx, cy, R, T2, T1, D, region, permeability) or
+ * Sphere(cx, cy, cz, R, ...).  Compartment ids: 0 = the lattice (extra-
+ * cellular) space, k+1 = inside primitive k.  Primitives must not overlap
+ * (under the minimum-image convention).  T2 <= 0 or +inf means no decay.
+ */
+typedef struct axx_substrate {
+    double a, b, c;          /* cell lengths                                    */
+    double D_lat;            /* diffusivity of the lattice space (length^2/time)*/
+    double T2_lat;           /* T2 of the lattice space (time)                   */
+    int n_prims;
+    const int*    type;      /* [n_prims] AXX_PRIM_*                             */
+    const double* center;    /* [n_prims][3] (z ignored for cylinders)           */
+    const double* radius;    /* [n_prims]                                        */
+    const double* D;         /* [n_prims] diffusivity inside                     */
+    const double* T2;        /* [n_prims] T2 inside                              */
+    const double* kappa;     /* [n_prims] membrane permeability (length/time); may be NULL */
+} axx_substrate;
+
+/*
+ * Acquisition: the reference's phaseAcquisition<F> (src/acquisition/
+ * phaseAcquisition.cuh:1-52) holds M gradient functors G_j(t) evaluated at
+ * t_i = i*dt.  Here each waveform j is amplitude_j * profile_{p(j)}(t_i) *
+ * direction_j, with the unit-amplitude profiles sampled once on the host
+ * (profiles[i*n_profiles + p], i in [0, n_steps)).  Phase of walker w:
+ *   phi_j = gamma*dt * sum_{i=0}^{n_steps-1} G_j(t_i) . r_unbounded_w(t_i)
+ * exactly as kernelLattice.cuh:38-44,81-85.
+ */
+typedef struct axx_acq {
+    int n_steps;             /* steps per walk (SimuParamsPhase::steps)          */
+    double dt;               /* time step                                        */
+    double gamma;            /* gyromagnetic ratio (GAMMA)                       */
+    int n_profiles;          /* 1 .. AXX_MAX_PROFILES                            */
+    const float* profiles;   /* [n_steps][n_profiles]                            */
+    int n_waveforms;         /* M                                                */
+    const int*    wf_profile;   /* [M] profile index                             */
+    const double* wf_amplitude; /* [M] gradient amplitude                        */
+    const double* wf_dir;       /* [M][3] gradient direction                     */
+} axx_acq;
+
+typedef struct axx_run {
+    int64_t  n_walkers;      /* N (any N >= 1; no multiple-of-block rule)        */
+    uint64_t seed;           /* Philox key                                       */
+    uint32_t instance;       /* independent stream selector (replicas, batch)    */
+    uint32_t walker_offset;  /* global id of walker 0 (sharding)                 */
+    int init_mode;           /* AXX_INIT_*                                       */
+    int init_arg;
+    int boundary_mode;       /* AXX_BOUNDARY_*                                   */
+    int relaxation;          /* 1: weight by exp(-sum dt/T2) (USE_RELAXATION)    */
+    int bounce_cap;          /* max boundary events per step (0 -> 256)          */
+} axx_run;
+
+/*
+ * Results: the reference returns raw Mx_j = sum_w cos(phi_j,w)*e^{-T2f_w} and
+ * My_j likewise (phaseAcquisitionStream.cuh:171-189).  Sums of squares give
+ * the Monte-Carlo standard error.  "intra" sums cover walkers that started
+ * inside a primitive (compartment != 0 at t = 0).  Any pointer may be NULL.
+ */
+typedef struct axx_result {
+    double* sum_cos;         /* [M] */
+    double* sum_sin;         /* [M] */
+    double* sum_cos2;        /* [M] */
+    double* sum_sin2;        /* [M] */
+    double* intra_cos;       /* [M] */
+    double* intra_sin;       /* [M] */
+    int64_t n_walkers;
+    int64_t n_intra;
+    uint64_t collisions;     /* boundary events (reflections + transmissions)  */
+    uint64_t transmissions;
+    uint64_t cap_hits;       /* steps truncated by bounce_cap                   */
+} axx_result;
+
+/*
+ * Walker state, structure-of-arrays, n_walkers entries per field (4 bytes
+ * each).  Exposed for parity checks and checkpoint/resume.  Field f of walker
+ * w lives at state[f*n_walkers + w].  Moments: AXX_F_MOM + 3*p + axis.
+ */
+#define AXX_F_X      0   /* wrapped position                                */
+#define AXX_F_Y      1
+#define AXX_F_Z      2
+#define AXX_F_DX     3   /* displacement since t=0 (r_unbounded - r0)       */
+#define AXX_F_DY     4
+#define AXX_F_DZ     5
+#define AXX_F_X0     6   /* initial position                                */
+#define AXX_F_Y0     7
+#define AXX_F_Z0     8
+#define AXX_F_E      9   /* accumulated sum dt/T2 up to the last compartment change */
+#define AXX_F_LF    10   /* fraction of step AXX_F_LI at the last compartment change */
+#define AXX_F_COMP  11   /* int: current compartment                        */
+#define AXX_F_COMP0 12   /* int: compartment at t = 0                       */
+#define AXX_F_LI    13   /* int: step index of the last compartment change  */
+#define AXX_F_NCOLL 14   /* uint: boundary events                           */
+#define AXX_F_NTRAN 15   /* uint: transmissions                             */
+#define AXX_F_NCAP  16   /* uint: bounce-cap truncations                    */
+#define AXX_F_STEP  17   /* int: next step index to run                     */
+#define AXX_F_MOM   18
+#define AXX_STATE_FIELDS(P) (AXX_F_MOM + 3 * (P))
+
+/* ---- context ---- */
+typedef struct axx_ctx axx_ctx;
+typedef struct axx_plan axx_plan;
+
+/* Select the device and create the context's stream.  One context per
+ * device; a context is used by one host thread at a time. */
+int  axx_create(axx_ctx** out, int device);
+void axx_destroy(axx_ctx* ctx);
+const char* axx_strerror(int code);
+/* Human-readable detail of the last error on this context (or "" ). */
+const char* axx_last_error(const axx_ctx* ctx);
+
+/*
+ * One-shot run: the whole of runAcquisitionLattice
+ * (phaseAcquisitionStream.cuh:126-190) — upload, walk, reduce, download.
+ * Synchronous.  Replaces runAcquisitionLattice / runAcquisition (single pore:
+ * a cell larger than the pore with AXX_INIT_INSIDE_PRIM).
+ */
+int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
+                  const axx_run* run, axx_result* out);
+
+/*
+ * Plan API: the same run split into its parts so that the device-resident
+ * walk can be timed, chunked (checkpoint/resume) and inspected.
+ *   create  : validate, build periodic images + cell bins, upload tables,
+ *             allocate walker state.             (phaseAcquisitionStream.cuh:134-158)
+ *   walk    : advance every walker over steps [step_begin, step_end); the
+ *             first call must start at 0 and initialises walkers.
+ *                                                  (kernelLattice.cuh:1-104)
+ *   reduce  : fp64 signal sums into device buffers  (reduction.cuh:76-195)
+ *   results : copy sums to the host (synchronises)  (phaseAcquisitionStream.cuh:185-189)
+ * walk/reduce are asynchronous on the context stream.
+ */
+int  axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
+                     const axx_run* run, axx_plan** out);
+int  axx_plan_walk(axx_plan* plan, int step_begin, int step_end);
+int  axx_plan_reduce(axx_plan* plan);
+int  axx_plan_results(axx_plan* plan, axx_result* out);
+int  axx_plan_sync(axx_plan* plan);
+/* Copy walker state to / from the host (n_walkers * AXX_STATE_FIELDS(P) words). */
+int  axx_plan_get_state(axx_plan* plan, uint32_t* host_state);
+int  axx_plan_set_state(axx_plan* plan, const uint32_t* host_state);
+/* Time a walk over [step_begin, step_end) with HIP events on the plan's stream. */
+int  axx_plan_time_walk(axx_plan* plan, int step_begin, int step_end, float* ms);
+/* Launch configuration the plan uses (threads per block, walkers per thread,
+ * bin width, LDS bytes) and a setter used by the autotuner. */
+int  axx_plan_get_config(const axx_plan* plan, int* block, int* wpt,
+                         double* bin_width, int* lds_bytes);
+int  axx_plan_set_config(axx_plan* plan, int block, int wpt);
+int  axx_plan_profile_count(const axx_plan* plan);
+void axx_plan_destroy(axx_plan* plan);
+
+/* Library version string. */
+const char* axx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXONEXA_H */

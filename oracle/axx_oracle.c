@@ -1,0 +1,801 @@
+/*
+ * axx_oracle.c — CPU restatement of axonexa's random-walk phase simulator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+ * check and bench.py's CPU-baseline leg may call it; the product library
+ * (axonexa_amd/) never links or calls it.
+ *
+ * It restates, in plain C and fp32 arithmetic written out operation by
+ * operation (built with -ffp-contract=off), the semantics of the reference's
+ * lattice walk:
+ *   kernelLattice.cuh:1-104           per-walker loop (init, step, phase, T2)
+ *   boundaryCheck.cuh:138-217         multi-reflection / permeable boundary
+ *   lattice.cuh:31-247                periodic cell, wrap (lmod), region lookup
+ *   cylinderXY.cuh:69-131, Sphere.cuh:89-149   inside / intersect / normal
+ *   kernelSetup.cuh:1-5               per-walker RNG (replaced: Philox4x32-10)
+ *   phaseAcquisitionStream.cuh:171-189 + reduction.cuh:76-195   signal sums
+ * with the deliberate, documented deviations of the build (counter-based
+ * Philox, sphere-point picking without acos, segment form of the reflection
+ * loop, side-tracked crossing test, true T2 time split, continued boundary
+ * checks after a transmission).  Every function cites the reference lines it
+ * follows.
+ *
+ * Pins (what checks this file, tests/): Philox known-answer vectors
+ * (Random123) and rocRAND's engine; closed-form ray/circle and ray/sphere
+ * roots in fp64; the exact free-diffusion sinc product; exp(-bD); Neuman /
+ * GPD restricted limits; invariants (step length, reflection isometry,
+ * lattice translation, uniform equilibrium under permeability 4*kappa/v).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/axonexa.h"
+#include "axx_oracle.h"
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Replaces cuRAND
+ * XORWOW (kernelSetup.cuh:1-5, curand_uniform at kernelLattice.cuh:52-53). */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * (uint64_t)c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Uniform in (0,1): odd multiples of 2^-24, exactly representable. */
+static inline float unif(uint32_t w) { return (float)((w >> 8) | 1u) * 0x1p-24f; }
+float orc_unif(uint32_t w) { return unif(w); }
+
+/* Counter streams: ctr = (walker, index, stream<<24 | sub, instance). */
+#define STREAM_STEP 0u
+#define STREAM_COLL 1u
+#define STREAM_INIT 2u
+
+static inline void draw(uint32_t wid, uint32_t idx, uint32_t stream_sub, uint32_t inst,
+                        const uint32_t key[2], uint32_t out[4]) {
+    uint32_t ctr[4] = {wid, idx, stream_sub, inst};
+    orc_philox4x32_10(ctr, key, out);
+}
+
+/* ------------------------------------------------------------------ */
+/* sin/cos of 2*pi*u from the Philox word directly.  Quarter-turn range
+ * reduction in integers, then Taylor polynomials of sin(pi/2 r) and
+ * cos(pi/2 r) on r in [-1/2, 1/2] (truncation < 2e-9).  Replaces
+ * phi = 2*PI*curand_uniform; sin(phi), cos(phi) (kernelLattice.cuh:52-57). */
+#define S1  1.5707963267948966f
+#define S3 -0.6459640975062462f
+#define S5  0.07969262624616704f
+#define S7 -0.004681754135318687f
+#define S9  0.00016044118478735982f
+#define C2 -1.2337005501361697f
+#define C4  0.25366950790104802f
+#define C6 -0.020863480763352960f
+#define C8  0.00091926027483942659f
+#define C10 -0.000025202042373060605f
+
+void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
+    uint32_t v = (w >> 8) | 1u;                  /* 24-bit odd: 4u = v * 2^-22 */
+    uint32_t q = (v + (1u << 21)) >> 22;         /* nearest quarter 0..4       */
+    int32_t ri = (int32_t)v - (int32_t)(q << 22);
+    float r = (float)ri * 0x1p-22f;              /* exact, in [-1/2, 1/2]      */
+    float y = r * r;
+    float ps = fmaf(y, S9, S7);
+    ps = fmaf(y, ps, S5);
+    ps = fmaf(y, ps, S3);
+    ps = fmaf(y, ps, S1);
+    float sp = r * ps;
+    float pc = fmaf(y, C10, C8);
+    pc = fmaf(y, pc, C6);
+    pc = fmaf(y, pc, C4);
+    pc = fmaf(y, pc, C2);
+    float cp = fmaf(y, pc, 1.0f);
+    switch (q & 3u) {
+    case 0: *c_out = cp;  *s_out = sp;  break;
+    case 1: *c_out = -sp; *s_out = cp;  break;
+    case 2: *c_out = -cp; *s_out = -sp; break;
+    default: *c_out = sp; *s_out = -cp; break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Prepared substrate: primitives with their periodic images (what the
+ * reference's two-image test plus correctEdges copies cover,
+ * lattice.cuh:205-247 and RPSinitializer.h:131-186), per-compartment step
+ * lengths and T2 rates, per-primitive transmission thresholds. */
+typedef struct {
+    float cx, cy, cz, R2;
+    int type;    /* AXX_PRIM_* */
+    int comp;    /* compartment id = original index + 1 */
+    int orig;
+} oprim;
+
+typedef struct {
+    float a, b, c;
+    int n_img;
+    oprim* img;
+    int n_comp;
+    float* L;      /* [n_comp] step length sqrt(6 D dt) */
+    float* rate;   /* [n_comp] dt / T2 (0 if no decay)   */
+    float* thr_out;/* [n_prims] 4 kappa / v_inside          */
+    float* thr_in; /* [n_prims] 4 kappa / v_lattice         */
+    float Lmax;
+    float H2c, H2s; /* squared half of the smallest cell side (xy; xyz) */
+    /* optional cell bins (binned variant of the scan) */
+    int nb[3];
+    float binw_inv[3];
+    int* bin_off;  /* [nbins+1] */
+    int* bin_idx;
+} oprep;
+
+static int prim_reach_hits_box(const axx_substrate* s, int k, double sx, double sy, double sz,
+                               double lo[3], double hi[3], double reach) {
+    /* Does primitive k shifted by (sx,sy,sz), grown by `reach`, touch the box? */
+    double c[3] = {s->center[3 * k] + sx, s->center[3 * k + 1] + sy, s->center[3 * k + 2] + sz};
+    double R = s->radius[k] + reach;
+    int dims = s->type[k] == AXX_PRIM_SPHERE ? 3 : 2;
+    double d2 = 0.0;
+    for (int a = 0; a < dims; ++a) {
+        double d = 0.0;
+        if (c[a] < lo[a]) d = lo[a] - c[a];
+        else if (c[a] > hi[a]) d = c[a] - hi[a];
+        d2 += d * d;
+    }
+    return d2 <= R * R;
+}
+
+/* Boundary reachable from the box: the circle/sphere surface comes within
+ * `reach` of some point of the box. */
+static int prim_surface_near_box(const oprim* p, double lo[3], double hi[3], double reach) {
+    double c[3] = {p->cx, p->cy, p->cz};
+    double R = sqrt((double)p->R2);
+    int dims = p->type == AXX_PRIM_SPHERE ? 3 : 2;
+    double dmin2 = 0.0, dmax2 = 0.0;
+    for (int a = 0; a < dims; ++a) {
+        double d = 0.0;
+        if (c[a] < lo[a]) d = lo[a] - c[a];
+        else if (c[a] > hi[a]) d = c[a] - hi[a];
+        dmin2 += d * d;
+        double e = fmax(fabs(c[a] - lo[a]), fabs(c[a] - hi[a]));
+        dmax2 += e * e;
+    }
+    double dmin = sqrt(dmin2), dmax = sqrt(dmax2);
+    return dmin <= R + reach && dmax >= R - reach;
+}
+
+static void oprep_free(oprep* P) {
+    free(P->img); free(P->L); free(P->rate); free(P->thr_out); free(P->thr_in);
+    free(P->bin_off); free(P->bin_idx);
+    memset(P, 0, sizeof(*P));
+}
+
+int orc_validate(const axx_substrate* s, const axx_acq* q) {
+    if (!s || !q) return AXX_ERR_INVALID;
+    if (!(s->a > 0 && s->b > 0 && s->c > 0)) return AXX_ERR_INVALID;
+    if (s->n_prims < 0 || s->n_prims > AXX_MAX_PRIMS) return AXX_ERR_LIMIT;
+    if (!(q->dt > 0) || q->n_steps < 1) return AXX_ERR_INVALID;
+    if (q->n_profiles < 1 || q->n_profiles > AXX_MAX_PROFILES) return AXX_ERR_LIMIT;
+    if (!(s->D_lat >= 0)) return AXX_ERR_INVALID;
+    double Lmax = sqrt(6.0 * s->D_lat * q->dt);
+    for (int k = 0; k < s->n_prims; ++k) {
+        if (s->type[k] != AXX_PRIM_CYLINDER_XY && s->type[k] != AXX_PRIM_SPHERE) return AXX_ERR_INVALID;
+        if (!(s->radius[k] > 0) || !(s->D[k] >= 0)) return AXX_ERR_INVALID;
+        double R = s->radius[k];
+        if (2 * R >= s->a || 2 * R >= s->b || (s->type[k] == AXX_PRIM_SPHERE && 2 * R >= s->c))
+            return AXX_ERR_GEOMETRY;
+        double L = sqrt(6.0 * s->D[k] * q->dt);
+        if (L > Lmax) Lmax = L;
+    }
+    if (2 * Lmax >= s->a || 2 * Lmax >= s->b || 2 * Lmax >= s->c) return AXX_ERR_GEOMETRY;
+    /* non-overlap under the minimum image convention (fixes D8,
+     * RPSinitializer.h:191-219) */
+    for (int i = 0; i < s->n_prims; ++i)
+        for (int j = i + 1; j < s->n_prims; ++j) {
+            double d[3];
+            int dims = (s->type[i] == AXX_PRIM_SPHERE && s->type[j] == AXX_PRIM_SPHERE) ? 3 : 2;
+            double L3[3] = {s->a, s->b, s->c};
+            double d2 = 0;
+            for (int a = 0; a < dims; ++a) {
+                d[a] = s->center[3 * i + a] - s->center[3 * j + a];
+                d[a] -= L3[a] * nearbyint(d[a] / L3[a]);
+                d2 += d[a] * d[a];
+            }
+            double rr = s->radius[i] + s->radius[j];
+            if (d2 < rr * rr) return AXX_ERR_GEOMETRY;
+        }
+    return AXX_OK;
+}
+
+static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width) {
+    memset(P, 0, sizeof(*P));
+    int rc = orc_validate(s, q);
+    if (rc) return rc;
+    P->a = (float)s->a; P->b = (float)s->b; P->c = (float)s->c;
+    {
+        double hc = 0.5 * fmin(s->a, s->b), hs = 0.5 * fmin(fmin(s->a, s->b), s->c);
+        P->H2c = (float)(hc * hc);
+        P->H2s = (float)(hs * hs);
+    }
+    int n = s->n_prims;
+    P->n_comp = n + 1;
+    P->L = malloc(sizeof(float) * (n + 1));
+    P->rate = malloc(sizeof(float) * (n + 1));
+    P->thr_out = malloc(sizeof(float) * (n + 1));
+    P->thr_in = malloc(sizeof(float) * (n + 1));
+    double dt = q->dt;
+    double Lmax = sqrt(6.0 * s->D_lat * dt);
+    P->L[0] = (float)sqrt(6.0 * s->D_lat * dt);
+    P->rate[0] = (s->T2_lat > 0 && isfinite(s->T2_lat)) ? (float)(dt / s->T2_lat) : 0.0f;
+    double v_lat = sqrt(6.0 * s->D_lat / dt);
+    for (int k = 0; k < n; ++k) {
+        double L = sqrt(6.0 * s->D[k] * dt);
+        if (L > Lmax) Lmax = L;
+        P->L[k + 1] = (float)L;
+        P->rate[k + 1] = (s->T2[k] > 0 && isfinite(s->T2[k])) ? (float)(dt / s->T2[k]) : 0.0f;
+        double kap = s->kappa ? s->kappa[k] : 0.0;
+        double v_in = sqrt(6.0 * s->D[k] / dt);
+        P->thr_out[k] = kap > 0 ? (float)(4.0 * kap / v_in) : 0.0f;
+        P->thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
+    }
+    P->Lmax = (float)Lmax;
+    /* periodic images: every shift in {-1,0,1}^2 (x{-1,0,1} for spheres)
+     * whose grown shape touches the cell, in (original, shift) order */
+    double reach = Lmax * 1.001 + 1e-6 * fmax(s->a, s->b);
+    double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
+    int cap = n * 27 + 1;
+    P->img = malloc(sizeof(oprim) * cap);
+    int m = 0;
+    for (int k = 0; k < n; ++k) {
+        int zr = s->type[k] == AXX_PRIM_SPHERE ? 1 : 0;
+        for (int kz = -zr; kz <= zr; ++kz)
+            for (int ky = -1; ky <= 1; ++ky)
+                for (int kx = -1; kx <= 1; ++kx) {
+                    double sx = kx * s->a, sy = ky * s->b, sz = kz * s->c;
+                    if (!prim_reach_hits_box(s, k, sx, sy, sz, lo, hi, reach)) continue;
+                    oprim* p = &P->img[m++];
+                    p->cx = (float)(s->center[3 * k] + sx);
+                    p->cy = (float)(s->center[3 * k + 1] + sy);
+                    p->cz = (float)(s->center[3 * k + 2] + sz);
+                    float Rf = (float)s->radius[k];
+                    p->R2 = Rf * Rf;
+                    p->type = s->type[k];
+                    p->comp = k + 1;
+                    p->orig = k;
+                }
+    }
+    P->n_img = m;
+    /* cell bins (binned variant): surface within reach of the bin */
+    if (bin_width > 0) {
+        int any_sphere = 0;
+        for (int k = 0; k < n; ++k) any_sphere |= (s->type[k] == AXX_PRIM_SPHERE);
+        double Ls[3] = {s->a, s->b, s->c};
+        for (int a = 0; a < 3; ++a) {
+            int nb = (int)floor(Ls[a] / bin_width);
+            if (nb < 1) nb = 1;
+            if (nb > 1024) nb = 1024;
+            if (a == 2 && !any_sphere) nb = 1;
+            P->nb[a] = nb;
+            P->binw_inv[a] = (float)(nb / Ls[a]);
+        }
+        int nbins = P->nb[0] * P->nb[1] * P->nb[2];
+        P->bin_off = calloc((size_t)nbins + 1, sizeof(int));
+        /* two passes over (image, touched bin): count, then fill in image order */
+        int* fillp = calloc((size_t)nbins + 1, sizeof(int));
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = 0; i < m; ++i) {
+                const oprim* pi = &P->img[i];
+                double R = sqrt((double)pi->R2) + reach;
+                double c3[3] = {pi->cx, pi->cy, pi->cz};
+                int lo_b[3], hi_b[3];
+                for (int a = 0; a < 3; ++a) {
+                    if (a == 2 && pi->type != AXX_PRIM_SPHERE) { lo_b[a] = 0; hi_b[a] = P->nb[a] - 1; continue; }
+                    double bw = Ls[a] / P->nb[a];
+                    lo_b[a] = (int)floor((c3[a] - R) / bw) - 1;
+                    hi_b[a] = (int)floor((c3[a] + R) / bw) + 1;
+                    if (lo_b[a] < 0) lo_b[a] = 0;
+                    if (hi_b[a] > P->nb[a] - 1) hi_b[a] = P->nb[a] - 1;
+                }
+                for (int bz = lo_b[2]; bz <= hi_b[2]; ++bz)
+                    for (int by = lo_b[1]; by <= hi_b[1]; ++by)
+                        for (int bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
+                            double blo[3] = {bx * s->a / P->nb[0], by * s->b / P->nb[1], bz * s->c / P->nb[2]};
+                            double bhi[3] = {(bx + 1) * s->a / P->nb[0], (by + 1) * s->b / P->nb[1],
+                                             (bz + 1) * s->c / P->nb[2]};
+                            if (!prim_surface_near_box(pi, blo, bhi, reach)) continue;
+                            int b = (bz * P->nb[1] + by) * P->nb[0] + bx;
+                            if (pass == 0) P->bin_off[b + 1]++;
+                            else P->bin_idx[fillp[b]++] = i;
+                        }
+            }
+            if (pass == 0) {
+                for (int b = 0; b < nbins; ++b) P->bin_off[b + 1] += P->bin_off[b];
+                P->bin_idx = malloc(sizeof(int) * ((size_t)P->bin_off[nbins] + 1));
+                memcpy(fillp, P->bin_off, sizeof(int) * (size_t)nbins);
+            }
+        }
+        free(fillp);
+    }
+    return AXX_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Geometry predicates (cylinderXY.cuh:69-80, Sphere.cuh:89-94). */
+static inline int inside_at(const oprim* p, float x, float y, float z) {
+    float ox = x - p->cx, oy = y - p->cy;
+    if (p->type == AXX_PRIM_SPHERE) {
+        float oz = z - p->cz;
+        return fmaf(oz, oz, fmaf(oy, oy, ox * ox)) < p->R2;
+    }
+    return fmaf(oy, oy, ox * ox) < p->R2;
+}
+
+/* Region of a point (lattice.cuh:134-148): compartment of the primitive that
+ * contains it, else the lattice (0).  Non-overlap makes the match unique. */
+static int region_of(const oprep* P, float x, float y, float z) {
+    for (int i = 0; i < P->n_img; ++i)
+        if (inside_at(&P->img[i], x, y, z)) return P->img[i].comp;
+    return 0;
+}
+
+/* Crossing test of the segment s + t*u, t in [0, ell], against one surface
+ * (cylinderXY.cuh:82-122, Sphere.cuh:102-139 restated).  The walker's side
+ * of the surface is taken from its tracked compartment, and the start point
+ * is clamped onto that side (C' = min(C,0) inside, max(C,0) outside), which
+ * replaces the reference's epsilon test (real_equal(root*|dr|, 0, eps)). */
+static inline int hit_test(const oprim* p, int same_comp, float H2c, float H2s, float sx,
+                           float sy, float sz, float ux, float uy, float uz, float Axy, float A3,
+                           float ell, float* t_out, int* ins_out) {
+    float ox = sx - p->cx, oy = sy - p->cy;
+    float A, B, q, H2;
+    if (p->type == AXX_PRIM_SPHERE) {
+        float oz = sz - p->cz;
+        A = A3;
+        B = fmaf(oz, uz, fmaf(oy, uy, ox * ux));
+        q = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+        H2 = H2s;
+    } else {
+        A = Axy;
+        B = fmaf(oy, uy, ox * ux);
+        q = fmaf(oy, oy, ox * ox);
+        H2 = H2c;
+    }
+    float C = q - p->R2;
+    /* the walker is inside this image iff it is in the image's compartment
+     * and within half a cell of its centre (other images of the same
+     * primitive are at least half a cell away) */
+    int inside = same_comp && q < H2;
+    *ins_out = inside;
+    if (!(A > 0.0f)) return 0;
+    if (inside) {
+        float Cp = C < 0.0f ? C : 0.0f;
+        float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
+        if (!(fe > 0.0f)) return 0;
+        float disc = fmaf(B, B, -(A * Cp));
+        float sq = sqrtf(disc);
+        float t;
+        if (B < 0.0f) {
+            t = (sq - B) / A;
+        } else {
+            float den = B + sq;
+            t = den > 0.0f ? (-Cp) / den : 0.0f;
+        }
+        *t_out = t < ell ? t : ell;
+        return 1;
+    }
+    if (!(B < 0.0f)) return 0;
+    float Cp = C > 0.0f ? C : 0.0f;
+    if (Cp == 0.0f) {
+        *t_out = 0.0f;
+        return 1;
+    }
+    float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
+    float disc = fmaf(B, B, -(A * Cp));
+    if (!(fe < 0.0f) && !((-B) < A * ell && disc > 0.0f)) return 0;
+    float sq = sqrtf(disc > 0.0f ? disc : 0.0f);
+    float t = Cp / (sq - B);
+    *t_out = t < ell ? t : ell;
+    return 1;
+}
+
+/* lmod (lattice.cuh:66-88): strict comparisons, one cell at most. */
+static inline float wrap1(float v, float a) {
+    if (v > a) v = v - a;
+    if (v < 0.0f) v = v + a;
+    return v;
+}
+
+static inline void cand_range(const oprep* P, int use_bins, float x, float y, float z,
+                              const int** list, int* cnt, int* dense) {
+    if (!use_bins) {
+        *list = NULL;
+        *cnt = P->n_img;
+        *dense = 1;
+        return;
+    }
+    int bx = (int)(x * P->binw_inv[0]);
+    int by = (int)(y * P->binw_inv[1]);
+    int bz = (int)(z * P->binw_inv[2]);
+    if (bx > P->nb[0] - 1) bx = P->nb[0] - 1;
+    if (by > P->nb[1] - 1) by = P->nb[1] - 1;
+    if (bz > P->nb[2] - 1) bz = P->nb[2] - 1;
+    if (bx < 0) bx = 0;
+    if (by < 0) by = 0;
+    if (bz < 0) bz = 0;
+    int b = (bz * P->nb[1] + by) * P->nb[0] + bx;
+    *list = P->bin_idx + P->bin_off[b];
+    *cnt = P->bin_off[b + 1] - P->bin_off[b];
+    *dense = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* One walker over steps [i0, i1) (kernelLattice.cuh:46-99). */
+typedef struct {
+    const oprep* P;
+    const axx_acq* q;
+    const axx_run* run;
+    uint32_t* st;     /* SoA state */
+    int64_t N;
+    int Pn;           /* profiles */
+    int axes;         /* bit mask of axes carrying phase */
+    int i0, i1;
+    int use_bins;
+    int64_t w_begin, w_end;
+    int err;
+} job_t;
+
+#define FS(f) (((float*)J->st) + (size_t)(f) * J->N)
+#define IS(f) (((int32_t*)J->st) + (size_t)(f) * J->N)
+
+static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t key[2],
+                       float* x, float* y, float* z, int* comp) {
+    const oprep* P = J->P;
+    const axx_run* R = J->run;
+    uint32_t inst = R->instance;
+    uint32_t o[4];
+    if (R->init_mode == AXX_INIT_UNIFORM) {
+        draw(wid, 0, STREAM_INIT << 24, inst, key, o);
+        *x = unif(o[0]) * P->a;
+        *y = unif(o[1]) * P->b;
+        *z = unif(o[2]) * P->c;
+        *comp = region_of(P, *x, *y, *z);
+        return 0;
+    }
+    for (uint32_t it = 0; it < (1u << 20); ++it) {
+        draw(wid, it, STREAM_INIT << 24, inst, key, o);
+        float px, py, pz;
+        if (R->init_mode == AXX_INIT_COMPARTMENT) {
+            px = unif(o[0]) * P->a;
+            py = unif(o[1]) * P->b;
+            pz = unif(o[2]) * P->c;
+            int cm = region_of(P, px, py, pz);
+            if (cm != R->init_arg) continue;
+            *x = px; *y = py; *z = pz; *comp = cm;
+            return 0;
+        } else { /* inside primitive k: its bounding cube (cylinderXY.cuh:176-183) */
+            int k = R->init_arg;
+            const oprim* pk = NULL;
+            for (int i = 0; i < P->n_img; ++i)
+                if (P->img[i].orig == k) { pk = &P->img[i]; break; }
+            if (!pk) return AXX_ERR_INVALID;
+            float Rf = sqrtf(pk->R2);
+            px = pk->cx + (unif(o[0]) * 2.0f - 1.0f) * Rf;
+            py = pk->cy + (unif(o[1]) * 2.0f - 1.0f) * Rf;
+            pz = pk->cz + (unif(o[2]) * 2.0f - 1.0f) * Rf;
+            if (!inside_at(pk, px, py, pz)) continue;
+            px = wrap1(px, P->a); py = wrap1(py, P->b); pz = wrap1(pz, P->c);
+            *x = px; *y = py; *z = pz;
+            *comp = region_of(P, px, py, pz);
+            return 0;
+        }
+    }
+    (void)w;
+    return AXX_ERR_INIT;
+}
+
+static void* job_run(void* arg) {
+    job_t* J = (job_t*)arg;
+    const oprep* P = J->P;
+    const axx_acq* q = J->q;
+    const axx_run* R = J->run;
+    const int Pn = J->Pn;
+    const int mode = R->boundary_mode;
+    const int cap = R->bounce_cap > 0 ? R->bounce_cap : 256;
+    const uint32_t key[2] = {(uint32_t)R->seed, (uint32_t)(R->seed >> 32)};
+    const uint32_t inst = R->instance;
+    float m[AXX_MAX_PROFILES][3];
+
+    for (int64_t w = J->w_begin; w < J->w_end; ++w) {
+        uint32_t wid = R->walker_offset + (uint32_t)w;
+        float x, y, z, dx, dy, dz, E, lf;
+        int comp, li;
+        uint32_t ncoll, ntran, ncap;
+        if (J->i0 == 0) {
+            int rc = init_walker(J, w, wid, key, &x, &y, &z, &comp);
+            if (rc) { J->err = rc; return NULL; }
+            dx = dy = dz = 0.0f;
+            E = 0.0f; lf = 0.0f; li = 1;
+            ncoll = ntran = ncap = 0;
+            memset(m, 0, sizeof(m));
+            FS(AXX_F_X0)[w] = x; FS(AXX_F_Y0)[w] = y; FS(AXX_F_Z0)[w] = z;
+            IS(AXX_F_COMP0)[w] = comp;
+        } else {
+            x = FS(AXX_F_X)[w]; y = FS(AXX_F_Y)[w]; z = FS(AXX_F_Z)[w];
+            dx = FS(AXX_F_DX)[w]; dy = FS(AXX_F_DY)[w]; dz = FS(AXX_F_DZ)[w];
+            E = FS(AXX_F_E)[w]; lf = FS(AXX_F_LF)[w];
+            comp = IS(AXX_F_COMP)[w]; li = IS(AXX_F_LI)[w];
+            ncoll = (uint32_t)IS(AXX_F_NCOLL)[w]; ntran = (uint32_t)IS(AXX_F_NTRAN)[w];
+            ncap = (uint32_t)IS(AXX_F_NCAP)[w];
+            for (int p = 0; p < Pn; ++p)
+                for (int a = 0; a < 3; ++a) m[p][a] = FS(AXX_F_MOM + 3 * p + a)[w];
+        }
+        int ibeg = J->i0 < 1 ? 1 : J->i0;
+        uint32_t o[4] = {0, 0, 0, 0};
+        for (int i = ibeg; i < J->i1; ++i) {
+            /* draws: one Philox call per pair of steps (i-1)>>1 */
+            uint32_t h = (uint32_t)(i - 1) & 1u;
+            if (h == 0 || i == ibeg) draw(wid, (uint32_t)(i - 1) >> 1, STREAM_STEP << 24, inst, key, o);
+            uint32_t wa = o[2 * h], wb = o[2 * h + 1];
+            float c = unif(wb) * 2.0f - 1.0f;
+            float sth = sqrtf((1.0f - c) * (1.0f + c));
+            float sphi, cphi;
+            orc_sincos_turn(wa, &sphi, &cphi);
+            float ux = sth * cphi, uy = sth * sphi, uz = c;
+            float ell = P->L[comp];
+            float sx = x, sy = y, sz = z;
+            const int* list;
+            int cnt, dense;
+            cand_range(P, J->use_bins, x, y, z, &list, &cnt, &dense);
+            if (mode == AXX_BOUNDARY_REJECTION) {
+                float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
+                int rej = 0;
+                for (int j = 0; j < cnt; ++j) {
+                    const oprim* pk = &P->img[dense ? j : list[j]];
+                    if (inside_at(pk, sx, sy, sz) != inside_at(pk, ex, ey, ez)) { rej = 1; break; }
+                }
+                if (rej) { ncoll++; ell = 0.0f; }
+            } else {
+                float Axy = fmaf(uy, uy, ux * ux);
+                float A3 = fmaf(uz, uz, Axy);
+                int nb = 0, kc = 0;
+                for (;;) {
+                    float best = INFINITY;
+                    int bk = -1, bins_ = 0;
+                    for (int j = 0; j < cnt; ++j) {
+                        int k = dense ? j : list[j];
+                        const oprim* pk = &P->img[k];
+                        float t;
+                        int ins;
+                        if (hit_test(pk, comp == pk->comp, P->H2c, P->H2s, sx, sy, sz, ux, uy, uz,
+                                     Axy, A3, ell, &t, &ins) &&
+                            t < best) {
+                            best = t;
+                            bk = k;
+                            bins_ = ins;
+                        }
+                    }
+                    if (bk < 0) break;
+                    const oprim* pk = &P->img[bk];
+                    float t = best;
+                    ncoll++;
+                    float px = fmaf(t, ux, sx), py = fmaf(t, uy, sy), pz = fmaf(t, uz, sz);
+                    int inside = bins_;
+                    int transmit = 0;
+                    if (mode == AXX_BOUNDARY_PERMEABLE) {
+                        uint32_t oc[4];
+                        draw(wid, (uint32_t)i, (STREAM_COLL << 24) | ((uint32_t)kc >> 2), inst, key, oc);
+                        float U = unif(oc[kc & 3]);
+                        float thr = inside ? P->thr_out[pk->orig] : P->thr_in[pk->orig];
+                        transmit = U <= thr;
+                        kc++;
+                    }
+                    if (transmit) {
+                        int nc = inside ? 0 : pk->comp;
+                        float rem = ell - t;
+                        float f = rem / P->L[comp];
+                        float ph = 1.0f - f;
+                        E = fmaf((float)(i - li) + (ph - lf), P->rate[comp], E);
+                        li = i;
+                        lf = ph;
+                        comp = nc;
+                        ell = f * P->L[nc];
+                        ntran++;
+                    } else {
+                        float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
+                        if (pk->type == AXX_PRIM_SPHERE) {
+                            nz = pz - pk->cz;
+                            float inv = 1.0f / sqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
+                            nx = nx * inv; ny = ny * inv; nz = nz * inv;
+                            kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
+                        } else {
+                            float inv = 1.0f / sqrtf(fmaf(ny, ny, nx * nx));
+                            nx = nx * inv; ny = ny * inv;
+                            kd = fmaf(ny, uy, nx * ux);
+                        }
+                        float k2 = kd + kd;
+                        ux = fmaf(-k2, nx, ux);
+                        uy = fmaf(-k2, ny, uy);
+                        uz = fmaf(-k2, nz, uz);
+                        Axy = fmaf(uy, uy, ux * ux);
+                        A3 = fmaf(uz, uz, Axy);
+                        ell = ell - t;
+                    }
+                    sx = px; sy = py; sz = pz;
+                    if (++nb >= cap) { ncap++; ell = 0.0f; break; }
+                }
+            }
+            float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
+            dx = dx + (ex - x);
+            dy = dy + (ey - y);
+            dz = dz + (ez - z);
+            x = wrap1(ex, P->a);
+            y = wrap1(ey, P->b);
+            z = wrap1(ez, P->c);
+            /* phase moments (kernelLattice.cuh:81-85, factored by profile) */
+            const float* g = q->profiles + (size_t)i * Pn;
+            for (int p = 0; p < Pn; ++p) {
+                float gp = g[p];
+                if (J->axes & 1) m[p][0] = fmaf(gp, dx, m[p][0]);
+                if (J->axes & 2) m[p][1] = fmaf(gp, dy, m[p][1]);
+                if (J->axes & 4) m[p][2] = fmaf(gp, dz, m[p][2]);
+            }
+        }
+        FS(AXX_F_X)[w] = x; FS(AXX_F_Y)[w] = y; FS(AXX_F_Z)[w] = z;
+        FS(AXX_F_DX)[w] = dx; FS(AXX_F_DY)[w] = dy; FS(AXX_F_DZ)[w] = dz;
+        FS(AXX_F_E)[w] = E; FS(AXX_F_LF)[w] = lf;
+        IS(AXX_F_COMP)[w] = comp; IS(AXX_F_LI)[w] = li;
+        IS(AXX_F_NCOLL)[w] = (int32_t)ncoll; IS(AXX_F_NTRAN)[w] = (int32_t)ntran;
+        IS(AXX_F_NCAP)[w] = (int32_t)ncap;
+        IS(AXX_F_STEP)[w] = J->i1;
+        for (int p = 0; p < Pn; ++p)
+            for (int a = 0; a < 3; ++a) FS(AXX_F_MOM + 3 * p + a)[w] = m[p][a];
+    }
+    return NULL;
+}
+
+/* Axes that carry phase, widened to the smallest of {x}, {x,y}, {x,z},
+ * {x,y,z} that contains them (the set of moment axes the walk keeps). */
+int orc_axes_mask(const axx_acq* q) {
+    int mask = 0;
+    for (int j = 0; j < q->n_waveforms; ++j)
+        for (int a = 0; a < 3; ++a)
+            if (q->wf_dir[3 * j + a] != 0.0 && q->wf_amplitude[j] != 0.0) mask |= 1 << a;
+    static const int widen[8] = {1, 1, 3, 3, 5, 5, 7, 7};
+    return widen[mask];
+}
+
+int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+             int step_end, uint32_t* state, int nthreads, double bin_width) {
+    if (!s || !q || !run || !state) return AXX_ERR_INVALID;
+    if (run->n_walkers < 1) return AXX_ERR_INVALID;
+    if (step_begin < 0 || step_end > q->n_steps || step_begin > step_end) return AXX_ERR_INVALID;
+    if (run->boundary_mode < 0 || run->boundary_mode > 2) return AXX_ERR_INVALID;
+    oprep P;
+    int rc = oprep_build(&P, s, q, bin_width);
+    if (rc) { oprep_free(&P); return rc; }
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (nthreads > run->n_walkers) nthreads = (int)run->n_walkers;
+    job_t* jobs = calloc(nthreads, sizeof(job_t));
+    pthread_t* th = calloc(nthreads, sizeof(pthread_t));
+    int axes = orc_axes_mask(q);
+    for (int t = 0; t < nthreads; ++t) {
+        job_t* J = &jobs[t];
+        J->P = &P; J->q = q; J->run = run; J->st = state; J->N = run->n_walkers;
+        J->Pn = q->n_profiles; J->axes = axes; J->i0 = step_begin; J->i1 = step_end;
+        J->use_bins = bin_width > 0;
+        J->w_begin = run->n_walkers * t / nthreads;
+        J->w_end = run->n_walkers * (t + 1) / nthreads;
+        if (nthreads > 1) pthread_create(&th[t], NULL, job_run, J);
+        else job_run(J);
+    }
+    if (nthreads > 1)
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    for (int t = 0; t < nthreads; ++t)
+        if (jobs[t].err) rc = jobs[t].err;
+    free(jobs); free(th);
+    oprep_free(&P);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Signal sums in fp64 (phaseAcquisitionStream.cuh:171-189, misc.cuh:21-39):
+ *   phi_j = gamma*dt*amp_j * sum_a dir_ja * (m_{p(j),a} + r0_a * S_p),
+ *   S_p = sum_i profile_p(t_i);  weight = exp(-(E + tail*rate)). */
+int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
+               const uint32_t* state, axx_result* out) {
+    if (!s || !q || !run || !state || !out) return AXX_ERR_INVALID;
+    oprep P;
+    int rc = oprep_build(&P, s, q, 0.0);
+    if (rc) { oprep_free(&P); return rc; }
+    int64_t N = run->n_walkers;
+    int M = q->n_waveforms, Pn = q->n_profiles, n = q->n_steps;
+    double* S = calloc(Pn, sizeof(double));
+    for (int i = 0; i < n; ++i)
+        for (int p = 0; p < Pn; ++p) S[p] += (double)q->profiles[(size_t)i * Pn + p];
+    double* acc = calloc((size_t)6 * M, sizeof(double));
+    const float* F = (const float*)state;
+    const int32_t* I = (const int32_t*)state;
+    int64_t nintra = 0;
+    uint64_t coll = 0, tran = 0, capn = 0;
+    for (int64_t w = 0; w < N; ++w) {
+        int comp = I[AXX_F_COMP * N + w];
+        int li = I[AXX_F_LI * N + w];
+        double Ef = (double)F[AXX_F_E * N + w] +
+                    ((double)(n - li) - (double)F[AXX_F_LF * N + w]) * (double)P.rate[comp];
+        double wt = run->relaxation ? exp(-Ef) : 1.0;
+        int intra = I[AXX_F_COMP0 * N + w] != 0;
+        nintra += intra;
+        coll += (uint32_t)I[AXX_F_NCOLL * N + w];
+        tran += (uint32_t)I[AXX_F_NTRAN * N + w];
+        capn += (uint32_t)I[AXX_F_NCAP * N + w];
+        double r0[3] = {F[AXX_F_X0 * N + w], F[AXX_F_Y0 * N + w], F[AXX_F_Z0 * N + w]};
+        for (int j = 0; j < M; ++j) {
+            int p = q->wf_profile[j];
+            double ph = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                double d = q->wf_dir[3 * j + a];
+                if (d == 0.0) continue;
+                ph += d * ((double)F[(AXX_F_MOM + 3 * p + a) * N + w] + r0[a] * S[p]);
+            }
+            double phi = q->gamma * q->dt * q->wf_amplitude[j] * ph;
+            double cr = cos(phi) * wt, sr = sin(phi) * wt;
+            acc[6 * j + 0] += cr;
+            acc[6 * j + 1] += sr;
+            acc[6 * j + 2] += cr * cr;
+            acc[6 * j + 3] += sr * sr;
+            if (intra) { acc[6 * j + 4] += cr; acc[6 * j + 5] += sr; }
+        }
+    }
+    for (int j = 0; j < M; ++j) {
+        if (out->sum_cos) out->sum_cos[j] = acc[6 * j + 0];
+        if (out->sum_sin) out->sum_sin[j] = acc[6 * j + 1];
+        if (out->sum_cos2) out->sum_cos2[j] = acc[6 * j + 2];
+        if (out->sum_sin2) out->sum_sin2[j] = acc[6 * j + 3];
+        if (out->intra_cos) out->intra_cos[j] = acc[6 * j + 4];
+        if (out->intra_sin) out->intra_sin[j] = acc[6 * j + 5];
+    }
+    out->n_walkers = N;
+    out->n_intra = nintra;
+    out->collisions = coll;
+    out->transmissions = tran;
+    out->cap_hits = capn;
+    free(S); free(acc);
+    oprep_free(&P);
+    return AXX_OK;
+}
+
+/* Expanded primitive table (for tests of the image construction). */
+int orc_image_count(const axx_substrate* s, const axx_acq* q) {
+    oprep P;
+    int rc = oprep_build(&P, s, q, 0.0);
+    int n = rc ? rc : P.n_img;
+    oprep_free(&P);
+    return n;
+}
+
+int orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
+                 const float u3[3], float ell, float* t) {
+    oprim p = {cx, cy, cz, R * R, type, 1, 0};
+    float Axy = fmaf(u3[1], u3[1], u3[0] * u3[0]);
+    float A3 = fmaf(u3[2], u3[2], Axy);
+    int ins;
+    return hit_test(&p, inside, INFINITY, INFINITY, s3[0], s3[1], s3[2], u3[0], u3[1], u3[2], Axy,
+                    A3, ell, t, &ins);
+}

@@ -1,0 +1,30 @@
+/*
+ * axx_oracle.h — CPU oracle (test infrastructure only; see axx_oracle.c).
+ * Uses the descriptor structs of include/axonexa.h so that a test hands the
+ * same inputs to the oracle and to the product library.
+ */
+#ifndef AXX_ORACLE_H
+#define AXX_ORACLE_H
+#include <stdint.h>
+#include "../include/axonexa.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+void  orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float orc_unif(uint32_t w);
+void  orc_sincos_turn(uint32_t w, float* s, float* c);
+int   orc_validate(const axx_substrate* s, const axx_acq* q);
+int   orc_axes_mask(const axx_acq* q);
+/* bin_width <= 0: reference-faithful linear scan over all primitives
+ * (lattice.cuh:184-199); > 0: the same walk through cell bins. */
+int   orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+               int step_end, uint32_t* state, int nthreads, double bin_width);
+int   orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
+                 const uint32_t* state, axx_result* out);
+int   orc_image_count(const axx_substrate* s, const axx_acq* q);
+int   orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
+                   const float u3[3], float ell, float* t);
+#ifdef __cplusplus
+}
+#endif
+#endif

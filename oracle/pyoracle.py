@@ -1,0 +1,128 @@
+"""ctypes wrapper of the CPU oracle (``oracle/liboracle.so``).
+
+Test infrastructure only: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` — never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from axonexa_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        u32p = C.POINTER(C.c_uint32)
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_unif.argtypes = [C.c_uint32]
+        L.orc_unif.restype = C.c_float
+        L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_validate.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
+        L.orc_walk.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                               C.c_int, C.c_int, u32p, C.c_int, C.c_double]
+        L.orc_reduce.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                                 u32p, C.POINTER(abi.Result)]
+        L.orc_image_count.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
+        L.orc_hit_test.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                   C.POINTER(C.c_float)]
+        _LIB = L
+    return _LIB
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise OracleError(f"{what}: {abi.ERRORS.get(rc, rc)} ({rc})")
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return np.array(list(o), dtype=np.uint32)
+
+
+def unif(w: int) -> float:
+    return float(lib().orc_unif(int(w) & 0xFFFFFFFF))
+
+
+def sincos_turn(w: int):
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincos_turn(int(w) & 0xFFFFFFFF, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def validate(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:
+    s, q = sub.struct(), acq.struct()
+    return lib().orc_validate(C.byref(s), C.byref(q))
+
+
+def image_count(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:
+    s, q = sub.struct(), acq.struct()
+    return lib().orc_image_count(C.byref(s), C.byref(q))
+
+
+def hit_test(ptype, center, R, inside, s3, u3, ell):
+    sa = (C.c_float * 3)(*s3)
+    ua = (C.c_float * 3)(*u3)
+    t = C.c_float(-1.0)
+    hit = lib().orc_hit_test(int(ptype), float(center[0]), float(center[1]), float(center[2]),
+                             float(R), int(inside), sa, ua, float(ell), C.byref(t))
+    return bool(hit), t.value
+
+
+def new_state(n_walkers: int, n_profiles: int) -> np.ndarray:
+    return np.zeros(abi.state_fields(n_profiles) * n_walkers, dtype=np.uint32)
+
+
+def walk(sub, acq, run, step_begin=0, step_end=None, state=None, threads=None,
+         bin_width=0.0) -> np.ndarray:
+    """Advance walkers over [step_begin, step_end) in place; returns the state."""
+    if step_end is None:
+        step_end = acq.n_steps
+    if state is None:
+        state = new_state(run.n_walkers, acq.n_profiles)
+    if threads is None:
+        threads = os.cpu_count() or 1
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    rc = lib().orc_walk(C.byref(s), C.byref(q), C.byref(r), int(step_begin), int(step_end),
+                        state.ctypes.data_as(C.POINTER(C.c_uint32)), int(threads),
+                        float(bin_width))
+    _check(rc, "orc_walk")
+    return state
+
+
+def reduce(sub, acq, run, state) -> abi.Signals:
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    res, bufs = abi.alloc_result(acq.n_waveforms)
+    rc = lib().orc_reduce(C.byref(s), C.byref(q), C.byref(r),
+                          state.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(res))
+    _check(rc, "orc_reduce")
+    return abi.signals_from(res, bufs)
+
+
+def run_phase(sub, acq, run, threads=None, bin_width=0.0):
+    st = walk(sub, acq, run, threads=threads, bin_width=bin_width)
+    return reduce(sub, acq, run, st), st

@@ -1,0 +1,139 @@
+"""HIP walk vs CPU oracle, walker by walker (the parity gate).
+
+Same seeded inputs on both sides; integer fields (compartment, collision,
+transmission and bounce-cap counts) must be identical, float fields within
+1e-5 relative; signal sums within 1e-12 relative.  Cases cover the
+reference's boundary policies (kernelLattice.cuh:64-79,
+boundaryCheck.cuh:138-217), initial placements (kernelLattice.cuh:29-37),
+cylinders and spheres, relaxation, chunked (checkpointed) walks.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from axonexa_amd import abi
+from axonexa_amd import substrates as sb
+from axonexa_amd import waveforms as wf
+from oracle import pyoracle as po
+from tests.helpers import compare_states
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 2.675e5
+
+
+def _engine():
+    from axonexa_amd import engine
+    return engine
+
+
+def _pgse(n_steps, dt, Gs=(0.0, 0.1, 0.3), d=None, gap=None, axis=(1, 0, 0)):
+    T = n_steps * dt
+    d = d if d is not None else 0.3 * T
+    gap = gap if gap is not None else 0.3 * T
+    return wf.build_acq([wf.RectGrad(g * 1e-6, d, gap, axis) for g in Gs], n_steps, dt, GAMMA)
+
+
+def _run_both(name, sub, acq, run, chunks=None):
+    print(f"[parity] {name}: N={run.n_walkers} steps={acq.n_steps}", flush=True)
+    eng = _engine()
+    with eng.Plan(sub, acq, run) as plan:
+        if chunks:
+            prev = 0
+            for c in list(chunks) + [acq.n_steps]:
+                plan.walk(prev, c)
+                prev = c
+        else:
+            plan.walk()
+        plan.reduce()
+        gsig = plan.results()
+        gst = plan.get_state()
+    ost = po.walk(sub, acq, run)
+    osig = po.reduce(sub, acq, run, ost)
+    info = compare_states(gst, ost, run.n_walkers, acq.n_profiles)
+    for k in ("sum_cos", "sum_sin", "sum_cos2", "sum_sin2", "intra_cos", "intra_sin"):
+        a, b = getattr(gsig, k), getattr(osig, k)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-9 * run.n_walkers), k
+    assert gsig.collisions == osig.collisions
+    assert gsig.transmissions == osig.transmissions
+    assert gsig.cap_hits == osig.cap_hits
+    assert gsig.n_intra == osig.n_intra
+    print(f"[parity] {name}: bitwise {info['bitwise_fraction']:.4f} max_rel {info['max_rel']:.2e} "
+          f"collisions {gsig.collisions} transmissions {gsig.transmissions}", flush=True)
+    return gsig, info
+
+
+def test_free_box():
+    sub = sb.empty_box(20.0, 2.0)
+    acq = _pgse(2000, 1e-3)
+    _run_both("free_box", sub, acq, abi.RunSpec(n_walkers=4096, seed=1234))
+
+
+def test_cylinder_specular():
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(2000, 1e-2)
+    sig, info = _run_both("cylinder_specular", sub, acq, abi.RunSpec(n_walkers=4096, seed=1234))
+    assert sig.collisions > 1000
+
+
+def test_cylinder_ragged_n_and_offset():
+    sub = sb.single_cylinder(20.0, 4.0, D_in=1.7, D_ex=2.0)
+    acq = _pgse(777, 1e-2, axis=(0.6, 0.8, 0.0))
+    _run_both("cylinder_ragged", sub, acq,
+              abi.RunSpec(n_walkers=1001, seed=99, walker_offset=123456, instance=7))
+
+
+def test_hex_permeable_relaxation():
+    dt = 2e-3
+    sub = sb.hex_lattice(2.5, 6.0, 3, 2, D_in=1.7, D_ex=2.0, T2_in=70.0, T2_ex=50.0,
+                         kappa=sb.kappa_for_probability(0.05, 1.7, dt))
+    acq = wf.build_acq([wf.CosGFunc(g * 1e-6, 3.0, 0.5, 2.5, ax)
+                        for g in (0.1, 0.4) for ax in ((1, 0, 0), (0, 0, 1))], 3500, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=4096, seed=1234, boundary_mode=abi.BOUNDARY_PERMEABLE,
+                      relaxation=1)
+    sig, _ = _run_both("hex_permeable", sub, acq, run)
+    assert sig.transmissions > 100
+
+
+def test_spheres_specular_3d():
+    # 2 x 2 x 2 simple-cubic sphere lattice
+    cs = [[2.5 + 5 * i, 2.5 + 5 * j, 2.5 + 5 * k] for i in range(2) for j in range(2) for k in range(2)]
+    sub = abi.SubstrateSpec(a=10.0, b=10.0, c=10.0, D_lat=2.0, types=[abi.PRIM_SPHERE] * 8,
+                            centers=cs, radii=[2.0] * 8, D=2.0, T2=np.inf, kappa=0.0)
+    acq = _pgse(1500, 1e-2, axis=(0, 0, 1))
+    sig, _ = _run_both("spheres", sub, acq, abi.RunSpec(n_walkers=2048, seed=5))
+    assert sig.collisions > 1000
+
+
+def test_sphere_permeable_inside_init():
+    sub = sb.single_sphere(16.0, 5.0, D_in=1.0, D_ex=2.5, T2_in=40.0, T2_ex=80.0,
+                           kappa=sb.kappa_for_probability(0.1, 1.0, 1e-2))
+    acq = _pgse(1200, 1e-2, axis=(1, 1, 0))
+    run = abi.RunSpec(n_walkers=2048, seed=11, init_mode=abi.INIT_INSIDE_PRIM, init_arg=0,
+                      boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both("sphere_perm_inside", sub, acq, run)
+    assert sig.n_intra == run.n_walkers
+
+
+def test_rejection_mode_compartment_init():
+    sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(1000, 1e-2)
+    run = abi.RunSpec(n_walkers=2048, seed=3, init_mode=abi.INIT_COMPARTMENT, init_arg=0,
+                      boundary_mode=abi.BOUNDARY_REJECTION)
+    sig, _ = _run_both("rejection", sub, acq, run)
+    assert sig.n_intra == 0 and sig.collisions > 0
+
+
+def test_chunked_walk_equals_oracle():
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(1000, 1e-2)
+    _run_both("chunked", sub, acq, abi.RunSpec(n_walkers=2048, seed=8), chunks=[1, 2, 333, 600])
+
+
+def test_bounce_cap_counted():
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(800, 1e-2)
+    sig, _ = _run_both("bounce_cap", sub, acq, abi.RunSpec(n_walkers=2048, seed=2, bounce_cap=1))
+    assert sig.cap_hits > 0
