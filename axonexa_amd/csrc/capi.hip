@@ -21,6 +21,7 @@
 
 #include "../../include/axonexa.h"
 #include "reduce_kernel.h"
+#include "init_kernel.h"
 #include "walk_kernel.h"
 
 using namespace axx;
@@ -45,7 +46,8 @@ struct axx_plan {
     std::vector<int> wf_p;
     // geometry sizes
     int n_img = 0, n_bins = 0, n_cand = 0, nb[3] = {1, 1, 1};
-    float ib[3] = {1, 1, 1};
+    float ib[3] = {1, 1, 1}, ib4[2] = {4, 4};
+    size_t init_lds = 0;
     double bin_width = 0;
     size_t lds_bytes = 0;
     int init_img = 0;
@@ -55,8 +57,9 @@ struct axx_plan {
     // device buffers
     float4* d_geom = nullptr;
     int* d_meta = nullptr;
+    float4* d_cyl = nullptr;
     uint32_t* d_bins = nullptr;
-    uint16_t* d_cand = nullptr;
+    uint32_t* d_cand = nullptr;
     float* d_L = nullptr;
     float* d_rate = nullptr;
     float* d_thr_out = nullptr;
@@ -152,10 +155,12 @@ struct HostPrep {
     std::vector<int> meta;
     std::vector<int> orig;
     std::vector<float> L, rate, thr_out, thr_in;
+    std::vector<float4> cyl;
     std::vector<uint32_t> bins;
-    std::vector<uint16_t> cand;
+    std::vector<uint32_t> cand;
     int nb[3] = {1, 1, 1};
     float ib[3] = {1, 1, 1};
+    float ib4[2] = {4, 4};
     double bin_width = 0;
     double Lmax = 0;
     bool sph = false;
@@ -241,9 +246,11 @@ static bool surface_near_box(const float4& g, bool sph, const double lo[3], cons
     return std::sqrt(dmin2) <= R + reach && std::sqrt(dmax2) >= R - reach;
 }
 
-static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand) {
-    return n_img * 16 + n_img * 4 + n_bins * 4 + n_cand * 2;
+static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool sph) {
+    return n_img * (sph ? 20 : 16) + n_bins * 4 + n_cand * 4;
 }
+
+constexpr int kMaxRelevant = 4;   // candidate surfaces per sub-bin the kernel keeps
 
 // Images, compartments and bins.  The float32 image centres are
 // (float)(centre + shift) computed in double; compartments use
@@ -291,7 +298,14 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
     }
     const int n_img = (int)H.geom.size();
     if (n_img > 65535) return fail(ctx, AXX_ERR_LIMIT, "too many primitive images (%d)", n_img);
-    // bin width: at least 4 step lengths; widen until the tables fit in LDS
+    for (int i = 0; i < n_img; ++i) {
+        float mf;
+        std::memcpy(&mf, &H.meta[i], sizeof(mf));
+        H.cyl.push_back(make_float4(H.geom[i].x, H.geom[i].y, H.geom[i].w, mf));
+    }
+    // bin width: at least 4 step lengths; widen until the tables fit in LDS.
+    // Each candidate carries a 4 x 4 mask of the sub-bins (xy) whose
+    // rectangle its surface comes within reach of.
     const double Ls[3] = {s->a, s->b, s->c};
     double w = bin_width_req > 0 ? bin_width_req : std::max(4.0 * Lmax, 1e-9);
     for (int attempt = 0; attempt < 64; ++attempt) {
@@ -302,14 +316,15 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         }
         if (!H.sph) nb[2] = 1;
         const long long nbins = (long long)nb[0] * nb[1] * nb[2];
-        if (nbins > (1 << 22) || lds_size(n_img, nbins, 0) > lds_budget) {
+        if (nbins > (1 << 22) || lds_size(n_img, nbins, 0, H.sph) > lds_budget) {
             if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
             w *= 1.25;
             continue;
         }
         // per-bin lists, filled image by image (ascending image index) over
         // the bins that the image's grown bounding box touches
-        std::vector<std::vector<uint16_t>> lists(nbins);
+        std::vector<std::vector<uint32_t>> lists(nbins);
+        const double bw[3] = {Ls[0] / nb[0], Ls[1] / nb[1], Ls[2] / nb[2]};
         for (int i = 0; i < n_img; ++i) {
             const bool sk = (H.meta[i] & kSphereBit) != 0;
             const double R = std::sqrt((double)H.geom[i].w) + reach;
@@ -317,32 +332,48 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
             int lo_b[3], hi_b[3];
             for (int a = 0; a < 3; ++a) {
                 if (a == 2 && !sk) { lo_b[a] = 0; hi_b[a] = nb[a] - 1; continue; }
-                const double bw = Ls[a] / nb[a];
-                lo_b[a] = std::max(0, (int)std::floor((c3[a] - R) / bw) - 1);
-                hi_b[a] = std::min(nb[a] - 1, (int)std::floor((c3[a] + R) / bw) + 1);
+                lo_b[a] = std::max(0, (int)std::floor((c3[a] - R) / bw[a]) - 1);
+                hi_b[a] = std::min(nb[a] - 1, (int)std::floor((c3[a] + R) / bw[a]) + 1);
             }
             for (int bz = lo_b[2]; bz <= hi_b[2]; ++bz)
                 for (int by = lo_b[1]; by <= hi_b[1]; ++by)
                     for (int bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
-                        const double lo[3] = {bx * s->a / nb[0], by * s->b / nb[1], bz * s->c / nb[2]};
-                        const double hi[3] = {(bx + 1) * s->a / nb[0], (by + 1) * s->b / nb[1],
-                                              (bz + 1) * s->c / nb[2]};
-                        if (surface_near_box(H.geom[i], sk, lo, hi, reach))
-                            lists[(bz * nb[1] + by) * nb[0] + bx].push_back((uint16_t)i);
+                        const double lo[3] = {bx * bw[0], by * bw[1], bz * bw[2]};
+                        const double hi[3] = {(bx + 1) * bw[0], (by + 1) * bw[1], (bz + 1) * bw[2]};
+                        if (!surface_near_box(H.geom[i], sk, lo, hi, reach)) continue;
+                        uint32_t mask = 0;
+                        for (int ly = 0; ly < 4; ++ly)
+                            for (int lx = 0; lx < 4; ++lx) {
+                                const double slo[3] = {lo[0] + lx * bw[0] / 4, lo[1] + ly * bw[1] / 4, lo[2]};
+                                const double shi[3] = {lo[0] + (lx + 1) * bw[0] / 4, lo[1] + (ly + 1) * bw[1] / 4,
+                                                       hi[2]};
+                                if (surface_near_box(H.geom[i], sk, slo, shi, reach)) mask |= 1u << (lx + 4 * ly);
+                            }
+                        if (mask) lists[(bz * nb[1] + by) * nb[0] + bx].push_back((uint32_t)i | (mask << 16));
                     }
         }
         std::vector<uint32_t> bins(nbins);
-        std::vector<uint16_t> cand;
-        bool ok = true;
-        for (long long bi = 0; bi < nbins; ++bi) {
+        std::vector<uint32_t> cand;
+        bool ok = true, crowded = false;
+        for (long long bi = 0; bi < nbins && ok; ++bi) {
             const size_t off = cand.size();
             const size_t cnt = lists[bi].size();
             if (cnt > 255 || off >= (1u << 24)) { ok = false; break; }
+            for (int sb = 0; sb < 16; ++sb) {
+                int nrel = 0;
+                for (uint32_t e : lists[bi]) nrel += (e >> (16 + sb)) & 1u;
+                if (nrel > kMaxRelevant) crowded = true;
+            }
             cand.insert(cand.end(), lists[bi].begin(), lists[bi].end());
             bins[bi] = (uint32_t)(off << 8) | (uint32_t)cnt;
         }
-        if (!ok || lds_size(n_img, nbins, cand.size()) > lds_budget) {
-            if (bin_width_req > 0 && !ok) return fail(ctx, AXX_ERR_LIMIT, "bin with > 255 candidates");
+        if (crowded) {
+            if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "more than %d surfaces near one sub-bin", kMaxRelevant);
+            w *= 0.8;
+            if (w < Lmax) return fail(ctx, AXX_ERR_LIMIT, "surfaces too dense to bin");
+            continue;
+        }
+        if (!ok || lds_size(n_img, nbins, cand.size(), H.sph) > lds_budget) {
             if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
             if (!ok) { w *= 0.8; if (w < Lmax) return fail(ctx, AXX_ERR_LIMIT, "cannot bin substrate"); continue; }
             w *= 1.25;
@@ -352,6 +383,8 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
             H.nb[a] = nb[a];
             H.ib[a] = (float)(nb[a] / Ls[a]);
         }
+        H.ib4[0] = (float)(4.0 * nb[0] / Ls[0]);
+        H.ib4[1] = (float)(4.0 * nb[1] / Ls[1]);
         H.bins.swap(bins);
         H.cand.swap(cand);
         if (H.cand.empty()) H.cand.push_back(0);
@@ -364,30 +397,15 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
 // ---------------------------------------------------------------------------
 // Kernel dispatch over (profile width, axes, sphere) instantiations
 
-typedef void (*walk_fn)(WalkArgs);
-
-template <int P, int AX>
-static walk_fn pick_sph(bool sph) {
-    return sph ? (walk_fn)walk_kernel<P, AX, true> : (walk_fn)walk_kernel<P, AX, false>;
-}
-template <int P>
-static walk_fn pick_ax(int ax, bool sph) {
-    switch (ax) {
-    case 1: return pick_sph<P, 1>(sph);
-    case 3: return pick_sph<P, 3>(sph);
-    case 5: return pick_sph<P, 5>(sph);
-    default: return pick_sph<P, 7>(sph);
-    }
-}
-static walk_fn pick_walk(int Pt, int ax, bool sph) {
+static walk_fn pick_walk(int Pt, int ax, bool sph, bool rej) {
     switch (Pt) {
-    case 1: return pick_ax<1>(ax, sph);
-    case 2: return pick_ax<2>(ax, sph);
-    case 4: return pick_ax<4>(ax, sph);
-    case 8: return pick_ax<8>(ax, sph);
-    case 16: return pick_ax<16>(ax, sph);
-    case 32: return pick_ax<32>(ax, sph);
-    default: return pick_ax<64>(ax, sph);
+    case 1: return pick_walk_p1(ax, sph, rej);
+    case 2: return pick_walk_p2(ax, sph, rej);
+    case 4: return pick_walk_p4(ax, sph, rej);
+    case 8: return pick_walk_p8(ax, sph, rej);
+    case 16: return pick_walk_p16(ax, sph, rej);
+    case 32: return pick_walk_p32(ax, sph, rej);
+    default: return pick_walk_p64(ax, sph, rej);
     }
 }
 
@@ -410,7 +428,7 @@ static int dupload(axx_ctx* ctx, T** dst, const T* src, size_t n) {
 
 static void plan_free(axx_plan* p) {
     if (!p) return;
-    void* bufs[] = {p->d_geom, p->d_meta, p->d_bins, p->d_cand, p->d_L, p->d_rate,
+    void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_bins, p->d_cand, p->d_L, p->d_rate,
                     p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
                     p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
     for (void* b : bufs)
@@ -465,9 +483,12 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     p->n_bins = (int)H.bins.size();
     p->n_cand = (int)H.cand.size();
     for (int a = 0; a < 3; ++a) { p->nb[a] = H.nb[a]; p->ib[a] = H.ib[a]; }
+    p->ib4[0] = H.ib4[0]; p->ib4[1] = H.ib4[1];
     p->bin_width = H.bin_width;
-    p->lds_bytes = lds_size(p->n_img, p->n_bins, p->n_cand);
+    p->lds_bytes = lds_size(p->n_img, p->n_bins, p->n_cand, p->sph);
     p->lds_bytes = (p->lds_bytes + 15) & ~(size_t)15;
+    p->init_lds = ((size_t)p->n_img * 20 + 15) & ~(size_t)15;
+    if (p->init_lds > ctx->lds_max) return fail(ctx, AXX_ERR_LIMIT, "too many primitive images for LDS");
     p->init_img = 0;
     if (r->init_mode == AXX_INIT_INSIDE_PRIM) {
         p->init_img = -1;
@@ -489,6 +510,7 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
 
     if ((rc = dupload(ctx, &p->d_geom, H.geom.data(), H.geom.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_cyl, H.cyl.data(), H.cyl.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_L, H.L.data(), H.L.size()))) return rc;
@@ -513,7 +535,7 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     if ((rc = dupload<unsigned long long>(ctx, &p->d_cnt, nullptr, 4))) return rc;
     HIP_TRY(ctx, hipEventCreate(&p->ev0));
     HIP_TRY(ctx, hipEventCreate(&p->ev1));
-    p->block = 256;
+    p->block = 512;
     p->wpt = 1;
     HIP_TRY(ctx, hipDeviceSynchronize());
     return AXX_OK;
@@ -542,10 +564,10 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.H2c = p->H2c; A.H2s = p->H2s;
     A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
     A.ibx = p->ib[0]; A.iby = p->ib[1]; A.ibz = p->ib[2];
+    A.ibx4 = p->ib4[0]; A.iby4 = p->ib4[1];
     A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
-    A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
+    A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_cyl = p->d_cyl; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
     A.compL = p->d_L; A.rate = p->d_rate; A.thr_out = p->d_thr_out; A.thr_in = p->d_thr_in;
-    A.prof = p->d_prof;
     A.state = p->d_state;
     A.N = p->N;
     A.k0 = (uint32_t)p->run.seed; A.k1 = (uint32_t)(p->run.seed >> 32);
@@ -555,12 +577,21 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.i0 = i0; A.i1 = i1;
     A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
     A.err = p->d_err;
-    walk_fn fn = pick_walk(p->Pt, p->ax, p->sph);
+    walk_fn fn = pick_walk(p->Pt, p->ax, p->sph, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
     const int block = p->block;
     const long long grid = (p->N + block - 1) / block;
     if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), p->lds_bytes, ctx->stream, A);
-    HIP_TRY(ctx, hipGetLastError());
+    if (i0 == 0) {
+        hipLaunchKernelGGL(init_kernel, dim3((unsigned)grid), dim3(block), p->init_lds, ctx->stream, A,
+                           3 * p->Pt);
+        HIP_TRY(ctx, hipGetLastError());
+        A.i0 = 1;
+    }
+    if (A.i0 < A.i1) {
+        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), p->lds_bytes, ctx->stream, A,
+                           (const float*)p->d_prof);
+        HIP_TRY(ctx, hipGetLastError());
+    }
     return AXX_OK;
 }
 
@@ -570,8 +601,8 @@ extern "C" int axx_plan_walk(axx_plan* p, int step_begin, int step_end) {
     if (step_begin < 0 || step_end > p->n_steps || step_begin > step_end)
         return fail(ctx, AXX_ERR_INVALID, "step range [%d, %d) outside [0, %d)", step_begin, step_end,
                     p->n_steps);
-    if (step_begin != p->next_step)
-        return fail(ctx, AXX_ERR_STATE, "walk must continue at step %d", p->next_step);
+    if (step_begin != 0 && step_begin != p->next_step)
+        return fail(ctx, AXX_ERR_STATE, "walk must restart at 0 or continue at step %d", p->next_step);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int rc = launch_walk(p, step_begin, step_end);
     if (rc) return rc;

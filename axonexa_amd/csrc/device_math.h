@@ -136,4 +136,56 @@ __device__ __forceinline__ bool hit_test(float ox, float oy, float oz, float ux,
     return true;
 }
 
+
+// Correctly rounded sqrt for arguments known to be normal and finite
+// (here (1-c)(1+c) in [2^-23, 1]): hardware estimate plus the one-ulp
+// residual correction; equal to IEEE sqrtf on that domain.
+__device__ __forceinline__ float sqrt_unit(float x) {
+    float y = __builtin_amdgcn_sqrtf(x);
+    const float ydn = __int_as_float(__float_as_int(y) - 1);
+    const float yup = __int_as_float(__float_as_int(y) + 1);
+    const float rdn = fmaf(-ydn, y, x);
+    const float rup = fmaf(-yup, y, x);
+    y = rdn <= 0.0f ? ydn : y;
+    y = rup > 0.0f ? yup : y;
+    return y;
+}
+
+// The crossing test of hit_test written as a branch-free decision followed
+// by one root evaluation for the lanes that hit (same operations and
+// operands, so the same results).
+template <bool SPH>
+__device__ __forceinline__ bool cross(float ox, float oy, float oz, float ux, float uy, float uz,
+                                      float A, float R2, bool same_comp, float H2, float ell,
+                                      float& t_out, bool& ins_out) {
+    float B, q;
+    if (SPH) {
+        B = fmaf(oz, uz, fmaf(oy, uy, ox * ux));
+        q = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+    } else {
+        B = fmaf(oy, uy, ox * ux);
+        q = fmaf(oy, oy, ox * ox);
+    }
+    const float C = q - R2;
+    const bool inside = same_comp && q < H2;
+    ins_out = inside;
+    const float Cp = inside ? (C < 0.0f ? C : 0.0f) : (C > 0.0f ? C : 0.0f);
+    const float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
+    const float disc = fmaf(B, B, -(A * Cp));
+    const bool hit_in = fe > 0.0f;
+    const bool hit_out = B < 0.0f && (Cp == 0.0f || fe < 0.0f || ((-B) < A * ell && disc > 0.0f));
+    const bool hit = A > 0.0f && (inside ? hit_in : hit_out);
+    if (hit) {
+        const float sq = sqrtf(disc > 0.0f ? disc : 0.0f);
+        // inside, B < 0: (sq - B) / A;  inside, B >= 0: -Cp / (B + sq);
+        // outside: Cp / (sq - B)
+        const bool far_root = inside && B < 0.0f;
+        const float num = far_root ? sq - B : (inside ? -Cp : Cp);
+        const float den = far_root ? A : (inside ? B + sq : sq - B);
+        const float t = den > 0.0f ? num / den : 0.0f;
+        t_out = t < ell ? t : ell;
+    }
+    return hit;
+}
+
 }  // namespace axx

@@ -1,0 +1,83 @@
+// init_kernel.h — initial walker placement (included by one translation
+// unit only).
+#pragma once
+#include "walk_kernel.h"
+
+namespace axx {
+
+// Initial placement (kernelLattice.cuh:29-37, lattice.cuh:31-58,
+// cylinderXY.cuh:176-183), all draws on the init stream; writes the full
+// walker state (n_mom moment fields zeroed).
+__global__ __launch_bounds__(1024) void init_kernel(WalkArgs A, int n_mom) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* geom = reinterpret_cast<float4*>(lds);
+    int* meta = reinterpret_cast<int*>(geom + A.n_img);
+    for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
+        geom[i] = A.g_geom[i];
+        meta[i] = A.g_meta[i];
+    }
+    __syncthreads();
+    const long long N = A.N;
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= N) return;
+    uint32_t* st = A.state;
+    float* fs = reinterpret_cast<float*>(st);
+    int* is = reinterpret_cast<int*>(st);
+    const uint32_t wid = A.woff + (uint32_t)w;
+    bool done = false;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    int comp = 0;
+    if (A.init_mode == 0) {
+        u32x4 o = philox10(wid, 0u, kStreamInit, A.inst, A.k0, A.k1);
+        x = unif(o.x) * A.a;
+        y = unif(o.y) * A.b;
+        z = unif(o.z) * A.c;
+        comp = region_of(geom, meta, A.n_img, x, y, z);
+        done = true;
+    } else {
+        for (uint32_t it = 0; it < (1u << 20) && !done; ++it) {
+            u32x4 o = philox10(wid, it, kStreamInit, A.inst, A.k0, A.k1);
+            if (A.init_mode == 1) {
+                float px = unif(o.x) * A.a, py = unif(o.y) * A.b, pz = unif(o.z) * A.c;
+                int cm = region_of(geom, meta, A.n_img, px, py, pz);
+                if (cm != A.init_arg) continue;
+                x = px; y = py; z = pz; comp = cm;
+                done = true;
+            } else {
+                float4 gk = geom[A.init_img];
+                bool sk = (meta[A.init_img] & kSphereBit) != 0;
+                float Rf = sqrtf(gk.w);
+                float px = gk.x + (unif(o.x) * 2.0f - 1.0f) * Rf;
+                float py = gk.y + (unif(o.y) * 2.0f - 1.0f) * Rf;
+                float pz = gk.z + (unif(o.z) * 2.0f - 1.0f) * Rf;
+                if (!inside_at(gk, sk, px, py, pz)) continue;
+                px = wrap1(px, A.a); py = wrap1(py, A.b); pz = wrap1(pz, A.c);
+                x = px; y = py; z = pz;
+                comp = region_of(geom, meta, A.n_img, px, py, pz);
+                done = true;
+            }
+        }
+    }
+    if (!done) atomicOr(A.err, 1);
+    fs[(size_t)F_X * N + w] = x;
+    fs[(size_t)F_Y * N + w] = y;
+    fs[(size_t)F_Z * N + w] = z;
+    fs[(size_t)F_DX * N + w] = 0.0f;
+    fs[(size_t)F_DY * N + w] = 0.0f;
+    fs[(size_t)F_DZ * N + w] = 0.0f;
+    fs[(size_t)F_X0 * N + w] = x;
+    fs[(size_t)F_Y0 * N + w] = y;
+    fs[(size_t)F_Z0 * N + w] = z;
+    fs[(size_t)F_E * N + w] = 0.0f;
+    fs[(size_t)F_LF * N + w] = 0.0f;
+    is[(size_t)F_COMP * N + w] = comp;
+    is[(size_t)F_COMP0 * N + w] = comp;
+    is[(size_t)F_LI * N + w] = 1;
+    st[(size_t)F_NCOLL * N + w] = 0u;
+    st[(size_t)F_NTRAN * N + w] = 0u;
+    st[(size_t)F_NCAP * N + w] = 0u;
+    is[(size_t)F_STEP * N + w] = 1;
+    for (int f = 0; f < n_mom; ++f) fs[(size_t)(F_MOM + f) * N + w] = 0.0f;
+}
+
+}  // namespace axx

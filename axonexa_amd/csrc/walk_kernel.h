@@ -23,17 +23,18 @@ struct WalkArgs {
     float a, b, c;
     float H2c, H2s;                         // squared half of the smallest cell side (xy; xyz)
     int nbx, nby, nbz;
-    float ibx, iby, ibz;
+    float ibx, iby, ibz;                    // bins per unit length
+    float ibx4, iby4;                       // sub-bins (4 x 4 per bin) per unit length
     int n_img, n_bins, n_cand;
     const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R^2
     const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit
+    const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R^2, meta bits (cylinder-only substrates)
     const uint32_t* __restrict__ g_bins;    // [n_bins] offset << 8 | count
-    const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
+    const uint32_t* __restrict__ g_cand;    // [n_cand] image index | sub-bin mask << 16, ascending per bin
     const float* __restrict__ compL;        // [n_comp] step length
     const float* __restrict__ rate;         // [n_comp] dt / T2
     const float* __restrict__ thr_out;      // [n_prims] 4 kappa / v_inside
     const float* __restrict__ thr_in;       // [n_prims] 4 kappa / v_lattice
-    const float* __restrict__ prof;         // [n_steps][P] (P = template width)
     uint32_t* __restrict__ state;           // SoA, AXX_STATE_FIELDS(P) x N
     long long N;
     uint32_t k0, k1, inst, woff;
@@ -71,20 +72,67 @@ __device__ __forceinline__ int region_of(const float4* geom, const int* meta, in
     return 0;
 }
 
-template <int P, int AX, bool SPH>
-__global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    float4* geom = reinterpret_cast<float4*>(lds);
-    int* meta = reinterpret_cast<int*>(geom + A.n_img);
-    uint32_t* bins = reinterpret_cast<uint32_t*>(meta + A.n_img);
-    uint16_t* cand = reinterpret_cast<uint16_t*>(bins + A.n_bins);
+// Substrate tables in LDS.  Cylinder-only substrates keep one float4 per
+// image (cx, cy, R^2, meta); others keep (cx, cy, cz, R^2) plus meta.
+struct LdsTables {
+    float4* geom;
+    int* meta;
+    uint32_t* bins;
+    uint32_t* cand;
+};
+
+template <bool SPH>
+__device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned char* lds) {
+    LdsTables T;
+    T.geom = reinterpret_cast<float4*>(lds);
+    T.meta = reinterpret_cast<int*>(T.geom + A.n_img);
+    T.bins = SPH ? reinterpret_cast<uint32_t*>(T.meta + A.n_img) : reinterpret_cast<uint32_t*>(T.geom + A.n_img);
+    T.cand = T.bins + A.n_bins;
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
-        geom[i] = A.g_geom[i];
-        meta[i] = A.g_meta[i];
+        if (SPH) {
+            T.geom[i] = A.g_geom[i];
+            T.meta[i] = A.g_meta[i];
+        } else {
+            T.geom[i] = A.g_cyl[i];
+        }
     }
-    for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) bins[i] = A.g_bins[i];
-    for (int i = threadIdx.x; i < A.n_cand; i += blockDim.x) cand[i] = A.g_cand[i];
+    for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
+    for (int i = threadIdx.x; i < A.n_cand; i += blockDim.x) T.cand[i] = A.g_cand[i];
     __syncthreads();
+    return T;
+}
+
+// Image k as (cx, cy, cz, R^2) and its meta word.
+template <bool SPH>
+__device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
+    const float4 g = T.geom[k];
+    if (SPH) {
+        mt = T.meta[k];
+        return g;
+    }
+    mt = __float_as_int(g.w);
+    return make_float4(g.x, g.y, 0.0f, g.z);
+}
+
+// Profile samples are read through the constant address space: the address
+// is wave-uniform, so these become scalar loads (s_load) into SGPRs.
+typedef const __attribute__((address_space(4))) float* const_fptr;
+
+typedef void (*walk_fn)(WalkArgs, const float*);
+walk_fn pick_walk_p1(int ax, bool sph, bool rej);
+walk_fn pick_walk_p2(int ax, bool sph, bool rej);
+walk_fn pick_walk_p4(int ax, bool sph, bool rej);
+walk_fn pick_walk_p8(int ax, bool sph, bool rej);
+walk_fn pick_walk_p16(int ax, bool sph, bool rej);
+walk_fn pick_walk_p32(int ax, bool sph, bool rej);
+walk_fn pick_walk_p64(int ax, bool sph, bool rej);
+
+template <int P, int AX, bool SPH, bool REJ>
+__global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsTables T = stage_tables<SPH>(A, lds);
+    const uint32_t* bins = T.bins;
+    const uint32_t* cand = T.cand;
 
     const long long N = A.N;
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,148 +141,107 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
     float* fs = reinterpret_cast<float*>(st);
     int* is = reinterpret_cast<int*>(st);
     const uint32_t wid = A.woff + (uint32_t)w;
+    const_fptr prof = (const_fptr)prof_g;
 
+    // field f of this walker: base + f*N (wave-uniform) + w (32-bit lane offset)
+    const uint32_t wo = (uint32_t)w;
     float x, y, z, dx, dy, dz, E, lf;
     int comp, li;
     uint32_t ncoll, ntran, ncap;
     float m[P][3];
-
-    if (A.i0 == 0) {
-        // Initial placement (kernelLattice.cuh:29-37, lattice.cuh:31-58,
-        // cylinderXY.cuh:176-183), all draws on the init stream.
-        bool done = false;
-        x = y = z = 0.0f;
-        comp = 0;
-        if (A.init_mode == 0) {
-            u32x4 o = philox10(wid, 0u, kStreamInit, A.inst, A.k0, A.k1);
-            x = unif(o.x) * A.a;
-            y = unif(o.y) * A.b;
-            z = unif(o.z) * A.c;
-            comp = region_of(geom, meta, A.n_img, x, y, z);
-            done = true;
-        } else {
-            for (uint32_t it = 0; it < (1u << 20) && !done; ++it) {
-                u32x4 o = philox10(wid, it, kStreamInit, A.inst, A.k0, A.k1);
-                if (A.init_mode == 1) {
-                    float px = unif(o.x) * A.a, py = unif(o.y) * A.b, pz = unif(o.z) * A.c;
-                    int cm = region_of(geom, meta, A.n_img, px, py, pz);
-                    if (cm != A.init_arg) continue;
-                    x = px; y = py; z = pz; comp = cm;
-                    done = true;
-                } else {
-                    float4 gk = geom[A.init_img];
-                    bool sk = (meta[A.init_img] & kSphereBit) != 0;
-                    float Rf = sqrtf(gk.w);
-                    float px = gk.x + (unif(o.x) * 2.0f - 1.0f) * Rf;
-                    float py = gk.y + (unif(o.y) * 2.0f - 1.0f) * Rf;
-                    float pz = gk.z + (unif(o.z) * 2.0f - 1.0f) * Rf;
-                    if (!inside_at(gk, sk, px, py, pz)) continue;
-                    px = wrap1(px, A.a); py = wrap1(py, A.b); pz = wrap1(pz, A.c);
-                    x = px; y = py; z = pz;
-                    comp = region_of(geom, meta, A.n_img, px, py, pz);
-                    done = true;
-                }
-            }
-        }
-        if (!done) atomicOr(A.err, 1);
-        dx = dy = dz = 0.0f;
-        E = 0.0f; lf = 0.0f; li = 1;
-        ncoll = ntran = ncap = 0u;
+    x = (fs + F_X * N)[wo]; y = (fs + F_Y * N)[wo]; z = (fs + F_Z * N)[wo];
+    dx = (fs + F_DX * N)[wo]; dy = (fs + F_DY * N)[wo]; dz = (fs + F_DZ * N)[wo];
+    E = (fs + F_E * N)[wo]; lf = (fs + F_LF * N)[wo];
+    comp = (is + F_COMP * N)[wo]; li = (is + F_LI * N)[wo];
+    ncoll = (st + F_NCOLL * N)[wo]; ntran = (st + F_NTRAN * N)[wo];
+    ncap = (st + F_NCAP * N)[wo];
 #pragma unroll
-        for (int p = 0; p < P; ++p) m[p][0] = m[p][1] = m[p][2] = 0.0f;
-        fs[(size_t)F_X0 * N + w] = x;
-        fs[(size_t)F_Y0 * N + w] = y;
-        fs[(size_t)F_Z0 * N + w] = z;
-        is[(size_t)F_COMP0 * N + w] = comp;
-    } else {
-        x = fs[(size_t)F_X * N + w]; y = fs[(size_t)F_Y * N + w]; z = fs[(size_t)F_Z * N + w];
-        dx = fs[(size_t)F_DX * N + w]; dy = fs[(size_t)F_DY * N + w]; dz = fs[(size_t)F_DZ * N + w];
-        E = fs[(size_t)F_E * N + w]; lf = fs[(size_t)F_LF * N + w];
-        comp = is[(size_t)F_COMP * N + w]; li = is[(size_t)F_LI * N + w];
-        ncoll = st[(size_t)F_NCOLL * N + w]; ntran = st[(size_t)F_NTRAN * N + w];
-        ncap = st[(size_t)F_NCAP * N + w];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            m[p][0] = (AX & 1) ? fs[(size_t)(F_MOM + 3 * p + 0) * N + w] : 0.0f;
-            m[p][1] = (AX & 2) ? fs[(size_t)(F_MOM + 3 * p + 1) * N + w] : 0.0f;
-            m[p][2] = (AX & 4) ? fs[(size_t)(F_MOM + 3 * p + 2) * N + w] : 0.0f;
-        }
+    for (int p = 0; p < P; ++p) {
+        m[p][0] = (AX & 1) ? (fs + (F_MOM + 3 * p + 0) * N)[wo] : 0.0f;
+        m[p][1] = (AX & 2) ? (fs + (F_MOM + 3 * p + 1) * N)[wo] : 0.0f;
+        m[p][2] = (AX & 4) ? (fs + (F_MOM + 3 * p + 2) * N)[wo] : 0.0f;
     }
 
-    const float a = A.a, b = A.b, c = A.c;
     const int cap = A.cap;
     const bool permeable = A.mode == 1;
-    const bool rejection = A.mode == 2;
     float Lc = A.compL[comp];
-    const int ibeg = A.i0 < 1 ? 1 : A.i0;
-    u32x4 o = {0u, 0u, 0u, 0u};
 
-    for (int i = ibeg; i < A.i1; ++i) {
-        const uint32_t h = (uint32_t)(i - 1) & 1u;
-        if (h == 0u || i == ibeg) o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        const uint32_t wa = h ? o.z : o.x;
-        const uint32_t wb = h ? o.w : o.y;
+    // One step i: direction from the two Philox words, boundary handling,
+    // wrap, displacement and phase moments (kernelLattice.cuh:46-99).
+    auto step = [&](const int i, const uint32_t wa, const uint32_t wb) {
         // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
         const float ct = unif(wb) * 2.0f - 1.0f;
-        const float sth = sqrtf((1.0f - ct) * (1.0f + ct));
+        const float sth = sqrt_unit((1.0f - ct) * (1.0f + ct));
         float sphi, cphi;
         sincos_turn(wa, sphi, cphi);
         float ux = sth * cphi, uy = sth * sphi, uz = ct;
         float ell = Lc;
         float sx = x, sy = y, sz = z;
-        // candidate surfaces of the start bin
-        int bx = (int)(x * A.ibx), by = (int)(y * A.iby), bz = SPH ? (int)(z * A.ibz) : 0;
-        bx = bx > A.nbx - 1 ? A.nbx - 1 : bx;
-        by = by > A.nby - 1 ? A.nby - 1 : by;
-        bz = bz > A.nbz - 1 ? A.nbz - 1 : bz;
-        bx = bx < 0 ? 0 : bx;
-        by = by < 0 ? 0 : by;
-        bz = bz < 0 ? 0 : bz;
-        const uint32_t rec = bins[(bz * A.nby + by) * A.nbx + bx];
+        // candidate surfaces of the start bin, filtered by the walker's
+        // sub-bin: up to four image indices packed in two registers
+        const int sbx = min(max((int)(x * A.ibx4), 0), 4 * A.nbx - 1);
+        const int sby = min(max((int)(y * A.iby4), 0), 4 * A.nby - 1);
+        const int bz = SPH ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
+        const uint32_t rec = bins[(bz * A.nby + (sby >> 2)) * A.nbx + (sbx >> 2)];
         const int cnt = (int)(rec & 0xFFu);
-        const uint16_t* cl = cand + (rec >> 8);
+        const uint32_t* cl = cand + (rec >> 8);
+        const uint32_t sh = 16u + (uint32_t)((sbx & 3) | ((sby & 3) << 2));
+        uint32_t r01 = 0u, r23 = 0u;
+        int nrel = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const uint32_t e = cl[j];
+            const uint32_t bit = (e >> sh) & 1u;
+            const uint32_t v = bit ? (e & 0xFFFFu) << ((nrel & 1) << 4) : 0u;
+            if (nrel < 2) r01 |= v; else r23 |= v;
+            nrel += (int)bit;
+        }
+        auto rel = [&](int r) -> int {
+            return (int)(((r < 2 ? r01 : r23) >> ((r & 1) << 4)) & 0xFFFFu);
+        };
 
-        if (rejection) {
+        if (REJ) {
+            // simple rejection (kernelLattice.cuh:73-78)
             const float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
             bool rej = false;
-            for (int j = 0; j < cnt && !rej; ++j) {
-                const int k = cl[j];
-                const float4 g = geom[k];
-                const bool sk = SPH && (meta[k] & kSphereBit);
-                rej = inside_at(g, sk, sx, sy, sz) != inside_at(g, sk, ex, ey, ez);
+            for (int r = 0; r < nrel; ++r) {
+                int mt;
+                const float4 g = image_at<SPH>(T, rel(r), mt);
+                const bool sk = SPH && (mt & kSphereBit);
+                rej = rej || (inside_at(g, sk, sx, sy, sz) != inside_at(g, sk, ex, ey, ez));
             }
             if (rej) { ncoll++; ell = 0.0f; }
-        } else if (cnt > 0) {
+        } else {
             float Axy = fmaf(uy, uy, ux * ux);
             float A3 = fmaf(uz, uz, Axy);
             int nb = 0, kc = 0;
-            for (;;) {
+            bool live = nrel > 0;
+            while (live) {
+                // nearest crossing over the candidates (lattice.cuh:205-247)
                 float best = __builtin_inff();
                 int bk = -1;
                 bool bins_ = false;
-                for (int j = 0; j < cnt; ++j) {
-                    const int k = cl[j];
-                    const float4 g = geom[k];
-                    const int mt = meta[k];
+                for (int r = 0; r < nrel; ++r) {
+                    const int k = rel(r);
+                    int mt;
+                    const float4 g = image_at<SPH>(T, k, mt);
                     const bool sk = SPH && (mt & kSphereBit);
                     const bool same = comp == (mt & kCompMask);
                     float t;
-                    bool hit, ins;
-                    if (sk)
-                        hit = hit_test<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w, same,
-                                             A.H2s, ell, t, ins);
-                    else
-                        hit = hit_test<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w, same,
-                                              A.H2c, ell, t, ins);
+                    bool ins;
+                    const bool hit = sk ? cross<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w, same,
+                                                      A.H2s, ell, t, ins)
+                                        : cross<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w, same,
+                                                       A.H2c, ell, t, ins);
                     if (hit && t < best) {
                         best = t;
                         bk = k;
                         bins_ = ins;
                     }
                 }
-                if (bk < 0) break;
-                const float4 g = geom[bk];
-                const int mt = meta[bk];
+                live = bk >= 0;
+                if (!live) break;
+                int mt;
+                const float4 g = image_at<SPH>(T, bk, mt);
                 const bool sk = SPH && (mt & kSphereBit);
                 const int pcomp = mt & kCompMask;
                 const float t = best;
@@ -243,18 +250,19 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
                 const bool ins = bins_;
                 bool transmit = false;
                 if (permeable) {
-                    u32x4 oc = philox10(wid, (uint32_t)i, kStreamColl | ((uint32_t)kc >> 2), A.inst, A.k0, A.k1);
+                    // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
+                    const u32x4 oc = philox10(wid, (uint32_t)i, kStreamColl | ((uint32_t)kc >> 2), A.inst,
+                                              A.k0, A.k1);
                     const uint32_t sel = (uint32_t)kc & 3u;
                     const uint32_t wv = sel == 0 ? oc.x : sel == 1 ? oc.y : sel == 2 ? oc.z : oc.w;
-                    const float U = unif(wv);
                     const float thr = ins ? A.thr_out[pcomp - 1] : A.thr_in[pcomp - 1];
-                    transmit = U <= thr;
+                    transmit = unif(wv) <= thr;
                     kc++;
                 }
                 if (transmit) {
+                    // continue on the far side with its speed (boundaryCheck.cuh:211-215)
                     const int nc = ins ? 0 : pcomp;
-                    const float rem = ell - t;
-                    const float f = rem / Lc;
+                    const float f = (ell - t) / Lc;
                     const float ph = 1.0f - f;
                     E = fmaf((float)(i - li) + (ph - lf), A.rate[comp], E);
                     li = i;
@@ -264,6 +272,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
                     ell = f * Lc;
                     ntran++;
                 } else {
+                    // specular reflection (boundaryCheck.cuh:149-167)
                     float nx = px - g.x, ny = py - g.y, nz = 0.0f, kd;
                     if (sk) {
                         nz = pz - g.z;
@@ -284,18 +293,22 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
                     ell = ell - t;
                 }
                 sx = px; sy = py; sz = pz;
-                if (++nb >= cap) { ncap++; ell = 0.0f; break; }
+                if (++nb >= cap) {
+                    ncap++;
+                    ell = 0.0f;
+                    live = false;
+                }
             }
         }
         const float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
         dx = dx + (ex - x);
         dy = dy + (ey - y);
         dz = dz + (ez - z);
-        x = wrap1(ex, a);
-        y = wrap1(ey, b);
-        z = wrap1(ez, c);
+        x = wrap1(ex, A.a);
+        y = wrap1(ey, A.b);
+        z = wrap1(ez, A.c);
         // phase moments (kernelLattice.cuh:81-85): wave-uniform profile samples
-        const float* g = A.prof + (size_t)i * P;
+        const_fptr g = prof + (size_t)i * P;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const float gp = g[p];
@@ -303,20 +316,47 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A) {
             if (AX & 2) m[p][1] = fmaf(gp, dy, m[p][1]);
             if (AX & 4) m[p][2] = fmaf(gp, dz, m[p][2]);
         }
+    };
+
+    // Steps are paired: one Philox call, counter (walker, (i-1)/2, 0, instance),
+    // gives words (x, y) to step i with i-1 even and (z, w) to the next one.
+    int i = A.i0 < 1 ? 1 : A.i0;
+    if (((i - 1) & 1) && i < A.i1) {
+        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+        step(i, o.z, o.w);
+        ++i;
+    }
+    for (; i + 1 < A.i1; i += 2) {
+        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+        step(i, o.x, o.y);
+        step(i + 1, o.z, o.w);
+    }
+    if (i < A.i1) {
+        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+        step(i, o.x, o.y);
     }
 
-    fs[(size_t)F_X * N + w] = x; fs[(size_t)F_Y * N + w] = y; fs[(size_t)F_Z * N + w] = z;
-    fs[(size_t)F_DX * N + w] = dx; fs[(size_t)F_DY * N + w] = dy; fs[(size_t)F_DZ * N + w] = dz;
-    fs[(size_t)F_E * N + w] = E; fs[(size_t)F_LF * N + w] = lf;
-    is[(size_t)F_COMP * N + w] = comp; is[(size_t)F_LI * N + w] = li;
-    st[(size_t)F_NCOLL * N + w] = ncoll; st[(size_t)F_NTRAN * N + w] = ntran;
-    st[(size_t)F_NCAP * N + w] = ncap;
-    is[(size_t)F_STEP * N + w] = A.i1;
+    // Re-derive the state addresses after the loop (the empty asm hides
+    // their equality with the pre-loop ones, so no 64-bit address stays live
+    // across the step loop).
+    float* fso = fs;
+    uint32_t wo2 = wo;
+    asm volatile("" : "+s"(fso));
+    asm volatile("" : "+v"(wo2));
+    int* iso = reinterpret_cast<int*>(fso);
+    uint32_t* sto = reinterpret_cast<uint32_t*>(fso);
+    (fso + F_X * N)[wo2] = x; (fso + F_Y * N)[wo2] = y; (fso + F_Z * N)[wo2] = z;
+    (fso + F_DX * N)[wo2] = dx; (fso + F_DY * N)[wo2] = dy; (fso + F_DZ * N)[wo2] = dz;
+    (fso + F_E * N)[wo2] = E; (fso + F_LF * N)[wo2] = lf;
+    (iso + F_COMP * N)[wo2] = comp; (iso + F_LI * N)[wo2] = li;
+    (sto + F_NCOLL * N)[wo2] = ncoll; (sto + F_NTRAN * N)[wo2] = ntran;
+    (sto + F_NCAP * N)[wo2] = ncap;
+    (iso + F_STEP * N)[wo2] = A.i1;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        fs[(size_t)(F_MOM + 3 * p + 0) * N + w] = m[p][0];
-        fs[(size_t)(F_MOM + 3 * p + 1) * N + w] = m[p][1];
-        fs[(size_t)(F_MOM + 3 * p + 2) * N + w] = m[p][2];
+        if (AX & 1) (fso + (F_MOM + 3 * p + 0) * N)[wo2] = m[p][0];
+        if (AX & 2) (fso + (F_MOM + 3 * p + 1) * N)[wo2] = m[p][1];
+        if (AX & 4) (fso + (F_MOM + 3 * p + 2) * N)[wo2] = m[p][2];
     }
 }
 

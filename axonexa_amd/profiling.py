@@ -1,0 +1,95 @@
+"""Profiling helpers.
+
+    python -m axonexa_amd.profiling copy --mib 1024
+        one device-to-device copy of a known size (calibrates the HBM byte
+        counters FETCH_SIZE / WRITE_SIZE against a plain copy kernel)
+
+    python -m axonexa_amd.profiling summarize <results.db> [--kernel NAME]
+        print per-kernel dispatch count, mean duration and the mean of every
+        collected counter per dispatch from a rocprofv3 SQLite database
+"""
+from __future__ import annotations
+
+import argparse
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def copy(mib: int) -> None:
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB ({n * 4} bytes read, {n * 4} bytes written)")
+
+
+def _kernels(con):
+    rows = con.execute(
+        "select k.id, s.string, k.start, k.end from rocpd_kernel_dispatch k "
+        "join rocpd_info_kernel_symbol ks on k.kernel_id = ks.id "
+        "join rocpd_string s on ks.display_name = s.id").fetchall()
+    return rows
+
+
+def summarize(db: str, kernel: str = "") -> str:
+    con = sqlite3.connect(db)
+    out = []
+    try:
+        rows = con.execute("select name, count(*), avg(duration) from kernels group by name").fetchall()
+        out.append(f"{'kernel':<60} {'dispatches':>10} {'avg_ms':>12}")
+        for name, cnt, dur in sorted(rows, key=lambda r: -r[1] * r[2]):
+            if kernel and kernel not in name:
+                continue
+            out.append(f"{name[:60]:<60} {cnt:>10} {dur / 1e6:>12.4f}")
+    except sqlite3.Error as e:
+        out.append(f"(no kernel table: {e})")
+    try:
+        pm = con.execute("select * from counters_collection limit 1")
+        cols = [d[0] for d in pm.description]
+        q = con.execute("select * from counters_collection").fetchall()
+        ci = {c: i for i, c in enumerate(cols)}
+        agg = defaultdict(list)
+        for r in q:
+            name = r[ci.get("kernel_name", ci.get("name", 0))]
+            if kernel and kernel not in str(name):
+                continue
+            cname = r[ci["counter_name"]]
+            val = r[ci["value"]]
+            disp = r[ci.get("dispatch_id", 0)]
+            agg[(str(name), cname)].append((disp, val))
+        if agg:
+            out.append("")
+            out.append(f"{'kernel':<40} {'counter':<28} {'per_dispatch_mean':>20}")
+            for (name, cname), vals in sorted(agg.items()):
+                per = defaultdict(float)
+                for d, v in vals:
+                    per[d] += v
+                mean = sum(per.values()) / len(per)
+                out.append(f"{name[:40]:<40} {cname:<28} {mean:>20.6g}")
+    except sqlite3.Error:
+        pass
+    return "\n".join(out)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=1024)
+    s = sub.add_parser("summarize")
+    s.add_argument("db")
+    s.add_argument("--kernel", default="")
+    args = ap.parse_args(argv)
+    if args.cmd == "copy":
+        copy(args.mib)
+    else:
+        print(summarize(args.db, args.kernel))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,141 @@
+"""The benchmark and validation workloads, as concrete synthetic inputs.
+
+Engine units: um, ms, T; gamma = 2.675e5 rad/(ms T); gradients in T/um
+(1 T/m = 1e-6 T/um).
+
+  c1_free      periodic 20 um box, D = 2.0, PGSE d = 5 ms, Delta = 15 ms,
+               16 gradients 0..0.3 T/m along x, 1e5 walkers, dt = 1e-3 ms,
+               20,000 steps, seed 1234
+  c1_cylinder  the same box with one impermeable cylinder R = 4 um
+  c2_hex       1000 cylinders R = 2.5 um, spacing 6 um, hexagonal, periodic
+               120 x 259.8 um cell; transmission probability 1e-3 per collision
+               (from inside); T2 70/50 ms; D 1.7/2.0; 1e6 walkers;
+               dt = 2e-4 ms, 250,000 steps; cosine OGSE, lobes 20 ms, gap 5 ms,
+               8 frequencies 25..200 Hz x 8 amplitudes to 0.4 T/m, along x
+               (perpendicular) and z (parallel)
+  c3_gamma     400 cylinders, radii Gamma(k=5, theta=0.4 um), packed to 65 %
+               area fraction; 4e6 walkers, dt = 5e-4 ms, 80,000 steps;
+               64 profiles (PGSE, cos, sin, square wave x 16 timings) x
+               30 directions x 3 amplitudes to 0.3 T/m
+
+The reference drivers these restate: examples/Cos_OneCyl.cu (one cylinder,
+cosine OGSE), examples/hexPhase.cu (hexagonal lattice), examples/gammadist.cu
+(gamma packing), examples/planePhase.cu / spherePhase.cu (validation).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List
+
+import numpy as np
+
+from . import abi
+from . import substrates as sb
+from . import waveforms as wf
+
+GAMMA = 2.675e5          # rad / (ms T)
+T_PER_M = 1e-6           # T/um per T/m
+
+
+@dataclass
+class Workload:
+    name: str
+    sub: abi.SubstrateSpec
+    acq: abi.AcqSpec
+    run: abi.RunSpec
+    description: str
+
+    @property
+    def walker_steps(self) -> int:
+        return int(self.run.n_walkers) * (self.acq.n_steps - 1)
+
+    def flops_per_walker_step(self, rho: float) -> float:
+        """Algorithmic FP32 work per walker-step:
+        F = 22 + 16*beta + 35*rho + tau + 2*P*A (FMA = 2, sqrt = 1)."""
+        beta = 1 if self.sub.n_prims > 0 else 0
+        tau = 1 if self.run.relaxation and (np.isfinite(self.sub.T2_lat) or
+                                            np.any(np.isfinite(self.sub.T2))) else 0
+        A = 0
+        for a in range(3):
+            if np.any((self.acq.wf_dir[:, a] != 0) & (self.acq.wf_amplitude != 0)):
+                A += 1
+        return 22 + 16 * beta + 35 * rho + tau + 2 * self.acq.n_profiles * A
+
+
+def c1_acq(n_steps: int = 20000, dt: float = 1e-3) -> abi.AcqSpec:
+    fs = [wf.RectGrad(g * T_PER_M, 5.0, 10.0, (1, 0, 0)) for g in np.linspace(0.0, 0.3, 16)]
+    return wf.build_acq(fs, n_steps, dt, GAMMA)
+
+
+def c1_free(n_walkers: int = 100000, n_steps: int = 20000) -> Workload:
+    return Workload("c1_free", sb.empty_box(20.0, 2.0), c1_acq(n_steps),
+                    abi.RunSpec(n_walkers=n_walkers, seed=1234),
+                    "free diffusion, periodic 20 um box, PGSE 16 gradients")
+
+
+def c1_cylinder(n_walkers: int = 100000, n_steps: int = 20000) -> Workload:
+    return Workload("c1_cylinder", sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0), c1_acq(n_steps),
+                    abi.RunSpec(n_walkers=n_walkers, seed=1234),
+                    "one impermeable cylinder R=4 um in a periodic 20 um box, PGSE 16 gradients")
+
+
+def c2_functors(lobe: float = 20.0, gap: float = 5.0) -> List[wf.CosGFunc]:
+    fs = []
+    for f_hz in np.arange(1, 9) * 25.0:
+        periods = f_hz * 1e-3 * lobe          # cycles per ms x ms
+        for g in np.arange(1, 9) * 0.05:
+            for ax in ((1, 0, 0), (0, 0, 1)):
+                fs.append(wf.CosGFunc(g * T_PER_M, lobe, gap, periods, ax))
+    return fs
+
+
+def c2_hex(n_walkers: int = 1000000, n_steps: int = 250000, dt: float = 2e-4) -> Workload:
+    sub = sb.hex_lattice(2.5, 6.0, 20, 25, D_in=1.7, D_ex=2.0, T2_in=70.0, T2_ex=50.0,
+                         kappa=sb.kappa_for_probability(1e-3, 1.7, dt), c=120.0)
+    acq = wf.build_acq(c2_functors(), n_steps, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=n_walkers, seed=1234, boundary_mode=abi.BOUNDARY_PERMEABLE,
+                      relaxation=1)
+    return Workload("c2_hex", sub, acq, run,
+                    "hex lattice 1000 cylinders R=2.5um s=6um, p=1e-3, T2 70/50 ms, D 1.7/2.0, "
+                    "cosine OGSE 8 freq x 8 amp x {x,z}")
+
+
+def fibonacci_directions(n: int) -> np.ndarray:
+    """n nearly uniform unit vectors (Fibonacci sphere)."""
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    r = np.sqrt(1.0 - z * z)
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def c3_functors(n_dirs: int = 30, amps=(0.1, 0.2, 0.3), total: float = 40.0):
+    fs = []
+    dirs = fibonacci_directions(n_dirs)
+    for k in range(16):
+        lobe = 4.0 + 1.0 * k                  # 4..19 ms lobes
+        gap = total - 2.0 * lobe - 1.0
+        for kind in range(4):
+            for d in dirs:
+                for g in amps:
+                    G = g * T_PER_M
+                    if kind == 0:
+                        fs.append(wf.RectGrad(G, lobe, gap, d))
+                    elif kind == 1:
+                        fs.append(wf.CosGFunc(G, lobe, gap, 2.0, d))
+                    elif kind == 2:
+                        fs.append(wf.SinGFunc(G, lobe, gap, 2.0, d))
+                    else:
+                        fs.append(wf.SWOGSEFunc(G, lobe, gap, 2.0, d))
+    return fs
+
+
+def c3_gamma(n_walkers: int = 4000000, n_steps: int = 80000, dt: float = 5e-4,
+             seed: int = 7) -> Workload:
+    sub = sb.gamma_packing(400, 5.0, 0.4, 0.65, D_in=1.7, D_ex=2.0, seed=seed, T2_in=70.0,
+                           T2_ex=50.0, rmin=0.25)
+    acq = wf.build_acq(c3_functors(), n_steps, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=n_walkers, seed=1234, relaxation=1)
+    return Workload("c3_gamma", sub, acq, run,
+                    "400 gamma-radius cylinders at 65% packing, 64 profiles x 30 dirs x 3 amps")

@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark: walker-steps/s of the gfx950 random-walk kernel on the
+hexagonal-lattice cosine-OGSE workload (1000 permeable cylinders, 1e6
+walkers, 250,000 steps; see axonexa_amd/workloads.py::c2_hex), with the
+CPU oracle timed beside it on the host cores.
+
+One bench step = one full walk (init + 250,000 time steps of every walker)
+plus the fp64 signal reduction, inputs resident on the device.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (spec sheet)
+PEAK_HBM_GBS = 8000.0
+
+
+def _cpu_threads() -> int:
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walkers", type=int, default=1000000)
+    ap.add_argument("--n-steps", type=int, default=250000, help="time steps per walk")
+    ap.add_argument("--cpu-walkers", type=int, default=4096,
+                    help="walkers in the CPU-baseline sample (full step count)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0: plan default)")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(the path shards by walker id, see DESIGN.md)"}))
+        return 2
+
+    import numpy as np
+
+    from axonexa_amd import abi, engine
+    from axonexa_amd import workloads as W
+
+    wl = W.c2_hex(n_walkers=args.walkers, n_steps=args.n_steps)
+    n = wl.acq.n_steps
+    plan = engine.Plan(wl.sub, wl.acq, wl.run)
+    if args.block:
+        plan.set_config(args.block)
+    cfg = plan.config()
+
+    for _ in range(args.warmup):
+        plan.walk(0, n).reduce().sync()
+
+    kernel_ms = []
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(plan.time_walk(0, n))
+        plan.reduce()
+        plan.sync()
+    t1 = time.perf_counter()
+    sig = plan.results()
+    elapsed = t1 - t0
+    ws = wl.walker_steps
+    value = ws * args.steps / elapsed
+    avg_ms = float(np.mean(kernel_ms))
+    rho = sig.collisions / ws
+    F = wl.flops_per_walker_step(rho)
+    achieved = F * ws / (avg_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "walker-steps/s",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "c2_hex: " + wl.description,
+            "walkers": wl.run.n_walkers,
+            "time_steps": n,
+            "dt_ms": wl.acq.dt,
+            "waveforms": wl.acq.n_waveforms,
+            "profiles": wl.acq.n_profiles,
+            "cylinders": wl.sub.n_prims,
+            "block": cfg["block"],
+            "bin_width_um": round(cfg["bin_width"], 4),
+            "lds_bytes": cfg["lds_bytes"],
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "flops_per_walker_step": F,
+            "walker_steps_per_launch": ws,
+            "kernel_ms": avg_ms,
+            "collisions_per_walker_step": rho,
+        },
+        "signal": {
+            "E_perp_lowfreq_maxG": float(sig.E[14]),
+            "E_par_lowfreq_maxG": float(sig.E[15]),
+            "transmissions": sig.transmissions,
+            "cap_hits": sig.cap_hits,
+        },
+    }
+
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        nw = min(args.cpu_walkers, args.walkers)
+        sample = W.c2_hex(n_walkers=nw, n_steps=args.n_steps)
+        threads = _cpu_threads()
+        tc = time.perf_counter()
+        ost = po.walk(sample.sub, sample.acq, sample.run, threads=threads,
+                      bin_width=cfg["bin_width"])
+        tcpu = time.perf_counter() - tc
+        cpu_rate = sample.walker_steps / tcpu
+        osig = po.reduce(sample.sub, sample.acq, sample.run, ost)
+        # the same walkers on the GPU: walker-by-walker parity of the sample
+        with engine.Plan(sample.sub, sample.acq, sample.run) as sp:
+            sp.walk(0, n).reduce()
+            gsig = sp.results()
+            gst = sp.get_state()
+        bit = np.all(gst.reshape(-1, nw) == ost.reshape(-1, nw), axis=0)
+        dE = np.abs(gsig.E_re - osig.E_re) / np.maximum(osig.sigma_re, 1e-300)
+        out["cpu_baseline"] = {
+            "value": cpu_rate,
+            "unit": "walker-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{nw} walkers x {n} steps of the same workload (walker ids 0..{nw - 1}), "
+                      f"oracle with the GPU's cell bins, {threads} threads on {_cpu_model()}",
+            "seconds": tcpu,
+        }
+        out["vs_cpu"] = value / cpu_rate
+        out["signal_error_vs_oracle"] = {
+            "sample_walkers": nw,
+            "bitwise_equal_walkers": float(bit.mean()),
+            "collisions_equal": bool(gsig.collisions == osig.collisions),
+            "transmissions_equal": bool(gsig.transmissions == osig.transmissions),
+            "max_abs_dE_over_sigma": float(np.max(dE)),
+        }
+    print(json.dumps(out))
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

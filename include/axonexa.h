@@ -185,8 +185,9 @@ int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
  * walk can be timed, chunked (checkpoint/resume) and inspected.
  *   create  : validate, build periodic images + cell bins, upload tables,
  *             allocate walker state.             (phaseAcquisitionStream.cuh:134-158)
- *   walk    : advance every walker over steps [step_begin, step_end); the
- *             first call must start at 0 and initialises walkers.
+ *   walk    : advance every walker over steps [step_begin, step_end);
+ *             step_begin is 0 (initialise the walkers; also restarts a
+ *             finished run) or the step where the previous call stopped.
  *                                                  (kernelLattice.cuh:1-104)
  *   reduce  : fp64 signal sums into device buffers  (reduction.cuh:76-195)
  *   results : copy sums to the host (synchronises)  (phaseAcquisitionStream.cuh:185-189)
