@@ -83,6 +83,10 @@ class Run(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("bin_width", C.c_double), ("lds_budget", C.c_int)]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("sum_cos", _dp), ("sum_sin", _dp), ("sum_cos2", _dp), ("sum_sin2", _dp),
@@ -109,9 +113,9 @@ class SubstrateSpec:
     types: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     centers: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     radii: np.ndarray = field(default_factory=lambda: np.zeros(0))
-    D: np.ndarray = field(default_factory=lambda: np.zeros(0))
-    T2: np.ndarray = field(default_factory=lambda: np.zeros(0))
-    kappa: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    D: np.ndarray = 0.0
+    T2: np.ndarray = float("inf")
+    kappa: np.ndarray = 0.0
 
     def __post_init__(self):
         self.types = np.ascontiguousarray(self.types, dtype=np.int32).reshape(-1)

@@ -58,8 +58,8 @@ struct axx_plan {
     float4* d_geom = nullptr;
     int* d_meta = nullptr;
     float4* d_cyl = nullptr;
-    uint32_t* d_bins = nullptr;
-    uint32_t* d_cand = nullptr;
+    uint4* d_bins = nullptr;
+    uint16_t* d_cand = nullptr;
     float* d_L = nullptr;
     float* d_rate = nullptr;
     float* d_thr_out = nullptr;
@@ -156,8 +156,8 @@ struct HostPrep {
     std::vector<int> orig;
     std::vector<float> L, rate, thr_out, thr_in;
     std::vector<float4> cyl;
-    std::vector<uint32_t> bins;
-    std::vector<uint32_t> cand;
+    std::vector<uint4> bins;
+    std::vector<uint16_t> cand;
     int nb[3] = {1, 1, 1};
     float ib[3] = {1, 1, 1};
     float ib4[2] = {4, 4};
@@ -232,7 +232,7 @@ static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], d
 
 static bool surface_near_box(const float4& g, bool sph, const double lo[3], const double hi[3], double reach) {
     const double c[3] = {g.x, g.y, g.z};
-    const double R = std::sqrt((double)g.w);
+    const double R = (double)g.w;
     const int dims = sph ? 3 : 2;
     double dmin2 = 0, dmax2 = 0;
     for (int a = 0; a < dims; ++a) {
@@ -247,10 +247,8 @@ static bool surface_near_box(const float4& g, bool sph, const double lo[3], cons
 }
 
 static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool sph) {
-    return n_img * (sph ? 20 : 16) + n_bins * 4 + n_cand * 4;
+    return n_bins * 16 + n_img * (sph ? 20 : 16) + n_cand * 2;
 }
-
-constexpr int kMaxRelevant = 4;   // candidate surfaces per sub-bin the kernel keeps
 
 // Images, compartments and bins.  The float32 image centres are
 // (float)(centre + shift) computed in double; compartments use
@@ -291,7 +289,7 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
                     const float Rf = (float)s->radius[k];
                     H.geom.push_back(make_float4((float)(s->center[3 * k] + sh[0]),
                                                  (float)(s->center[3 * k + 1] + sh[1]),
-                                                 (float)(s->center[3 * k + 2] + sh[2]), Rf * Rf));
+                                                 (float)(s->center[3 * k + 2] + sh[2]), Rf));
                     H.meta.push_back((k + 1) | (s->type[k] == AXX_PRIM_SPHERE ? kSphereBit : 0));
                     H.orig.push_back(k);
                 }
@@ -327,7 +325,7 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         const double bw[3] = {Ls[0] / nb[0], Ls[1] / nb[1], Ls[2] / nb[2]};
         for (int i = 0; i < n_img; ++i) {
             const bool sk = (H.meta[i] & kSphereBit) != 0;
-            const double R = std::sqrt((double)H.geom[i].w) + reach;
+            const double R = (double)H.geom[i].w + reach;
             const double c3[3] = {H.geom[i].x, H.geom[i].y, H.geom[i].z};
             int lo_b[3], hi_b[3];
             for (int a = 0; a < 3; ++a) {
@@ -352,26 +350,25 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
                         if (mask) lists[(bz * nb[1] + by) * nb[0] + bx].push_back((uint32_t)i | (mask << 16));
                     }
         }
-        std::vector<uint32_t> bins(nbins);
-        std::vector<uint32_t> cand;
-        bool ok = true, crowded = false;
+        // bin record: {offset << 8 | count, nibbles of sub-bins 0-7, 8-15, 0};
+        // nibble = position of the only surface within reach, 7 none, 8 several
+        std::vector<uint4> bins(nbins);
+        std::vector<uint16_t> cand;
+        bool ok = true;
         for (long long bi = 0; bi < nbins && ok; ++bi) {
             const size_t off = cand.size();
             const size_t cnt = lists[bi].size();
             if (cnt > 255 || off >= (1u << 24)) { ok = false; break; }
+            uint32_t nib[2] = {0, 0};
             for (int sb = 0; sb < 16; ++sb) {
-                int nrel = 0;
-                for (uint32_t e : lists[bi]) nrel += (e >> (16 + sb)) & 1u;
-                if (nrel > kMaxRelevant) crowded = true;
+                int nrel = 0, pos = 0;
+                for (size_t j = 0; j < cnt; ++j)
+                    if ((lists[bi][j] >> (16 + sb)) & 1u) { if (!nrel) pos = (int)j; ++nrel; }
+                uint32_t v = nrel == 0 ? kNibNone : (nrel == 1 && pos < 7 ? (uint32_t)pos : kNibMany);
+                nib[sb >> 3] |= v << ((sb & 7) * 4);
             }
-            cand.insert(cand.end(), lists[bi].begin(), lists[bi].end());
-            bins[bi] = (uint32_t)(off << 8) | (uint32_t)cnt;
-        }
-        if (crowded) {
-            if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "more than %d surfaces near one sub-bin", kMaxRelevant);
-            w *= 0.8;
-            if (w < Lmax) return fail(ctx, AXX_ERR_LIMIT, "surfaces too dense to bin");
-            continue;
+            for (size_t j = 0; j < cnt; ++j) cand.push_back((uint16_t)(lists[bi][j] & 0xFFFFu));
+            bins[bi] = make_uint4((uint32_t)(off << 8) | (uint32_t)cnt, nib[0], nib[1], 0u);
         }
         if (!ok || lds_size(n_img, nbins, cand.size(), H.sph) > lds_budget) {
             if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
@@ -438,6 +435,11 @@ static void plan_free(axx_plan* p) {
     delete p;
 }
 
+extern "C" int axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
+                               const axx_run* r, axx_plan** out) {
+    return axx_plan_create_ex(ctx, s, q, r, nullptr, out);
+}
+
 extern "C" void axx_plan_destroy(axx_plan* plan) {
     if (!plan) return;
     (void)hipSetDevice(plan->ctx->device);
@@ -446,7 +448,7 @@ extern "C" void axx_plan_destroy(axx_plan* plan) {
 }
 
 static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
-                            const axx_run* r, axx_plan* p) {
+                            const axx_run* r, const axx_tuning* tune, axx_plan* p) {
     int rc = validate(ctx, s, q);
     if (rc) return rc;
     if (r->n_walkers < 1) return fail(ctx, AXX_ERR_INVALID, "n_walkers must be >= 1");
@@ -459,8 +461,13 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
         return fail(ctx, AXX_ERR_INVALID, "init_arg: no such compartment");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HostPrep H;
-    const size_t budget = std::min<size_t>(ctx->lds_max, 96 * 1024);
-    rc = prepare(ctx, s, q, budget, 0.0, H);
+    size_t budget = 150 * 1024;
+    if (tune && tune->lds_budget > 0) budget = (size_t)tune->lds_budget;
+    budget = std::min<size_t>(budget, ctx->lds_max);
+    if (tune && tune->block && (tune->block < 64 || tune->block > 1024 || tune->block % 64))
+        return fail(ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+    if (tune && tune->bin_width < 0) return fail(ctx, AXX_ERR_INVALID, "bin_width < 0");
+    rc = prepare(ctx, s, q, budget, tune ? tune->bin_width : 0.0, H);
     if (rc) return rc;
     p->ctx = ctx;
     p->run = *r;
@@ -535,20 +542,20 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     if ((rc = dupload<unsigned long long>(ctx, &p->d_cnt, nullptr, 4))) return rc;
     HIP_TRY(ctx, hipEventCreate(&p->ev0));
     HIP_TRY(ctx, hipEventCreate(&p->ev1));
-    p->block = 512;
+    p->block = (tune && tune->block) ? tune->block : 1024;
     p->wpt = 1;
     HIP_TRY(ctx, hipDeviceSynchronize());
     return AXX_OK;
 }
 
-extern "C" int axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
-                               const axx_run* r, axx_plan** out) {
+extern "C" int axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
+                                  const axx_run* r, const axx_tuning* tune, axx_plan** out) {
     if (!ctx || !s || !q || !r || !out) return fail(ctx, AXX_ERR_INVALID, "null argument");
     *out = nullptr;
     axx_plan* p = new (std::nothrow) axx_plan();
     if (!p) return AXX_ERR_NOMEM;
     p->ctx = ctx;
-    int rc = plan_create_impl(ctx, s, q, r, p);
+    int rc = plan_create_impl(ctx, s, q, r, tune, p);
     if (rc) {
         plan_free(p);
         return rc;
@@ -563,7 +570,7 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
     A.H2c = p->H2c; A.H2s = p->H2s;
     A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
-    A.ibx = p->ib[0]; A.iby = p->ib[1]; A.ibz = p->ib[2];
+    A.ibz = p->ib[2];
     A.ibx4 = p->ib4[0]; A.iby4 = p->ib4[1];
     A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
     A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_cyl = p->d_cyl; A.g_bins = p->d_bins; A.g_cand = p->d_cand;

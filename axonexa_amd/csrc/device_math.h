@@ -151,41 +151,51 @@ __device__ __forceinline__ float sqrt_unit(float x) {
     return y;
 }
 
-// The crossing test of hit_test written as a branch-free decision followed
-// by one root evaluation for the lanes that hit (same operations and
-// operands, so the same results).
+// The crossing test of hit_test split in two: a branch-free decision (does
+// the segment cross this surface within [0, ell]?) and the root evaluation
+// for a surface that is crossed.  Same operations and operands as hit_test,
+// so the same results.
+struct Quad {
+    float A, B, Cp, fe, disc;
+    bool inside;
+};
+
 template <bool SPH>
-__device__ __forceinline__ bool cross(float ox, float oy, float oz, float ux, float uy, float uz,
-                                      float A, float R2, bool same_comp, float H2, float ell,
-                                      float& t_out, bool& ins_out) {
-    float B, q;
+__device__ __forceinline__ Quad quad_of(float ox, float oy, float oz, float ux, float uy, float uz,
+                                        float A, float R2, bool same_comp, float H2, float ell) {
+    Quad Q;
+    float q;
     if (SPH) {
-        B = fmaf(oz, uz, fmaf(oy, uy, ox * ux));
+        Q.B = fmaf(oz, uz, fmaf(oy, uy, ox * ux));
         q = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
     } else {
-        B = fmaf(oy, uy, ox * ux);
+        Q.B = fmaf(oy, uy, ox * ux);
         q = fmaf(oy, oy, ox * ox);
     }
     const float C = q - R2;
-    const bool inside = same_comp && q < H2;
-    ins_out = inside;
-    const float Cp = inside ? (C < 0.0f ? C : 0.0f) : (C > 0.0f ? C : 0.0f);
-    const float fe = fmaf(fmaf(A, ell, B + B), ell, Cp);
-    const float disc = fmaf(B, B, -(A * Cp));
-    const bool hit_in = fe > 0.0f;
-    const bool hit_out = B < 0.0f && (Cp == 0.0f || fe < 0.0f || ((-B) < A * ell && disc > 0.0f));
-    const bool hit = A > 0.0f && (inside ? hit_in : hit_out);
-    if (hit) {
-        const float sq = sqrtf(disc > 0.0f ? disc : 0.0f);
-        // inside, B < 0: (sq - B) / A;  inside, B >= 0: -Cp / (B + sq);
-        // outside: Cp / (sq - B)
-        const bool far_root = inside && B < 0.0f;
-        const float num = far_root ? sq - B : (inside ? -Cp : Cp);
-        const float den = far_root ? A : (inside ? B + sq : sq - B);
-        const float t = den > 0.0f ? num / den : 0.0f;
-        t_out = t < ell ? t : ell;
-    }
-    return hit;
+    Q.A = A;
+    Q.inside = same_comp && q < H2;
+    Q.Cp = Q.inside ? (C < 0.0f ? C : 0.0f) : (C > 0.0f ? C : 0.0f);
+    Q.fe = fmaf(fmaf(A, ell, Q.B + Q.B), ell, Q.Cp);
+    Q.disc = fmaf(Q.B, Q.B, -(A * Q.Cp));
+    return Q;
+}
+
+__device__ __forceinline__ bool quad_hits(const Quad& Q, float ell) {
+    const bool hit_in = Q.fe > 0.0f;
+    const bool hit_out = Q.B < 0.0f && (Q.Cp == 0.0f || Q.fe < 0.0f || ((-Q.B) < Q.A * ell && Q.disc > 0.0f));
+    return Q.A > 0.0f && (Q.inside ? hit_in : hit_out);
+}
+
+// inside, B < 0: (sq - B) / A;  inside, B >= 0: -Cp / (B + sq);
+// outside: Cp / (sq - B); clamped to ell
+__device__ __forceinline__ float quad_root(const Quad& Q, float ell) {
+    const float sq = sqrtf(Q.disc > 0.0f ? Q.disc : 0.0f);
+    const bool far_root = Q.inside && Q.B < 0.0f;
+    const float num = far_root ? sq - Q.B : (Q.inside ? -Q.Cp : Q.Cp);
+    const float den = far_root ? Q.A : (Q.inside ? Q.B + sq : sq - Q.B);
+    const float t = den > 0.0f ? num / den : 0.0f;
+    return t < ell ? t : ell;
 }
 
 }  // namespace axx

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(1024) void init_kernel(WalkArgs A, int n_mom) {
             } else {
                 float4 gk = geom[A.init_img];
                 bool sk = (meta[A.init_img] & kSphereBit) != 0;
-                float Rf = sqrtf(gk.w);
+                float Rf = gk.w;
                 float px = gk.x + (unif(o.x) * 2.0f - 1.0f) * Rf;
                 float py = gk.y + (unif(o.y) * 2.0f - 1.0f) * Rf;
                 float pz = gk.z + (unif(o.z) * 2.0f - 1.0f) * Rf;

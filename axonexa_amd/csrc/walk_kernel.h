@@ -23,14 +23,14 @@ struct WalkArgs {
     float a, b, c;
     float H2c, H2s;                         // squared half of the smallest cell side (xy; xyz)
     int nbx, nby, nbz;
-    float ibx, iby, ibz;                    // bins per unit length
+    float ibz;                              // z bins per unit length (spheres)
     float ibx4, iby4;                       // sub-bins (4 x 4 per bin) per unit length
     int n_img, n_bins, n_cand;
-    const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R^2
+    const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R
     const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit
-    const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R^2, meta bits (cylinder-only substrates)
-    const uint32_t* __restrict__ g_bins;    // [n_bins] offset << 8 | count
-    const uint32_t* __restrict__ g_cand;    // [n_cand] image index | sub-bin mask << 16, ascending per bin
+    const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
+    const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
+    const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
     const float* __restrict__ compL;        // [n_comp] step length
     const float* __restrict__ rate;         // [n_comp] dt / T2
     const float* __restrict__ thr_out;      // [n_prims] 4 kappa / v_inside
@@ -45,6 +45,10 @@ struct WalkArgs {
 
 constexpr int kSphereBit = 1 << 30;
 constexpr int kCompMask = (1 << 30) - 1;
+// Sub-bin nibble: 0..6 = position in the bin's list of the single surface
+// within reach, 7 = none, 8 (bit 3) = several (test the whole list).
+constexpr uint32_t kNibNone = 7u;
+constexpr uint32_t kNibMany = 8u;
 
 // state field indices (include/axonexa.h)
 enum {
@@ -52,13 +56,16 @@ enum {
     F_COMP, F_COMP0, F_LI, F_NCOLL, F_NTRAN, F_NCAP, F_STEP, F_MOM
 };
 
+// g = (cx, cy, cz, R); inside test against R^2 = R * R (cylinderXY.cuh:69-74,
+// Sphere.cuh:89-94)
 __device__ __forceinline__ bool inside_at(float4 g, bool sph, float x, float y, float z) {
+    const float R2 = g.w * g.w;
     float ox = x - g.x, oy = y - g.y;
     if (sph) {
         float oz = z - g.z;
-        return fmaf(oz, oz, fmaf(oy, oy, ox * ox)) < g.w;
+        return fmaf(oz, oz, fmaf(oy, oy, ox * ox)) < R2;
     }
-    return fmaf(oy, oy, ox * ox) < g.w;
+    return fmaf(oy, oy, ox * ox) < R2;
 }
 
 // Region of a point: compartment of the containing primitive (unique by the
@@ -72,22 +79,24 @@ __device__ __forceinline__ int region_of(const float4* geom, const int* meta, in
     return 0;
 }
 
-// Substrate tables in LDS.  Cylinder-only substrates keep one float4 per
-// image (cx, cy, R^2, meta); others keep (cx, cy, cz, R^2) plus meta.
+// Substrate tables in LDS: bins (16 B each), then one float4 per image
+// (cylinder-only: cx, cy, R, meta; otherwise cx, cy, cz, R plus a meta
+// array), then the u16 candidate lists.
 struct LdsTables {
+    uint4* bins;
     float4* geom;
     int* meta;
-    uint32_t* bins;
-    uint32_t* cand;
+    uint16_t* cand;
 };
 
 template <bool SPH>
 __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned char* lds) {
     LdsTables T;
-    T.geom = reinterpret_cast<float4*>(lds);
+    T.bins = reinterpret_cast<uint4*>(lds);
+    T.geom = reinterpret_cast<float4*>(T.bins + A.n_bins);
     T.meta = reinterpret_cast<int*>(T.geom + A.n_img);
-    T.bins = SPH ? reinterpret_cast<uint32_t*>(T.meta + A.n_img) : reinterpret_cast<uint32_t*>(T.geom + A.n_img);
-    T.cand = T.bins + A.n_bins;
+    T.cand = SPH ? reinterpret_cast<uint16_t*>(T.meta + A.n_img) : reinterpret_cast<uint16_t*>(T.meta);
+    for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
         if (SPH) {
             T.geom[i] = A.g_geom[i];
@@ -96,13 +105,12 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
             T.geom[i] = A.g_cyl[i];
         }
     }
-    for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
     for (int i = threadIdx.x; i < A.n_cand; i += blockDim.x) T.cand[i] = A.g_cand[i];
     __syncthreads();
     return T;
 }
 
-// Image k as (cx, cy, cz, R^2) and its meta word.
+// Image k as (cx, cy, cz, R) and its meta word.
 template <bool SPH>
 __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
     const float4 g = T.geom[k];
@@ -131,8 +139,6 @@ template <int P, int AX, bool SPH, bool REJ>
 __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsTables T = stage_tables<SPH>(A, lds);
-    const uint32_t* bins = T.bins;
-    const uint32_t* cand = T.cand;
 
     const long long N = A.N;
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,27 +183,17 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __r
         float ux = sth * cphi, uy = sth * sphi, uz = ct;
         float ell = Lc;
         float sx = x, sy = y, sz = z;
-        // candidate surfaces of the start bin, filtered by the walker's
-        // sub-bin: up to four image indices packed in two registers
+        // surfaces within reach of the start: the bin's list, narrowed by the
+        // sub-bin nibble to one entry (or the whole list when several)
         const int sbx = min(max((int)(x * A.ibx4), 0), 4 * A.nbx - 1);
         const int sby = min(max((int)(y * A.iby4), 0), 4 * A.nby - 1);
         const int bz = SPH ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
-        const uint32_t rec = bins[(bz * A.nby + (sby >> 2)) * A.nbx + (sbx >> 2)];
-        const int cnt = (int)(rec & 0xFFu);
-        const uint32_t* cl = cand + (rec >> 8);
-        const uint32_t sh = 16u + (uint32_t)((sbx & 3) | ((sby & 3) << 2));
-        uint32_t r01 = 0u, r23 = 0u;
-        int nrel = 0;
-        for (int j = 0; j < cnt; ++j) {
-            const uint32_t e = cl[j];
-            const uint32_t bit = (e >> sh) & 1u;
-            const uint32_t v = bit ? (e & 0xFFFFu) << ((nrel & 1) << 4) : 0u;
-            if (nrel < 2) r01 |= v; else r23 |= v;
-            nrel += (int)bit;
-        }
-        auto rel = [&](int r) -> int {
-            return (int)(((r < 2 ? r01 : r23) >> ((r & 1) << 4)) & 0xFFFFu);
-        };
+        const uint4 rec = T.bins[(bz * A.nby + (sby >> 2)) * A.nbx + (sbx >> 2)];
+        const int sbi = (sbx & 3) | ((sby & 3) << 2);
+        const uint32_t nib = ((sbi < 8 ? rec.y : rec.z) >> ((sbi & 7) << 2)) & 0xFu;
+        const bool many = (nib & kNibMany) != 0u;
+        const int nrel = many ? (int)(rec.x & 0xFFu) : (nib == kNibNone ? 0 : 1);
+        const uint16_t* cl = T.cand + (rec.x >> 8) + (many ? 0u : nib);
 
         if (REJ) {
             // simple rejection (kernelLattice.cuh:73-78)
@@ -205,7 +201,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __r
             bool rej = false;
             for (int r = 0; r < nrel; ++r) {
                 int mt;
-                const float4 g = image_at<SPH>(T, rel(r), mt);
+                const float4 g = image_at<SPH>(T, cl[r], mt);
                 const bool sk = SPH && (mt & kSphereBit);
                 rej = rej || (inside_at(g, sk, sx, sy, sz) != inside_at(g, sk, ex, ey, ez));
             }
@@ -221,21 +217,22 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __r
                 int bk = -1;
                 bool bins_ = false;
                 for (int r = 0; r < nrel; ++r) {
-                    const int k = rel(r);
+                    const int k = cl[r];
                     int mt;
                     const float4 g = image_at<SPH>(T, k, mt);
                     const bool sk = SPH && (mt & kSphereBit);
                     const bool same = comp == (mt & kCompMask);
-                    float t;
-                    bool ins;
-                    const bool hit = sk ? cross<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w, same,
-                                                      A.H2s, ell, t, ins)
-                                        : cross<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w, same,
-                                                       A.H2c, ell, t, ins);
-                    if (hit && t < best) {
-                        best = t;
-                        bk = k;
-                        bins_ = ins;
+                    const Quad Q = sk ? quad_of<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w * g.w,
+                                                      same, A.H2s, ell)
+                                      : quad_of<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w,
+                                                       same, A.H2c, ell);
+                    if (quad_hits(Q, ell)) {
+                        const float t = quad_root(Q, ell);
+                        if (t < best) {
+                            best = t;
+                            bk = k;
+                            bins_ = Q.inside;
+                        }
                     }
                 }
                 live = bk >= 0;
@@ -272,18 +269,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __r
                     ell = f * Lc;
                     ntran++;
                 } else {
-                    // specular reflection (boundaryCheck.cuh:149-167)
-                    float nx = px - g.x, ny = py - g.y, nz = 0.0f, kd;
-                    if (sk) {
-                        nz = pz - g.z;
-                        const float inv = 1.0f / sqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
-                        nx = nx * inv; ny = ny * inv; nz = nz * inv;
-                        kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
-                    } else {
-                        const float inv = 1.0f / sqrtf(fmaf(ny, ny, nx * nx));
-                        nx = nx * inv; ny = ny * inv;
-                        kd = fmaf(ny, uy, nx * ux);
-                    }
+                    // specular reflection about the surface normal (p - c) / R
+                    // (boundaryCheck.cuh:149-167, cylinderXY.cuh:125-131)
+                    const float inv = 1.0f / g.w;
+                    const float nx = (px - g.x) * inv, ny = (py - g.y) * inv;
+                    const float nz = sk ? (pz - g.z) * inv : 0.0f;
+                    const float kd = sk ? fmaf(nz, uz, fmaf(ny, uy, nx * ux)) : fmaf(ny, uy, nx * ux);
                     const float k2 = kd + kd;
                     ux = fmaf(-k2, nx, ux);
                     uy = fmaf(-k2, ny, uy);

@@ -54,6 +54,7 @@ def lib():
                     C.POINTER(abi.Result))
     L.axx_run_phase.argtypes = [vp, S, Q, R, Res]
     L.axx_plan_create.argtypes = [vp, S, Q, R, C.POINTER(vp)]
+    L.axx_plan_create_ex.argtypes = [vp, S, Q, R, C.POINTER(abi.Tuning), C.POINTER(vp)]
     L.axx_plan_walk.argtypes = [vp, C.c_int, C.c_int]
     L.axx_plan_reduce.argtypes = [vp]
     L.axx_plan_results.argtypes = [vp, Res]
@@ -120,14 +121,16 @@ class Plan:
     """Device-resident run: create -> walk (chunks) -> reduce -> results."""
 
     def __init__(self, sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
-                 ctx: Optional[Context] = None):
+                 ctx: Optional[Context] = None, block: int = 0, bin_width: float = 0.0,
+                 lds_budget: int = 0):
         self.ctx = ctx or default_context()
         self.sub, self.acq, self.run = sub, acq, run
         self._s, self._q, self._r = sub.struct(), acq.struct(), run.struct()
+        self._t = abi.Tuning(int(block), float(bin_width), int(lds_budget))
         self._h = C.c_void_p()
-        rc = lib().axx_plan_create(self.ctx.handle, C.byref(self._s), C.byref(self._q),
-                                   C.byref(self._r), C.byref(self._h))
-        self.ctx.check(rc, "axx_plan_create")
+        rc = lib().axx_plan_create_ex(self.ctx.handle, C.byref(self._s), C.byref(self._q),
+                                      C.byref(self._r), C.byref(self._t), C.byref(self._h))
+        self.ctx.check(rc, "axx_plan_create_ex")
 
     def walk(self, step_begin: int = 0, step_end: Optional[int] = None):
         if step_end is None:

@@ -132,7 +132,7 @@ def c3_functors(n_dirs: int = 30, amps=(0.1, 0.2, 0.3), total: float = 40.0):
 
 
 def c3_gamma(n_walkers: int = 4000000, n_steps: int = 80000, dt: float = 5e-4,
-             seed: int = 7) -> Workload:
+             seed: int = 4) -> Workload:
     sub = sb.gamma_packing(400, 5.0, 0.4, 0.65, D_in=1.7, D_ex=2.0, seed=seed, T2_in=70.0,
                            T2_ex=50.0, rmin=0.25)
     acq = wf.build_acq(c3_functors(), n_steps, dt, GAMMA)

@@ -195,6 +195,17 @@ int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
  */
 int  axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
                      const axx_run* run, axx_plan** out);
+
+/* Launch tuning (no reference counterpart: the reference takes blocks,
+ * threads and numOfSM from the driver, phaseAcquisitionStream.cuh:127-129).
+ * Zero fields select the defaults. */
+typedef struct axx_tuning {
+    int    block;       /* threads per block: multiple of 64 in [64, 1024]; 0 = 1024 */
+    double bin_width;   /* cell-bin edge length; 0 = automatic               */
+    int    lds_budget;  /* LDS bytes the substrate tables may use; 0 = 150 KiB */
+} axx_tuning;
+int  axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
+                        const axx_run* run, const axx_tuning* tune, axx_plan** out);
 int  axx_plan_walk(axx_plan* plan, int step_begin, int step_end);
 int  axx_plan_reduce(axx_plan* plan);
 int  axx_plan_results(axx_plan* plan, axx_result* out);

@@ -122,6 +122,7 @@ void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
  * lengths and T2 rates, per-primitive transmission thresholds. */
 typedef struct {
     float cx, cy, cz, R2;
+    float R;     /* radius (float), R2 = R * R */
     int type;    /* AXX_PRIM_* */
     int comp;    /* compartment id = original index + 1 */
     int orig;
@@ -275,6 +276,7 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                     p->cz = (float)(s->center[3 * k + 2] + sz);
                     float Rf = (float)s->radius[k];
                     p->R2 = Rf * Rf;
+                    p->R = Rf;
                     p->type = s->type[k];
                     p->comp = k + 1;
                     p->orig = k;
@@ -494,7 +496,7 @@ static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t k
             for (int i = 0; i < P->n_img; ++i)
                 if (P->img[i].orig == k) { pk = &P->img[i]; break; }
             if (!pk) return AXX_ERR_INVALID;
-            float Rf = sqrtf(pk->R2);
+            float Rf = pk->R;
             px = pk->cx + (unif(o[0]) * 2.0f - 1.0f) * Rf;
             py = pk->cy + (unif(o[1]) * 2.0f - 1.0f) * Rf;
             pz = pk->cz + (unif(o[2]) * 2.0f - 1.0f) * Rf;
@@ -620,11 +622,11 @@ static void* job_run(void* arg) {
                         float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
                         if (pk->type == AXX_PRIM_SPHERE) {
                             nz = pz - pk->cz;
-                            float inv = 1.0f / sqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
+                            float inv = 1.0f / pk->R;
                             nx = nx * inv; ny = ny * inv; nz = nz * inv;
                             kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
                         } else {
-                            float inv = 1.0f / sqrtf(fmaf(ny, ny, nx * nx));
+                            float inv = 1.0f / pk->R;
                             nx = nx * inv; ny = ny * inv;
                             kd = fmaf(ny, uy, nx * ux);
                         }
@@ -792,7 +794,7 @@ int orc_image_count(const axx_substrate* s, const axx_acq* q) {
 
 int orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
                  const float u3[3], float ell, float* t) {
-    oprim p = {cx, cy, cz, R * R, type, 1, 0};
+    oprim p = {cx, cy, cz, R * R, R, type, 1, 0};
     float Axy = fmaf(u3[1], u3[1], u3[0] * u3[0]);
     float A3 = fmaf(u3[2], u3[2], Axy);
     int ins;

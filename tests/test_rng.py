@@ -1,0 +1,88 @@
+"""Philox4x32-10, the uniform conversion and the step direction of the oracle.
+
+Pins: the Random123 known-answer vectors (Salmon et al., SC'11), rocRAND's
+header-only engine (fixture tests/golden/philox_rocrand.txt, generated by
+tests/golden/gen_philox_rocrand.cpp), closed-form properties of the uniform
+and of sin/cos of 2*pi*u.  Replaces curand XORWOW + acos/sin/cos of
+src/kernels/kernelLattice.cuh:52-57.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import pyoracle as po
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Random123 kat_vectors, philox4x32 10 rounds: ctr[4], key[2] -> out[4]
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", KAT)
+def test_philox_random123_kat(ctr, key, out):
+    assert list(po.philox(ctr, key)) == out
+
+
+def test_philox_matches_rocrand_fixture():
+    rows = [l.split() for l in open(os.path.join(HERE, "golden", "philox_rocrand.txt")) if l.strip()]
+    assert len(rows) >= 6
+    for r in rows:
+        v = [int(x, 16) for x in r]
+        assert list(po.philox(v[:4], v[4:6])) == v[6:], r
+
+
+def test_uniform_exact_and_open_interval():
+    # u = ((w >> 8) | 1) * 2^-24: odd multiples of 2^-24 in (0, 1)
+    for w in (0, 1, 255, 256, 0x7FFFFFFF, 0xFFFFFFFF, 0x80000000):
+        u = po.unif(w)
+        k = ((w >> 8) | 1)
+        assert u == k * 2.0 ** -24
+        assert 0.0 < u < 1.0
+    assert po.unif(0) == 2.0 ** -24
+    assert po.unif(0xFFFFFFFF) == 1.0 - 2.0 ** -24
+
+
+def test_sincos_turn_accuracy_and_quadrants():
+    rng = np.random.default_rng(0)
+    ws = list(rng.integers(0, 2 ** 32, 4000, dtype=np.uint64))
+    # quadrant boundaries and extremes
+    ws += [0, 0xFFFFFFFF, 1 << 30, (1 << 30) - 1, 1 << 31, 3 << 30, (3 << 30) + 255]
+    worst = 0.0
+    for w in ws:
+        w = int(w)
+        s, c = po.sincos_turn(w)
+        u = ((w >> 8) | 1) * 2.0 ** -24
+        phi = 2.0 * math.pi * u
+        worst = max(worst, abs(s - math.sin(phi)), abs(c - math.cos(phi)))
+        assert abs(s * s + c * c - 1.0) < 5e-7
+    # float32 rounding of values of magnitude <= 1 plus polynomial truncation
+    assert worst < 2.5e-7
+
+
+def test_direction_distribution_is_isotropic():
+    # cos(theta) = 2u - 1 must be uniform on (-1, 1) and phi uniform on (0, 2 pi):
+    # draw the step words exactly as the walk does (pair counter, words x/y)
+    n = 20000
+    cts, phis = [], []
+    for wid in range(n // 2):
+        o = po.philox([wid, 0, 0, 0], [1234, 0])
+        for wa, wb in ((o[0], o[1]), (o[2], o[3])):
+            cts.append(po.unif(int(wb)) * 2.0 - 1.0)
+            s, c = po.sincos_turn(int(wa))
+            phis.append(math.atan2(s, c) % (2 * math.pi))
+    cts, phis = np.array(cts), np.array(phis)
+    assert stats.kstest(cts, "uniform", args=(-1, 2)).pvalue > 1e-3
+    assert stats.kstest(phis, "uniform", args=(0, 2 * math.pi)).pvalue > 1e-3
+    # mean of the unit step is 0 and E[ux^2] = 1/3
+    st = np.sqrt(np.maximum(1 - cts ** 2, 0))
+    ux = st * np.cos(phis)
+    assert abs(ux.mean()) < 4 / math.sqrt(n)
+    assert abs((ux ** 2).mean() - 1 / 3) < 4 * math.sqrt(4 / 45 / n)
