@@ -64,6 +64,23 @@ def build_lib(verbose: bool = False) -> str:
     return lib
 
 
+DRIVERS = ["cos_onecyl", "gammadist"]
+
+
+def build_drivers() -> list:
+    """Compile the ported example drivers (C++ over include/axonexa_compat.hpp)."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = []
+    for name in DRIVERS:
+        exe = os.path.join(bindir, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+              os.path.join(HERE, "drivers", name + ".cpp"), "-o", exe,
+              "-L", HERE, "-laxonexa", "-Wl,-rpath,$ORIGIN/.."])
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
@@ -72,7 +89,8 @@ def build_oracle() -> str:
 def main(argv=None):
     t = time.time()
     path = build_lib()
-    print(f"built {path} in {time.time() - t:.1f}s")
+    build_drivers()
+    print(f"built {path} and drivers in {time.time() - t:.1f}s")
 
 
 if __name__ == "__main__":

@@ -585,7 +585,7 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
     A.err = p->d_err;
     walk_fn fn = pick_walk(p->Pt, p->ax, p->sph, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
-    const int block = p->block;
+    const int block = std::min(p->block, walk_max_block(p->Pt));
     const long long grid = (p->N + block - 1) / block;
     if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");
     if (i0 == 0) {

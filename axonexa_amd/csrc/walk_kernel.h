@@ -135,8 +135,12 @@ walk_fn pick_walk_p16(int ax, bool sph, bool rej);
 walk_fn pick_walk_p32(int ax, bool sph, bool rej);
 walk_fn pick_walk_p64(int ax, bool sph, bool rej);
 
+// Wide profile sets keep up to 192 moments per lane: those instantiations are
+// limited to 512 threads per block so that each lane may use 256 VGPRs.
+constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
+
 template <int P, int AX, bool SPH, bool REJ>
-__global__ __launch_bounds__(1024) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
+__global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsTables T = stage_tables<SPH>(A, lds);
 

@@ -1,0 +1,515 @@
+// axonexa_compat.hpp — the reference's C++ driver interface over the C ABI.
+//
+// Drivers written against the reference's headers
+//   Vector3 (src/util/vector3.cuh), Cylinder_XY (src/primitives/cylinderXY.cuh),
+//   Sphere (src/primitives/Sphere.cuh), Lattice (src/substrate/lattice.cuh),
+//   RectGrad / CosGFunc / SinGFunc / ApodCosGFunc / SWOGSEFunc
+//   (src/functors/gfunctors.cuh), phaseAcquisition<F>
+//   (src/acquisition/phaseAcquisition.cuh), phaseAcquisitionStream<F>
+//   (src/acquisition/phaseAcquisitionStream.cuh), RPSLatticeInitializer<B>
+//   (src/substrate/RPSinitializer.h)
+// compile against this header instead: same class names, constructors and
+// methods.  run*() methods sample the functors on the host at t_i = i*dt,
+// factor them into (unit profile, amplitude, direction), and call
+// axx_run_phase (include/axonexa.h) on device 0; blocks/threads/numOfSM are
+// accepted and ignored (the library picks its launch shape).
+//
+// As in the reference, the driver defines `real` (typedef) and may define
+// GAMMA, PI, SPECULAR_REFLECTION, USE_PERMEABLE, USE_RELAXATION and
+// USE_INITIALIZE_IN_REGION before including this header.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "axonexa.h"
+
+#ifndef GAMMA
+#define GAMMA 267500.0
+#endif
+#ifndef PI
+#define PI 3.1415926535897932384626433832795
+#endif
+
+// ---------------------------------------------------------------- Vector3
+class Vector3 {
+  public:
+    real x, y, z;
+    Vector3() : x(0), y(0), z(0) {}
+    Vector3(real x_, real y_, real z_) : x(x_), y(y_), z(z_) {}
+    real magnitude() const { return std::sqrt(x * x + y * y + z * z); }
+    real squareMagnitude() const { return x * x + y * y + z * z; }
+    void normalize() {
+        const real l = magnitude();
+        x /= l; y /= l; z /= l;
+    }
+    void invert() { x = -x; y = -y; z = -z; }
+    Vector3 operator+(const Vector3& o) const { return Vector3(x + o.x, y + o.y, z + o.z); }
+    Vector3 operator-(const Vector3& o) const { return Vector3(x - o.x, y - o.y, z - o.z); }
+    Vector3 operator*(real s) const { return Vector3(x * s, y * s, z * s); }
+    Vector3 operator/(real s) const { return Vector3(x / s, y / s, z / s); }
+    real operator*(const Vector3& o) const { return x * o.x + y * o.y + z * o.z; }      // dot
+    Vector3 operator%(const Vector3& o) const {                                        // cross
+        return Vector3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x);
+    }
+    void operator+=(const Vector3& o) { x += o.x; y += o.y; z += o.z; }
+    void operator-=(const Vector3& o) { x -= o.x; y -= o.y; z -= o.z; }
+    void operator*=(real s) { x *= s; y *= s; z *= s; }
+};
+inline std::ostream& operator<<(std::ostream& os, const Vector3& v) {
+    return os << v.x << " " << v.y << " " << v.z;
+}
+
+// -------------------------------------------------------------- primitives
+namespace axx_compat {
+struct Prim {
+    int type = AXX_PRIM_CYLINDER_XY;
+    real cx = 0, cy = 0, cz = 0, R = 0, D = 0, T2 = 0, T1 = 0, kappa = 0, eps = 0;
+    int region = 0;
+};
+}  // namespace axx_compat
+
+class Cylinder_XY {
+  public:
+    axx_compat::Prim p;
+    Cylinder_XY() {}
+    Cylinder_XY(real cx, real cy, real R, real T2, real T1, real D, int region, real perm, real eps = 0) {
+        p.type = AXX_PRIM_CYLINDER_XY;
+        p.cx = cx; p.cy = cy; p.R = R; p.T2 = T2; p.T1 = T1; p.D = D;
+        p.region = region; p.kappa = perm; p.eps = eps;
+    }
+    bool inside(const Vector3& r) const {
+        return (r.x - p.cx) * (r.x - p.cx) + (r.y - p.cy) * (r.y - p.cy) < p.R * p.R;
+    }
+    Vector3 getCenter() const { return Vector3(p.cx, p.cy, 0.0); }
+    void setCenter(Vector3 c) { p.cx = c.x; p.cy = c.y; }
+    real getRadius() const { return p.R; }
+    void setRadius(real r) { p.R = r; }
+    void setEPS(real e) { p.eps = e; }
+    void setRegion(int r) { p.region = r; }
+    int getRegion() const { return p.region; }
+    real getD() const { return p.D; }
+    real getT2() const { return p.T2; }
+    real getT1() const { return p.T1; }
+    real getPermeability() const { return p.kappa; }
+};
+
+class Sphere {
+  public:
+    axx_compat::Prim p;
+    Sphere() { p.type = AXX_PRIM_SPHERE; }
+    Sphere(real cx, real cy, real cz, real R, real T2, real T1, real D, int region, real perm, real eps = 0) {
+        p.type = AXX_PRIM_SPHERE;
+        p.cx = cx; p.cy = cy; p.cz = cz; p.R = R; p.T2 = T2; p.T1 = T1; p.D = D;
+        p.region = region; p.kappa = perm; p.eps = eps;
+    }
+    bool inside(const Vector3& r) const {
+        return (r - getCenter()).squareMagnitude() < p.R * p.R;
+    }
+    Vector3 getCenter() const { return Vector3(p.cx, p.cy, p.cz); }
+    void setCenter(Vector3 c) { p.cx = c.x; p.cy = c.y; p.cz = c.z; }
+    real getRadius() const { return p.R; }
+    void setRadius(real r) { p.R = r; }
+    void setEPS(real e) { p.eps = e; }
+    void setRegion(int r) { p.region = r; }
+    int getRegion() const { return p.region; }
+    real getD() const { return p.D; }
+    real getT2() const { return p.T2; }
+    real getPermeability() const { return p.kappa; }
+};
+
+class Lattice {
+  public:
+    real a = 0, b = 0, c = 0, T2 = 0, T1 = 0, D = 0;
+    int basisSize = 0;
+    Lattice() {}
+    Lattice(real a_, real b_, real c_, real T2_, real T1_, real D_, int n)
+        : a(a_), b(b_), c(c_), T2(T2_), T1(T1_), D(D_), basisSize(n) {}
+    real getA() const { return a; }
+    real getB() const { return b; }
+    real getC() const { return c; }
+    int getBasisSize() const { return basisSize; }
+    void setBasisSize(int n) { basisSize = n; }
+    bool inLatticeCell(const Vector3& r) const {
+        return !(r.x > a || r.x < 0 || r.y > b || r.y < 0 || r.z > c || r.z < 0);
+    }
+};
+
+// --------------------------------------------------------------- functors
+// Two-lobe waveform G(t) = G m(t) ghat on [0, T), -G m(t - gap - T) ghat on
+// [gap + T, 2T + gap) (gfunctors.cuh:36-54, 94-107).
+namespace axx_compat {
+class TwoLobe {
+  public:
+    real G = 0, T = 0, gap = 0, N = 1;
+    Vector3 ghat = Vector3(1, 0, 0);
+    int kind = 0;
+    virtual ~TwoLobe() {}
+    virtual double mod(double t) const = 0;
+    double unit(double t) const {
+        if (t < T) return mod(t);
+        if (t >= gap + T && t < 2.0 * T + gap) return -mod(t - gap - T);
+        return 0.0;
+    }
+    Vector3 operator()(real t) const {
+        return ghat * (G * (real)unit(t));
+    }
+    std::tuple<int, double, double, double> key() const { return std::make_tuple(kind, (double)T, (double)gap, (double)N); }
+    real getG() const { return G; }
+};
+}  // namespace axx_compat
+
+class RectGrad : public axx_compat::TwoLobe {
+  public:
+    RectGrad() { kind = 0; }
+    RectGrad(real g, real gd, real gs) { kind = 0; G = g; T = gd; gap = gs; }
+    // b-value form (gfunctors.cuh:14-23)
+    RectGrad(real bval, real gd, real gs, Vector3& gh) {
+        kind = 0; T = gd; gap = gs; ghat = gh;
+        G = std::sqrt(bval / (gd * gd * GAMMA * GAMMA * ((2.0 / 3.0) * gd + gs)));
+    }
+    double mod(double) const override { return 1.0; }
+    real getB() const { return G * G * T * T * GAMMA * GAMMA * ((2.0 / 3.0) * T + gap); }
+    real getDiffusionTime() const { return gap + T; }
+    real getTau() const { return T + gap * 0.5; }
+    real getFreq() const { return 0; }
+};
+
+class CosGFunc : public axx_compat::TwoLobe {
+  public:
+    CosGFunc() { kind = 1; }
+    CosGFunc(real g, real gd, real gs, real nper, const Vector3& gh) {
+        kind = 1; G = g; T = gd; gap = gs; N = nper; ghat = gh;
+    }
+    double mod(double t) const override { return std::cos(2.0 * PI * t * N / T); }
+    real getFreq() const { return N / T; }
+    real getB() const { return GAMMA * GAMMA * 0.25 * G * G * T * T * T / (PI * PI * N * N); }
+};
+
+class SinGFunc : public axx_compat::TwoLobe {
+  public:
+    SinGFunc() { kind = 2; }
+    SinGFunc(real g, real gd, real gs, real nper, const Vector3& gh) {
+        kind = 2; G = g; T = gd; gap = gs; N = nper; ghat = gh;
+    }
+    double mod(double t) const override { return std::sin(2.0 * PI * t * N / T); }
+    real getFreq() const { return N / T; }
+    real getB() const { return GAMMA * GAMMA * 0.75 * G * G * T * T * T / (PI * PI * N * N); }
+};
+
+class ApodCosGFunc : public axx_compat::TwoLobe {
+  public:
+    ApodCosGFunc() { kind = 3; }
+    ApodCosGFunc(real g, real gd, real gs, real nper, const Vector3& gh) {
+        kind = 3; G = g; T = gd; gap = gs; N = nper; ghat = gh;
+    }
+    double mod(double t) const override {
+        const double period = T / N;
+        if (t >= period / 4.0 && t < T - period / 4.0) return std::cos(2.0 * PI * t / period);
+        return std::sin(4.0 * PI * t / period);
+    }
+    real getFreq() const { return N / T; }
+    real getB() const {
+        return GAMMA * GAMMA * 0.25 * G * G * T * T * T / (PI * PI * N * N) * (1.0 - 1.0 / (8.0 * N));
+    }
+};
+
+class SWOGSEFunc : public axx_compat::TwoLobe {
+  public:
+    SWOGSEFunc() { kind = 4; }
+    SWOGSEFunc(real g, real gd, real gs, real nper, const Vector3& gh) {
+        kind = 4; G = g; T = gd; gap = gs; N = nper; ghat = gh;
+    }
+    // (-1)^floor(2 v t); the reference's pow(-1, 2 v t) is NaN off the grid
+    double mod(double t) const override {
+        const long long k = (long long)std::floor(2.0 * (N / T) * t);
+        return (k % 2 == 0) ? 1.0 : -1.0;
+    }
+    real getFreq() const { return N / T; }
+    real getB() const { const double v = N / T; return G * G * GAMMA * GAMMA * T / (6.0 * v * v); }
+};
+
+// ------------------------------------------------------------ acquisitions
+inline double compat_linear_regression(const std::vector<real>& y, const std::vector<real>& x) {
+    double n = (double)y.size(), sxy = 0, sx = 0, sy = 0, sxx = 0;
+    for (size_t i = 0; i < y.size(); ++i) {
+        sxy += x[i] * y[i]; sx += x[i]; sy += y[i]; sxx += x[i] * x[i];
+    }
+    return (n * sxy - sx * sy) / (n * sxx - sx * sx);
+}
+
+template <class F>
+class phaseAcquisition {
+  public:
+    int numOfMeasurements = 0, numOfSteps = 0, seed = 0, last = 0, points = 0;
+    std::vector<real> mx, my;
+    std::vector<F> functors;
+    real ADC = 0, ADCmeansquare = 0;
+
+    phaseAcquisition() {}
+    phaseAcquisition(int nMeas, int nSteps, int nParticles, int seed_)
+        : numOfMeasurements(nMeas), numOfSteps(nSteps), seed(seed_), mx(nMeas, 0), my(nMeas, 0), functors(nMeas) {
+        (void)nParticles;
+    }
+    void addMeasurement(F& f) { functors[last++] = f; }
+    std::vector<F>& getGradientFunctors() { return functors; }
+    std::vector<real>& getMx() { return mx; }
+    std::vector<real>& getMy() { return my; }
+    real getSignal(int i) { return std::sqrt(mx[i] * mx[i] + my[i] * my[i]); }
+    int getNumOfMeas() { return numOfMeasurements; }
+    int getNumOfSteps() { return numOfSteps; }
+    int& getSeed() { return seed; }
+    // ADC from a linear fit of ln|S| against b (phaseAcquisition.cuh:54-77)
+    void calcADC() {
+        points++;
+        std::vector<real> ln, b;
+        for (int i = 0; i < numOfMeasurements; ++i) {
+            b.push_back(functors[i].getB());
+            ln.push_back(std::log(getSignal(i)));
+        }
+        const real a = -compat_linear_regression(ln, b);
+        ADC = (ADC * (points - 1) + a) / points;
+        ADCmeansquare = (ADCmeansquare * (points - 1) + a * a) / points;
+    }
+    real& getADC() { return ADC; }
+    real getADCError() { return std::sqrt(std::max<real>(ADCmeansquare - ADC * ADC, 0)); }
+};
+
+namespace axx_compat {
+inline axx_ctx* context() {
+    static axx_ctx* ctx = nullptr;
+    if (!ctx) {
+        const int rc = axx_create(&ctx, 0);
+        if (rc) throw std::runtime_error(std::string("axx_create: ") + axx_strerror(rc));
+    }
+    return ctx;
+}
+
+template <class F>
+struct Sampled {
+    std::vector<float> prof;
+    std::vector<int> wf_p;
+    std::vector<double> amp, dir;
+    int P = 0;
+};
+
+template <class F>
+Sampled<F> sample(std::vector<F>& fs, int n, double dt) {
+    Sampled<F> S;
+    std::vector<std::tuple<int, double, double, double>> keys;
+    std::vector<const F*> reps;
+    for (auto& f : fs) {
+        auto k = f.key();
+        auto it = std::find(keys.begin(), keys.end(), k);
+        int idx = (int)(it - keys.begin());
+        if (it == keys.end()) { keys.push_back(k); reps.push_back(&f); }
+        S.wf_p.push_back(idx);
+        S.amp.push_back(f.G);
+        const double nrm = std::sqrt((double)(f.ghat * f.ghat));
+        S.dir.push_back(f.ghat.x / nrm); S.dir.push_back(f.ghat.y / nrm); S.dir.push_back(f.ghat.z / nrm);
+    }
+    S.P = (int)reps.size();
+    S.prof.resize((size_t)n * S.P);
+    for (int i = 0; i < n; ++i)
+        for (int p = 0; p < S.P; ++p) S.prof[(size_t)i * S.P + p] = (float)reps[p]->unit(dt * i);
+    return S;
+}
+
+inline void check(int rc, const char* what) {
+    if (rc) {
+        std::string msg = std::string(what) + ": " + axx_strerror(rc) + " — " + axx_last_error(context());
+        throw std::runtime_error(msg);
+    }
+}
+}  // namespace axx_compat
+
+template <class F>
+class phaseAcquisitionStream {
+  public:
+    int number_of_particles = 0;
+    std::vector<phaseAcquisition<F>> acqSet;
+    phaseAcquisitionStream() {}
+    explicit phaseAcquisitionStream(int n) : number_of_particles(n) {}
+    void addAcquisition(phaseAcquisition<F> a) { acqSet.push_back(a); }
+    phaseAcquisition<F>& getAcquisition(int i) { return acqSet[i]; }
+
+    // Periodic lattice of basis[0 .. lat.getBasisSize()) (phaseAcquisitionStream.cuh:126-190)
+    template <class B>
+    void runAcquisitionLattice(int num, B* basis, Lattice& lat, real dt, int blocks, int threads, int numOfSM) {
+        (void)blocks; (void)threads; (void)numOfSM;
+        std::vector<axx_compat::Prim> prims;
+        for (int i = 0; i < lat.getBasisSize(); ++i) prims.push_back(basis[i].p);
+        run(num, prims, lat, dt, init_mode(), init_arg());
+    }
+    // Single pore: walkers start inside basis (phaseAcquisitionStream.cuh:37-88);
+    // the pore sits in a cell three radii wider than it.
+    template <class B>
+    void runAcquisition(int num, B& basis, real dt, int blocks, int threads, int numOfSM) {
+        (void)blocks; (void)threads; (void)numOfSM;
+        axx_compat::Prim p = basis.p;
+        const real L = 6.0 * p.R;
+        p.cx = L / 2; p.cy = L / 2; p.cz = L / 2;
+        Lattice lat(L, L, L, p.T2, p.T1, p.D, 1);
+        run(num, std::vector<axx_compat::Prim>{p}, lat, dt, AXX_INIT_INSIDE_PRIM, 0);
+    }
+    // All acquisitions, one device (phaseAcquisitionStream.cuh:425-613)
+    template <class B>
+    void runAcquisitionStreamLattice(B* basis, Lattice& lat, real dt, int blocks, int threads, int numOfDevices,
+                                     std::vector<int>& plan, std::vector<int>& numOfSMPerDevice) {
+        (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
+        for (int i = 0; i < (int)acqSet.size(); ++i) runAcquisitionLattice(i, basis, lat, dt, blocks, threads, 0);
+    }
+    void changeSeeds(long s) {
+        for (auto& a : acqSet) a.getSeed() = (int)s;
+    }
+    void calcEveryADC() { for (auto& a : acqSet) a.calcADC(); }
+    void flushADC() { for (auto& a : acqSet) { a.ADC = 0; a.ADCmeansquare = 0; a.points = 0; } }
+
+  private:
+    static int init_mode() {
+#if defined USE_INITIALIZE_IN_REGION
+#if USE_INITIALIZE_IN_REGION == 0
+        return AXX_INIT_COMPARTMENT;
+#else
+        return AXX_INIT_INSIDE_PRIM;
+#endif
+#else
+        return AXX_INIT_UNIFORM;
+#endif
+    }
+    static int init_arg() {
+#if defined USE_INITIALIZE_IN_REGION && USE_INITIALIZE_IN_REGION > 0
+        return USE_INITIALIZE_IN_REGION - 1;
+#else
+        return 0;
+#endif
+    }
+    void run(int num, const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode, int iarg) {
+        phaseAcquisition<F>& acq = acqSet[num];
+        const int n = acq.getNumOfSteps();
+        auto S = axx_compat::sample(acq.getGradientFunctors(), n, (double)dt);
+        std::vector<int> type;
+        std::vector<double> center, radius, D, T2, kappa;
+        for (auto& p : prims) {
+            type.push_back(p.type);
+            center.push_back(p.cx); center.push_back(p.cy); center.push_back(p.cz);
+            radius.push_back(p.R); D.push_back(p.D); T2.push_back(p.T2); kappa.push_back(p.kappa);
+        }
+        axx_substrate s{};
+        s.a = lat.a; s.b = lat.b; s.c = lat.c; s.D_lat = lat.D; s.T2_lat = lat.T2;
+        s.n_prims = (int)prims.size();
+        s.type = type.data(); s.center = center.data(); s.radius = radius.data();
+        s.D = D.data(); s.T2 = T2.data(); s.kappa = kappa.data();
+        axx_acq q{};
+        q.n_steps = n; q.dt = dt; q.gamma = GAMMA;
+        q.n_profiles = S.P; q.profiles = S.prof.data();
+        q.n_waveforms = (int)S.wf_p.size(); q.wf_profile = S.wf_p.data();
+        q.wf_amplitude = S.amp.data(); q.wf_dir = S.dir.data();
+        axx_run r{};
+        r.n_walkers = number_of_particles;
+        r.seed = (uint64_t)(uint32_t)acq.getSeed();
+        r.instance = (uint32_t)num;
+        r.init_mode = imode; r.init_arg = iarg;
+#if defined USE_PERMEABLE
+        r.boundary_mode = AXX_BOUNDARY_PERMEABLE;
+#elif defined SPECULAR_REFLECTION
+        r.boundary_mode = AXX_BOUNDARY_SPECULAR;
+#else
+        r.boundary_mode = AXX_BOUNDARY_REJECTION;
+#endif
+#if defined USE_RELAXATION
+        r.relaxation = 1;
+#endif
+        const int M = q.n_waveforms;
+        std::vector<double> mx(M), my(M);
+        axx_result out{};
+        out.sum_cos = mx.data(); out.sum_sin = my.data();
+        axx_compat::check(axx_run_phase(axx_compat::context(), &s, &q, &r, &out), "axx_run_phase");
+        for (int j = 0; j < M; ++j) { acq.getMx()[j] = (real)mx[j]; acq.getMy()[j] = (real)my[j]; }
+    }
+};
+
+// ------------------------------------------------- random packing (host)
+// RPSLatticeInitializer<B> (src/substrate/RPSinitializer.h): radii from a
+// distribution, random sequential placement from the largest down with the
+// full minimum-image overlap test.  The engine adds periodic images itself,
+// so correctEdges() keeps one copy per primitive.
+template <class B>
+class RPSLatticeInitializer {
+  public:
+    std::vector<B> basis;
+    Lattice lat;
+    int basisType = 0;
+    bool verbose = false;
+    RPSLatticeInitializer(Lattice& l, int type, bool v = false) : basis(l.getBasisSize()), lat(l), basisType(type), verbose(v) {}
+    void uniformRadialDist(int seed, real maxr, real minr) {
+        std::mt19937_64 g((uint64_t)seed);
+        std::uniform_real_distribution<double> u(minr, maxr);
+        std::vector<real> r(basis.size());
+        for (auto& x : r) x = u(g);
+        assign(r);
+    }
+    void gaussianRadialDist(int seed, double mean, double sd) {
+        std::mt19937_64 g((uint64_t)seed);
+        std::normal_distribution<double> nd(mean, sd);
+        std::vector<real> r;
+        while (r.size() < basis.size()) { double x = nd(g); if (x > 0) r.push_back(x); }
+        assign(r);
+    }
+    // alpha = shape k, beta = rate 1/theta (RPSinitializer.h:76-101)
+    void gammaRadialDist(int seed, double alpha, double beta, real minimum, real maximum) {
+        std::mt19937_64 g((uint64_t)seed);
+        std::gamma_distribution<double> gd(alpha, 1.0 / beta);
+        std::vector<real> r;
+        while (r.size() < basis.size()) { double x = gd(g); if (x > minimum && x < maximum) r.push_back(x); }
+        assign(r);
+    }
+    bool uniformCenterDist(int seed, int triesMax) {
+        std::mt19937_64 g((uint64_t)seed);
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        const int n = (int)basis.size();
+        for (int i = n - 1; i >= 0; --i) {
+            int tries = 0;
+            do {
+                if (tries++ > triesMax * (n - i)) return false;
+                basis[i].setCenter(Vector3(u(g) * lat.getA(), u(g) * lat.getB(), basisType == 1 ? u(g) * lat.getC() : 0.0));
+            } while (overlaps(i));
+        }
+        return true;
+    }
+    void setRegions() { for (size_t i = 0; i < basis.size(); ++i) basis[i].setRegion((int)i); }
+    void correctEdges() { lat.setBasisSize((int)basis.size()); }
+    real getVI() {
+        real v = 0;
+        for (auto& b : basis) v += PI * b.getRadius() * b.getRadius();
+        return v / (lat.getA() * lat.getB());
+    }
+
+  private:
+    void assign(std::vector<real> r) {
+        std::sort(r.begin(), r.end());
+        for (size_t i = 0; i < basis.size(); ++i) basis[i].setRadius(r[i]);
+    }
+    bool overlaps(int i) const {
+        const Vector3 ci = basis[i].getCenter();
+        for (size_t j = i + 1; j < basis.size(); ++j) {
+            Vector3 d = ci - basis[j].getCenter();
+            d.x -= lat.getA() * std::nearbyint(d.x / lat.getA());
+            d.y -= lat.getB() * std::nearbyint(d.y / lat.getB());
+            if (basisType == 1) d.z -= lat.getC() * std::nearbyint(d.z / lat.getC());
+            else d.z = 0;
+            const real rr = basis[i].getRadius() + basis[j].getRadius();
+            if (d.squareMagnitude() < rr * rr) return true;
+        }
+        return false;
+    }
+};
