@@ -31,6 +31,7 @@ ERRORS = {
 
 PRIM_CYLINDER_XY = 0
 PRIM_SPHERE = 1
+PRIM_PLANE = 2
 
 BOUNDARY_SPECULAR = 0
 BOUNDARY_PERMEABLE = 1
@@ -62,7 +63,7 @@ class Substrate(C.Structure):
         ("D_lat", C.c_double), ("T2_lat", C.c_double),
         ("n_prims", C.c_int),
         ("type", _ip), ("center", _dp), ("radius", _dp),
-        ("D", _dp), ("T2", _dp), ("kappa", _dp),
+        ("D", _dp), ("T2", _dp), ("kappa", _dp), ("normal", _dp),
     ]
 
 
@@ -116,6 +117,7 @@ class SubstrateSpec:
     D: np.ndarray = 0.0
     T2: np.ndarray = float("inf")
     kappa: np.ndarray = 0.0
+    normals: Optional[np.ndarray] = None   # [n, 3] for planes
 
     def __post_init__(self):
         self.types = np.ascontiguousarray(self.types, dtype=np.int32).reshape(-1)
@@ -125,6 +127,9 @@ class SubstrateSpec:
         self.D = np.ascontiguousarray(np.broadcast_to(self.D, (n,)), dtype=np.float64)
         self.T2 = np.ascontiguousarray(np.broadcast_to(self.T2, (n,)), dtype=np.float64)
         self.kappa = np.ascontiguousarray(np.broadcast_to(self.kappa, (n,)), dtype=np.float64)
+        if self.normals is None:
+            self.normals = np.zeros((n, 3))
+        self.normals = np.ascontiguousarray(self.normals, dtype=np.float64).reshape(n, 3)
 
     @property
     def n_prims(self) -> int:
@@ -141,6 +146,7 @@ class SubstrateSpec:
         s.D = _ptr(self.D, C.c_double)
         s.T2 = _ptr(self.T2, C.c_double)
         s.kappa = _ptr(self.kappa, C.c_double)
+        s.normal = _ptr(self.normals, C.c_double)
         s._keep = self  # keep arrays alive with the struct
         return s
 

@@ -38,7 +38,7 @@ struct axx_plan {
     axx_ctx* ctx = nullptr;
     // host copies
     int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7;
-    bool sph = false;
+    bool gen = false;       // spheres or planes present: general kernel path
     long long N = 0;
     axx_run run{};
     double dt = 0;
@@ -76,7 +76,7 @@ struct axx_plan {
     unsigned long long* d_cnt = nullptr;
     int n_chunks = 0, chunk = 0;
     float cell[3] = {0, 0, 0};
-    float H2c = 0, H2s = 0;
+    float H2c = 0, H2s = 0, plane_eps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -163,8 +163,26 @@ struct HostPrep {
     float ib4[2] = {4, 4};
     double bin_width = 0;
     double Lmax = 0;
-    bool sph = false;
+    float plane_eps = 0;
+    bool gen = false;
 };
+
+// Unit normal of plane k (normalised in double).
+static void unit_normal(const axx_substrate* s, int k, double n[3]) {
+    const double* v = s->normal + 3 * k;
+    const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int a = 0; a < 3; ++a) n[a] = v[a] / l;
+}
+
+// [min, max] of n . p over the box [lo, hi].
+static void box_proj(const double n[3], const double lo[3], const double hi[3], double& pmin, double& pmax) {
+    pmin = pmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        const double u = n[a] * lo[a], v = n[a] * hi[a];
+        pmin += std::min(u, v);
+        pmax += std::max(u, v);
+    }
+}
 
 static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     if (!(s->a > 0 && s->b > 0 && s->c > 0)) return fail(ctx, AXX_ERR_INVALID, "cell lengths must be > 0");
@@ -186,6 +204,13 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     if (!(s->D_lat >= 0)) return fail(ctx, AXX_ERR_INVALID, "D_lat < 0");
     double Lmax = std::sqrt(6.0 * s->D_lat * q->dt);
     for (int k = 0; k < s->n_prims; ++k) {
+        if (s->type[k] == AXX_PRIM_PLANE) {
+            if (!s->normal) return fail(ctx, AXX_ERR_INVALID, "plane normals missing");
+            const double* v = s->normal + 3 * k;
+            if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0) || !std::isfinite(v[0] + v[1] + v[2]))
+                return fail(ctx, AXX_ERR_INVALID, "primitive %d: plane normal", k);
+            continue;
+        }
         if (s->type[k] != AXX_PRIM_CYLINDER_XY && s->type[k] != AXX_PRIM_SPHERE)
             return fail(ctx, AXX_ERR_INVALID, "primitive %d: unknown type", k);
         if (!(s->radius[k] > 0) || !(s->D[k] >= 0))
@@ -202,6 +227,7 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     const double Ls[3] = {s->a, s->b, s->c};
     for (int i = 0; i < s->n_prims; ++i)
         for (int j = i + 1; j < s->n_prims; ++j) {
+            if (s->type[i] == AXX_PRIM_PLANE || s->type[j] == AXX_PRIM_PLANE) continue;
             const int dims = (s->type[i] == AXX_PRIM_SPHERE && s->type[j] == AXX_PRIM_SPHERE) ? 3 : 2;
             double d2 = 0;
             for (int a = 0; a < dims; ++a) {
@@ -217,6 +243,14 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
 
 static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], double reach) {
     const double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
+    if (s->type[k] == AXX_PRIM_PLANE) {
+        double n[3], pmin, pmax;
+        unit_normal(s, k, n);
+        double d = 0;
+        for (int a = 0; a < 3; ++a) d += n[a] * (s->center[3 * k + a] + sh[a]);
+        box_proj(n, lo, hi, pmin, pmax);
+        return d >= pmin - reach && d <= pmax + reach;
+    }
     const int dims = s->type[k] == AXX_PRIM_SPHERE ? 3 : 2;
     const double R = s->radius[k] + reach;
     double d2 = 0;
@@ -230,7 +264,14 @@ static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], d
     return d2 <= R * R;
 }
 
-static bool surface_near_box(const float4& g, bool sph, const double lo[3], const double hi[3], double reach) {
+static bool surface_near_box(const float4& g, int meta, const double lo[3], const double hi[3], double reach) {
+    if (meta & kPlaneBit) {
+        const double n[3] = {g.x, g.y, g.z};
+        double pmin, pmax;
+        box_proj(n, lo, hi, pmin, pmax);
+        return (double)g.w >= pmin - reach && (double)g.w <= pmax + reach;
+    }
+    const bool sph = (meta & kSphereBit) != 0;
     const double c[3] = {g.x, g.y, g.z};
     const double R = (double)g.w;
     const int dims = sph ? 3 : 2;
@@ -246,8 +287,8 @@ static bool surface_near_box(const float4& g, bool sph, const double lo[3], cons
     return std::sqrt(dmin2) <= R + reach && std::sqrt(dmax2) >= R - reach;
 }
 
-static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool sph) {
-    return n_bins * 16 + n_img * (sph ? 20 : 16) + n_cand * 2;
+static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
+    return n_bins * 16 + n_img * (gen ? 20 : 16) + n_cand * 2;
 }
 
 // Images, compartments and bins.  The float32 image centres are
@@ -266,7 +307,17 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
     H.L[0] = (float)std::sqrt(6.0 * s->D_lat * dt);
     H.rate[0] = (s->T2_lat > 0 && std::isfinite(s->T2_lat)) ? (float)(dt / s->T2_lat) : 0.0f;
     const double v_lat = std::sqrt(6.0 * s->D_lat / dt);
+    H.plane_eps = (float)(64.0 * std::ldexp(1.0, -23) * std::max(std::max(s->a, s->b), s->c));
+    bool any_plane = false;
     for (int k = 0; k < n; ++k) {
+        if (s->type[k] == AXX_PRIM_PLANE) {
+            // planes bound no compartment
+            H.L[k + 1] = 0.0f;
+            H.rate[k + 1] = 0.0f;
+            any_plane = true;
+            H.gen = true;
+            continue;
+        }
         const double L = std::sqrt(6.0 * s->D[k] * dt);
         Lmax = std::max(Lmax, L);
         H.L[k + 1] = (float)L;
@@ -275,17 +326,35 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         const double v_in = std::sqrt(6.0 * s->D[k] / dt);
         H.thr_out[k] = kap > 0 ? (float)(4.0 * kap / v_in) : 0.0f;
         H.thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
-        if (s->type[k] == AXX_PRIM_SPHERE) H.sph = true;
+        if (s->type[k] == AXX_PRIM_SPHERE) H.gen = true;
     }
     H.Lmax = Lmax;
-    const double reach = Lmax * 1.001 + 1e-6 * std::max(s->a, s->b);
+    // a plane counts as hit within plane_eps of it, so its reach is longer
+    const double reach = Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) + (any_plane ? 2.0 * H.plane_eps : 0.0);
     for (int k = 0; k < n; ++k) {
-        const int zr = s->type[k] == AXX_PRIM_SPHERE ? 1 : 0;
+        const int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
+        const size_t first = H.geom.size();
         for (int kz = -zr; kz <= zr; ++kz)
             for (int ky = -1; ky <= 1; ++ky)
                 for (int kx = -1; kx <= 1; ++kx) {
                     const double sh[3] = {kx * s->a, ky * s->b, kz * s->c};
                     if (!reach_hits_cell(s, k, sh, reach)) continue;
+                    if (s->type[k] == AXX_PRIM_PLANE) {
+                        // shifts along the plane give the same plane: one image
+                        // per distinct float offset
+                        double nn[3];
+                        unit_normal(s, k, nn);
+                        const float dd = (float)(nn[0] * (s->center[3 * k] + sh[0]) +
+                                                 nn[1] * (s->center[3 * k + 1] + sh[1]) +
+                                                 nn[2] * (s->center[3 * k + 2] + sh[2]));
+                        bool dup = false;
+                        for (size_t q2 = first; q2 < H.geom.size(); ++q2) dup |= H.geom[q2].w == dd;
+                        if (dup) continue;
+                        H.geom.push_back(make_float4((float)nn[0], (float)nn[1], (float)nn[2], dd));
+                        H.meta.push_back((k + 1) | kPlaneBit);
+                        H.orig.push_back(k);
+                        continue;
+                    }
                     const float Rf = (float)s->radius[k];
                     H.geom.push_back(make_float4((float)(s->center[3 * k] + sh[0]),
                                                  (float)(s->center[3 * k + 1] + sh[1]),
@@ -312,9 +381,9 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
             int v = (int)std::floor(Ls[a] / w);
             nb[a] = std::min(std::max(v, 1), 2048);
         }
-        if (!H.sph) nb[2] = 1;
+        if (!H.gen) nb[2] = 1;
         const long long nbins = (long long)nb[0] * nb[1] * nb[2];
-        if (nbins > (1 << 22) || lds_size(n_img, nbins, 0, H.sph) > lds_budget) {
+        if (nbins > (1 << 22) || lds_size(n_img, nbins, 0, H.gen) > lds_budget) {
             if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
             w *= 1.25;
             continue;
@@ -324,12 +393,13 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         std::vector<std::vector<uint32_t>> lists(nbins);
         const double bw[3] = {Ls[0] / nb[0], Ls[1] / nb[1], Ls[2] / nb[2]};
         for (int i = 0; i < n_img; ++i) {
-            const bool sk = (H.meta[i] & kSphereBit) != 0;
+            const int mi = H.meta[i];
+            const bool sk = (mi & kSphereBit) != 0, pk = (mi & kPlaneBit) != 0;
             const double R = (double)H.geom[i].w + reach;
             const double c3[3] = {H.geom[i].x, H.geom[i].y, H.geom[i].z};
             int lo_b[3], hi_b[3];
             for (int a = 0; a < 3; ++a) {
-                if (a == 2 && !sk) { lo_b[a] = 0; hi_b[a] = nb[a] - 1; continue; }
+                if (pk || (a == 2 && !sk)) { lo_b[a] = 0; hi_b[a] = nb[a] - 1; continue; }
                 lo_b[a] = std::max(0, (int)std::floor((c3[a] - R) / bw[a]) - 1);
                 hi_b[a] = std::min(nb[a] - 1, (int)std::floor((c3[a] + R) / bw[a]) + 1);
             }
@@ -338,14 +408,14 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
                     for (int bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
                         const double lo[3] = {bx * bw[0], by * bw[1], bz * bw[2]};
                         const double hi[3] = {(bx + 1) * bw[0], (by + 1) * bw[1], (bz + 1) * bw[2]};
-                        if (!surface_near_box(H.geom[i], sk, lo, hi, reach)) continue;
+                        if (!surface_near_box(H.geom[i], mi, lo, hi, reach)) continue;
                         uint32_t mask = 0;
                         for (int ly = 0; ly < 4; ++ly)
                             for (int lx = 0; lx < 4; ++lx) {
                                 const double slo[3] = {lo[0] + lx * bw[0] / 4, lo[1] + ly * bw[1] / 4, lo[2]};
                                 const double shi[3] = {lo[0] + (lx + 1) * bw[0] / 4, lo[1] + (ly + 1) * bw[1] / 4,
                                                        hi[2]};
-                                if (surface_near_box(H.geom[i], sk, slo, shi, reach)) mask |= 1u << (lx + 4 * ly);
+                                if (surface_near_box(H.geom[i], mi, slo, shi, reach)) mask |= 1u << (lx + 4 * ly);
                             }
                         if (mask) lists[(bz * nb[1] + by) * nb[0] + bx].push_back((uint32_t)i | (mask << 16));
                     }
@@ -370,7 +440,7 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
             for (size_t j = 0; j < cnt; ++j) cand.push_back((uint16_t)(lists[bi][j] & 0xFFFFu));
             bins[bi] = make_uint4((uint32_t)(off << 8) | (uint32_t)cnt, nib[0], nib[1], 0u);
         }
-        if (!ok || lds_size(n_img, nbins, cand.size(), H.sph) > lds_budget) {
+        if (!ok || lds_size(n_img, nbins, cand.size(), H.gen) > lds_budget) {
             if (bin_width_req > 0) return fail(ctx, AXX_ERR_LIMIT, "bin tables exceed LDS");
             if (!ok) { w *= 0.8; if (w < Lmax) return fail(ctx, AXX_ERR_LIMIT, "cannot bin substrate"); continue; }
             w *= 1.25;
@@ -392,17 +462,17 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
 }
 
 // ---------------------------------------------------------------------------
-// Kernel dispatch over (profile width, axes, sphere) instantiations
+// Kernel dispatch over (profile width, axes, general geometry, rejection) instantiations
 
-static walk_fn pick_walk(int Pt, int ax, bool sph, bool rej) {
+static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej) {
     switch (Pt) {
-    case 1: return pick_walk_p1(ax, sph, rej);
-    case 2: return pick_walk_p2(ax, sph, rej);
-    case 4: return pick_walk_p4(ax, sph, rej);
-    case 8: return pick_walk_p8(ax, sph, rej);
-    case 16: return pick_walk_p16(ax, sph, rej);
-    case 32: return pick_walk_p32(ax, sph, rej);
-    default: return pick_walk_p64(ax, sph, rej);
+    case 1: return pick_walk_p1(ax, gen, rej);
+    case 2: return pick_walk_p2(ax, gen, rej);
+    case 4: return pick_walk_p4(ax, gen, rej);
+    case 8: return pick_walk_p8(ax, gen, rej);
+    case 16: return pick_walk_p16(ax, gen, rej);
+    case 32: return pick_walk_p32(ax, gen, rej);
+    default: return pick_walk_p64(ax, gen, rej);
     }
 }
 
@@ -459,6 +529,9 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
         return fail(ctx, AXX_ERR_INVALID, "init_arg: no such primitive");
     if (r->init_mode == AXX_INIT_COMPARTMENT && (r->init_arg < 0 || r->init_arg > s->n_prims))
         return fail(ctx, AXX_ERR_INVALID, "init_arg: no such compartment");
+    if ((r->init_mode == AXX_INIT_INSIDE_PRIM && s->type[r->init_arg] == AXX_PRIM_PLANE) ||
+        (r->init_mode == AXX_INIT_COMPARTMENT && r->init_arg > 0 && s->type[r->init_arg - 1] == AXX_PRIM_PLANE))
+        return fail(ctx, AXX_ERR_INVALID, "init_arg: planes bound no region");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HostPrep H;
     size_t budget = 150 * 1024;
@@ -479,7 +552,8 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     while (p->Pt < p->P) p->Pt *= 2;
     p->M = q->n_waveforms;
     p->ax = axes_mask(q);
-    p->sph = H.sph;
+    p->gen = H.gen;
+    p->plane_eps = H.plane_eps;
     p->cell[0] = (float)s->a; p->cell[1] = (float)s->b; p->cell[2] = (float)s->c;
     {
         const double hc = 0.5 * std::min(s->a, s->b), hs = 0.5 * std::min(std::min(s->a, s->b), s->c);
@@ -492,7 +566,7 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     for (int a = 0; a < 3; ++a) { p->nb[a] = H.nb[a]; p->ib[a] = H.ib[a]; }
     p->ib4[0] = H.ib4[0]; p->ib4[1] = H.ib4[1];
     p->bin_width = H.bin_width;
-    p->lds_bytes = lds_size(p->n_img, p->n_bins, p->n_cand, p->sph);
+    p->lds_bytes = lds_size(p->n_img, p->n_bins, p->n_cand, p->gen);
     p->lds_bytes = (p->lds_bytes + 15) & ~(size_t)15;
     p->init_lds = ((size_t)p->n_img * 20 + 15) & ~(size_t)15;
     if (p->init_lds > ctx->lds_max) return fail(ctx, AXX_ERR_LIMIT, "too many primitive images for LDS");
@@ -568,7 +642,7 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     axx_ctx* ctx = p->ctx;
     WalkArgs A{};
     A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
-    A.H2c = p->H2c; A.H2s = p->H2s;
+    A.H2c = p->H2c; A.H2s = p->H2s; A.plane_eps = p->plane_eps;
     A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
     A.ibz = p->ib[2];
     A.ibx4 = p->ib4[0]; A.iby4 = p->ib4[1];
@@ -584,7 +658,7 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.i0 = i0; A.i1 = i1;
     A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
     A.err = p->d_err;
-    walk_fn fn = pick_walk(p->Pt, p->ax, p->sph, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
+    walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
     const int block = std::min(p->block, walk_max_block(p->Pt));
     const long long grid = (p->N + block - 1) / block;
     if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");

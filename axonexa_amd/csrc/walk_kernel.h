@@ -22,12 +22,13 @@ namespace axx {
 struct WalkArgs {
     float a, b, c;
     float H2c, H2s;                         // squared half of the smallest cell side (xy; xyz)
+    float plane_eps;                        // planes count as hit within this distance
     int nbx, nby, nbz;
     float ibz;                              // z bins per unit length (spheres)
     float ibx4, iby4;                       // sub-bins (4 x 4 per bin) per unit length
     int n_img, n_bins, n_cand;
-    const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R
-    const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit
+    const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R (planes: nx, ny, nz, d)
+    const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit | kPlaneBit
     const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
     const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
     const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
@@ -44,7 +45,8 @@ struct WalkArgs {
 };
 
 constexpr int kSphereBit = 1 << 30;
-constexpr int kCompMask = (1 << 30) - 1;
+constexpr int kPlaneBit = 1 << 29;
+constexpr int kCompMask = (1 << 29) - 1;
 // Sub-bin nibble: 0..6 = position in the bin's list of the single surface
 // within reach, 7 = none, 8 (bit 3) = several (test the whole list).
 constexpr uint32_t kNibNone = 7u;
@@ -69,14 +71,40 @@ __device__ __forceinline__ bool inside_at(float4 g, bool sph, float x, float y, 
 }
 
 // Region of a point: compartment of the containing primitive (unique by the
-// non-overlap rule), else the lattice (lattice.cuh:134-148).
+// non-overlap rule), else the lattice (lattice.cuh:134-148).  Planes bound
+// no region (plane.cuh:38).
 __device__ __forceinline__ int region_of(const float4* geom, const int* meta, int n_img, float x,
                                          float y, float z) {
     for (int i = 0; i < n_img; ++i) {
         int mt = meta[i];
+        if (mt & kPlaneBit) continue;
         if (inside_at(geom[i], (mt & kSphereBit) != 0, x, y, z)) return mt & kCompMask;
     }
     return 0;
+}
+
+// Plane g = (nx, ny, nz, d), unit normal n and offset n . point
+// (plane.cuh:18-28 restated).  The walker's side is the side of the step's
+// start point (x0, y0, z0); the segment start (sx, sy, sz) is clamped onto
+// it.  The plane counts as hit when the segment comes closer than eps to it;
+// t is the time of reaching the plane, at most ell.  Together with the push
+// after a plane reflection this keeps every step end about eps or more away
+// on the walker's side.
+__device__ __forceinline__ bool plane_hit(float4 g, float x0, float y0, float z0, float sx, float sy,
+                                          float sz, float ux, float uy, float uz, float ell, float eps,
+                                          float& t) {
+    const float un = fmaf(g.z, uz, fmaf(g.y, uy, g.x * ux));
+    const float sd = fmaf(g.z, sz, fmaf(g.y, sy, g.x * sx)) - g.w;
+    const float s0 = fmaf(g.z, z0, fmaf(g.y, y0, g.x * x0)) - g.w;
+    const float sg = s0 >= 0.0f ? 1.0f : -1.0f;
+    float dd = sg * sd;
+    dd = dd > 0.0f ? dd : 0.0f;
+    const float v = -(sg * un);
+    if (!(v > 0.0f)) return false;
+    if (!(fmaf(-ell, v, dd) < eps)) return false;
+    const float tt = dd / v;
+    t = tt < ell ? tt : ell;
+    return true;
 }
 
 // Substrate tables in LDS: bins (16 B each), then one float4 per image
@@ -89,16 +117,16 @@ struct LdsTables {
     uint16_t* cand;
 };
 
-template <bool SPH>
+template <bool GEN>
 __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned char* lds) {
     LdsTables T;
     T.bins = reinterpret_cast<uint4*>(lds);
     T.geom = reinterpret_cast<float4*>(T.bins + A.n_bins);
     T.meta = reinterpret_cast<int*>(T.geom + A.n_img);
-    T.cand = SPH ? reinterpret_cast<uint16_t*>(T.meta + A.n_img) : reinterpret_cast<uint16_t*>(T.meta);
+    T.cand = GEN ? reinterpret_cast<uint16_t*>(T.meta + A.n_img) : reinterpret_cast<uint16_t*>(T.meta);
     for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
-        if (SPH) {
+        if (GEN) {
             T.geom[i] = A.g_geom[i];
             T.meta[i] = A.g_meta[i];
         } else {
@@ -110,11 +138,11 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
     return T;
 }
 
-// Image k as (cx, cy, cz, R) and its meta word.
-template <bool SPH>
+// Image k as (cx, cy, cz, R) or (nx, ny, nz, d) and its meta word.
+template <bool GEN>
 __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
     const float4 g = T.geom[k];
-    if (SPH) {
+    if (GEN) {
         mt = T.meta[k];
         return g;
     }
@@ -127,22 +155,24 @@ __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
 typedef const __attribute__((address_space(4))) float* const_fptr;
 
 typedef void (*walk_fn)(WalkArgs, const float*);
-walk_fn pick_walk_p1(int ax, bool sph, bool rej);
-walk_fn pick_walk_p2(int ax, bool sph, bool rej);
-walk_fn pick_walk_p4(int ax, bool sph, bool rej);
-walk_fn pick_walk_p8(int ax, bool sph, bool rej);
-walk_fn pick_walk_p16(int ax, bool sph, bool rej);
-walk_fn pick_walk_p32(int ax, bool sph, bool rej);
-walk_fn pick_walk_p64(int ax, bool sph, bool rej);
+walk_fn pick_walk_p1(int ax, bool gen, bool rej);
+walk_fn pick_walk_p2(int ax, bool gen, bool rej);
+walk_fn pick_walk_p4(int ax, bool gen, bool rej);
+walk_fn pick_walk_p8(int ax, bool gen, bool rej);
+walk_fn pick_walk_p16(int ax, bool gen, bool rej);
+walk_fn pick_walk_p32(int ax, bool gen, bool rej);
+walk_fn pick_walk_p64(int ax, bool gen, bool rej);
 
 // Wide profile sets keep up to 192 moments per lane: those instantiations are
 // limited to 512 threads per block so that each lane may use 256 VGPRs.
 constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 
-template <int P, int AX, bool SPH, bool REJ>
+// GEN: substrates with spheres or planes (3D bins, per-image meta words);
+// otherwise cylinders only.
+template <int P, int AX, bool GEN, bool REJ>
 __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const LdsTables T = stage_tables<SPH>(A, lds);
+    const LdsTables T = stage_tables<GEN>(A, lds);
 
     const long long N = A.N;
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -191,7 +221,7 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
         // sub-bin nibble to one entry (or the whole list when several)
         const int sbx = min(max((int)(x * A.ibx4), 0), 4 * A.nbx - 1);
         const int sby = min(max((int)(y * A.iby4), 0), 4 * A.nby - 1);
-        const int bz = SPH ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
+        const int bz = GEN ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
         const uint4 rec = T.bins[(bz * A.nby + (sby >> 2)) * A.nbx + (sbx >> 2)];
         const int sbi = (sbx & 3) | ((sby & 3) << 2);
         const uint32_t nib = ((sbi < 8 ? rec.y : rec.z) >> ((sbi & 7) << 2)) & 0xFu;
@@ -199,15 +229,21 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
         const int nrel = many ? (int)(rec.x & 0xFFu) : (nib == kNibNone ? 0 : 1);
         const uint16_t* cl = T.cand + (rec.x >> 8) + (many ? 0u : nib);
 
+        int lastpl = -1;     // image of the plane of the step's last reflection
         if (REJ) {
-            // simple rejection (kernelLattice.cuh:73-78)
+            // simple rejection (kernelLattice.cuh:73-78); a plane rejects the
+            // step that would come within plane_eps of it
             const float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
             bool rej = false;
             for (int r = 0; r < nrel; ++r) {
                 int mt;
-                const float4 g = image_at<SPH>(T, cl[r], mt);
-                const bool sk = SPH && (mt & kSphereBit);
-                rej = rej || (inside_at(g, sk, sx, sy, sz) != inside_at(g, sk, ex, ey, ez));
+                const float4 g = image_at<GEN>(T, cl[r], mt);
+                const bool sk = GEN && (mt & kSphereBit);
+                float tp;
+                const bool cross = (GEN && (mt & kPlaneBit))
+                                       ? plane_hit(g, x, y, z, sx, sy, sz, ux, uy, uz, ell, A.plane_eps, tp)
+                                       : inside_at(g, sk, sx, sy, sz) != inside_at(g, sk, ex, ey, ez);
+                rej = rej || cross;
             }
             if (rej) { ncoll++; ell = 0.0f; }
         } else {
@@ -223,8 +259,17 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                 for (int r = 0; r < nrel; ++r) {
                     const int k = cl[r];
                     int mt;
-                    const float4 g = image_at<SPH>(T, k, mt);
-                    const bool sk = SPH && (mt & kSphereBit);
+                    const float4 g = image_at<GEN>(T, k, mt);
+                    if (GEN && (mt & kPlaneBit)) {
+                        float t;
+                        if (plane_hit(g, x, y, z, sx, sy, sz, ux, uy, uz, ell, A.plane_eps, t) && t < best) {
+                            best = t;
+                            bk = k;
+                            bins_ = false;
+                        }
+                        continue;
+                    }
+                    const bool sk = GEN && (mt & kSphereBit);
                     const bool same = comp == (mt & kCompMask);
                     const Quad Q = sk ? quad_of<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w * g.w,
                                                       same, A.H2s, ell)
@@ -242,15 +287,16 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                 live = bk >= 0;
                 if (!live) break;
                 int mt;
-                const float4 g = image_at<SPH>(T, bk, mt);
-                const bool sk = SPH && (mt & kSphereBit);
+                const float4 g = image_at<GEN>(T, bk, mt);
+                const bool sk = GEN && (mt & kSphereBit);
+                const bool pl = GEN && (mt & kPlaneBit);
                 const int pcomp = mt & kCompMask;
                 const float t = best;
                 ncoll++;
                 const float px = fmaf(t, ux, sx), py = fmaf(t, uy, sy), pz = fmaf(t, uz, sz);
                 const bool ins = bins_;
                 bool transmit = false;
-                if (permeable) {
+                if (permeable && !pl) {     // planes never transmit (plane.cuh:42)
                     // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
                     const u32x4 oc = philox10(wid, (uint32_t)i, kStreamColl | ((uint32_t)kc >> 2), A.inst,
                                               A.k0, A.k1);
@@ -260,6 +306,7 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                     transmit = unif(wv) <= thr;
                     kc++;
                 }
+                lastpl = pl ? bk : -1;
                 if (transmit) {
                     // continue on the far side with its speed (boundaryCheck.cuh:211-215)
                     const int nc = ins ? 0 : pcomp;
@@ -273,12 +320,20 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                     ell = f * Lc;
                     ntran++;
                 } else {
-                    // specular reflection about the surface normal (p - c) / R
-                    // (boundaryCheck.cuh:149-167, cylinderXY.cuh:125-131)
-                    const float inv = 1.0f / g.w;
-                    const float nx = (px - g.x) * inv, ny = (py - g.y) * inv;
-                    const float nz = sk ? (pz - g.z) * inv : 0.0f;
-                    const float kd = sk ? fmaf(nz, uz, fmaf(ny, uy, nx * ux)) : fmaf(ny, uy, nx * ux);
+                    // specular reflection about the surface normal (p - c) / R, or
+                    // the plane's normal (boundaryCheck.cuh:149-167,
+                    // cylinderXY.cuh:125-131, plane.cuh:32)
+                    float nx, ny, nz, kd;
+                    if (pl) {
+                        nx = g.x; ny = g.y; nz = g.z;
+                        kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
+                    } else {
+                        const float inv = 1.0f / g.w;
+                        nx = (px - g.x) * inv;
+                        ny = (py - g.y) * inv;
+                        nz = sk ? (pz - g.z) * inv : 0.0f;
+                        kd = sk ? fmaf(nz, uz, fmaf(ny, uy, nx * ux)) : fmaf(ny, uy, nx * ux);
+                    }
                     const float k2 = kd + kd;
                     ux = fmaf(-k2, nx, ux);
                     uy = fmaf(-k2, ny, uy);
@@ -295,7 +350,23 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                 }
             }
         }
-        const float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
+        float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
+        if (GEN && lastpl >= 0) {
+            // after a plane reflection keep at least plane_eps from the plane on
+            // the walker's side, so that rounding cannot put the next start on
+            // the far side
+            int mt;
+            const float4 g = image_at<GEN>(T, lastpl, mt);
+            const float s0 = fmaf(g.z, z, fmaf(g.y, y, g.x * x)) - g.w;
+            const float se = fmaf(g.z, ez, fmaf(g.y, ey, g.x * ex)) - g.w;
+            const float sg = s0 >= 0.0f ? 1.0f : -1.0f;
+            const float gap = A.plane_eps - sg * se;
+            if (gap > 0.0f) {
+                ex = fmaf(sg * gap, g.x, ex);
+                ey = fmaf(sg * gap, g.y, ey);
+                ez = fmaf(sg * gap, g.z, ez);
+            }
+        }
         dx = dx + (ex - x);
         dy = dy + (ey - y);
         dz = dz + (ez - z);

@@ -4,21 +4,21 @@
 
 namespace axx {
 
-template <int AX, bool SPH>
+template <int AX, bool GEN>
 static walk_fn pick_rej(bool rej) {
-    return rej ? (walk_fn)walk_kernel<2, AX, SPH, true> : (walk_fn)walk_kernel<2, AX, SPH, false>;
+    return rej ? (walk_fn)walk_kernel<2, AX, GEN, true> : (walk_fn)walk_kernel<2, AX, GEN, false>;
 }
 template <int AX>
-static walk_fn pick_sph(bool sph, bool rej) {
-    return sph ? pick_rej<AX, true>(rej) : pick_rej<AX, false>(rej);
+static walk_fn pick_gen(bool gen, bool rej) {
+    return gen ? pick_rej<AX, true>(rej) : pick_rej<AX, false>(rej);
 }
 
-walk_fn pick_walk_p2(int ax, bool sph, bool rej) {
+walk_fn pick_walk_p2(int ax, bool gen, bool rej) {
     switch (ax) {
-    case 1: return pick_sph<1>(sph, rej);
-    case 3: return pick_sph<3>(sph, rej);
-    case 5: return pick_sph<5>(sph, rej);
-    default: return pick_sph<7>(sph, rej);
+    case 1: return pick_gen<1>(gen, rej);
+    case 3: return pick_gen<3>(gen, rej);
+    case 5: return pick_gen<5>(gen, rej);
+    default: return pick_gen<7>(gen, rej);
     }
 }
 

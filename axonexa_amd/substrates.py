@@ -1,4 +1,4 @@
-"""Substrate builders: periodic cells with cylinders or spheres.
+"""Substrate builders: periodic cells with cylinders, spheres or planes.
 
 Equivalents of how the reference drivers assemble a ``Lattice`` and its
 ``basis[]``:
@@ -8,6 +8,7 @@ Equivalents of how the reference drivers assemble a ``Lattice`` and its
     copies                           — src/substrate/RPSinitializer.h:76-186,
                                        examples/gammadist.cu:118-149
   * an empty cell (basisSize = 0)    — examples/Cos_Empty_Mel.cu:139-141
+  * two parallel planes              — examples/planePhase.cu:114-117
 Periodic images are added by the engine itself, so builders return each
 primitive once (centres may lie anywhere; they are wrapped into the cell).
 """
@@ -18,7 +19,7 @@ from typing import Optional
 
 import numpy as np
 
-from .abi import PRIM_CYLINDER_XY, PRIM_SPHERE, SubstrateSpec
+from .abi import PRIM_CYLINDER_XY, PRIM_PLANE, PRIM_SPHERE, SubstrateSpec
 
 
 def empty_box(L: float, D: float, T2: float = float("inf")) -> SubstrateSpec:
@@ -61,6 +62,19 @@ def hex_lattice(R: float, spacing: float, nx: int, ny: int, D_in: float, D_ex: f
     return SubstrateSpec(a=a, b=b, c=(c if c is not None else min(a, b)), D_lat=D_ex,
                          T2_lat=T2_ex, types=np.full(n, PRIM_CYLINDER_XY), centers=cs,
                          radii=np.full(n, R), D=D_in, T2=T2_in, kappa=kappa)
+
+
+def parallel_planes(cell: float, lo: float, hi: float, D: float, T2: float = float("inf"),
+                    axis: int = 0) -> SubstrateSpec:
+    """Two reflecting planes normal to `axis` at lo and hi in a cubic periodic
+    cell, normals pointing into the slab between them (planePhase.cu:114-117
+    puts them at 0 and at the cell length)."""
+    n = np.zeros((2, 3))
+    n[0, axis], n[1, axis] = 1.0, -1.0
+    cs = np.zeros((2, 3))
+    cs[0, axis], cs[1, axis] = lo, hi
+    return SubstrateSpec(a=cell, b=cell, c=cell, D_lat=D, T2_lat=T2, types=[PRIM_PLANE] * 2,
+                         centers=cs, radii=[0.0, 0.0], normals=n)
 
 
 def hex_intra_fraction(R: float, spacing: float) -> float:

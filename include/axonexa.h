@@ -40,9 +40,10 @@ extern "C" {
 #define AXX_ERR_NODEVICE    -7  /* no HIP device                              */
 #define AXX_ERR_STATE       -8  /* call out of order (e.g. results before run) */
 
-/* ---- primitive types (src/primitives/cylinderXY.cuh, Sphere.cuh) ---- */
+/* ---- primitive types (src/primitives/cylinderXY.cuh, Sphere.cuh, plane.cuh) ---- */
 #define AXX_PRIM_CYLINDER_XY 0  /* infinite cylinder along z */
 #define AXX_PRIM_SPHERE      1
+#define AXX_PRIM_PLANE       2  /* infinite reflecting plane (center = a point on it) */
 
 /* ---- boundary policies (kernelLattice.cuh:64-79, boundaryCheck.cuh:34-113) ---- */
 #define AXX_BOUNDARY_SPECULAR  0  /* multi-reflection, impermeable (SPECULAR_REFLECTION)   */
@@ -66,6 +67,9 @@ extern "C" {
  * Sphere(cx, cy, cz, R, ...).  Compartment ids: 0 = the lattice (extra-
  * cellular) space, k+1 = inside primitive k.  Primitives must not overlap
  * (under the minimum-image convention).  T2 <= 0 or +inf means no decay.
+ * Planes (reference Plane, plane.cuh:1-40) take a point (center) and a
+ * normal; they bound no compartment, reflect always and ignore radius, D,
+ * T2 and kappa.
  */
 typedef struct axx_substrate {
     double a, b, c;          /* cell lengths                                    */
@@ -78,6 +82,7 @@ typedef struct axx_substrate {
     const double* D;         /* [n_prims] diffusivity inside                     */
     const double* T2;        /* [n_prims] T2 inside                              */
     const double* kappa;     /* [n_prims] membrane permeability (length/time); may be NULL */
+    const double* normal;    /* [n_prims][3] plane normals; may be NULL without planes */
 } axx_substrate;
 
 /*

@@ -74,6 +74,7 @@ namespace axx_compat {
 struct Prim {
     int type = AXX_PRIM_CYLINDER_XY;
     real cx = 0, cy = 0, cz = 0, R = 0, D = 0, T2 = 0, T1 = 0, kappa = 0, eps = 0;
+    real nx = 0, ny = 0, nz = 0;   // planes: normal
     int region = 0;
 };
 }  // namespace axx_compat
@@ -125,6 +126,24 @@ class Sphere {
     real getD() const { return p.D; }
     real getT2() const { return p.T2; }
     real getPermeability() const { return p.kappa; }
+};
+
+// Infinite reflecting plane through `center` with normal `normal`
+// (src/primitives/plane.cuh:1-45): bounds no region, never transmits.
+class Plane {
+  public:
+    axx_compat::Prim p;
+    Plane() { p.type = AXX_PRIM_PLANE; p.nx = 1; }
+    Plane(Vector3 center, Vector3 normal) {
+        p.type = AXX_PRIM_PLANE;
+        p.cx = center.x; p.cy = center.y; p.cz = center.z;
+        p.nx = normal.x; p.ny = normal.y; p.nz = normal.z;
+    }
+    Vector3 getNormal(const Vector3&) const { return Vector3(p.nx, p.ny, p.nz); }
+    int getRegion(const Vector3&) const { return 0; }
+    real getT2(const Vector3&) const { return -1.0; }
+    real getD(const Vector3&) const { return -1.0; }
+    real getPermeability() const { return 0.0; }
 };
 
 class Lattice {
@@ -398,17 +417,18 @@ class phaseAcquisitionStream {
         const int n = acq.getNumOfSteps();
         auto S = axx_compat::sample(acq.getGradientFunctors(), n, (double)dt);
         std::vector<int> type;
-        std::vector<double> center, radius, D, T2, kappa;
+        std::vector<double> center, radius, D, T2, kappa, normal;
         for (auto& p : prims) {
             type.push_back(p.type);
             center.push_back(p.cx); center.push_back(p.cy); center.push_back(p.cz);
             radius.push_back(p.R); D.push_back(p.D); T2.push_back(p.T2); kappa.push_back(p.kappa);
+            normal.push_back(p.nx); normal.push_back(p.ny); normal.push_back(p.nz);
         }
         axx_substrate s{};
         s.a = lat.a; s.b = lat.b; s.c = lat.c; s.D_lat = lat.D; s.T2_lat = lat.T2;
         s.n_prims = (int)prims.size();
         s.type = type.data(); s.center = center.data(); s.radius = radius.data();
-        s.D = D.data(); s.T2 = T2.data(); s.kappa = kappa.data();
+        s.D = D.data(); s.T2 = T2.data(); s.kappa = kappa.data(); s.normal = normal.data();
         axx_acq q{};
         q.n_steps = n; q.dt = dt; q.gamma = GAMMA;
         q.n_profiles = S.P; q.profiles = S.prof.data();

@@ -126,6 +126,7 @@ typedef struct {
     int type;    /* AXX_PRIM_* */
     int comp;    /* compartment id = original index + 1 */
     int orig;
+    float nx, ny, nz, d;   /* planes: unit normal and offset n . point */
 } oprim;
 
 typedef struct {
@@ -139,6 +140,7 @@ typedef struct {
     float* thr_in; /* [n_prims] 4 kappa / v_lattice         */
     float Lmax;
     float H2c, H2s; /* squared half of the smallest cell side (xy; xyz) */
+    float plane_eps;/* minimum distance from a plane after reflecting from it */
     /* optional cell bins (binned variant of the scan) */
     int nb[3];
     float binw_inv[3];
@@ -146,10 +148,33 @@ typedef struct {
     int* bin_idx;
 } oprep;
 
+/* Projection of a box onto a unit normal: [min, max] of n . corner. */
+static void box_proj(const double n[3], const double lo[3], const double hi[3], double* pmin, double* pmax) {
+    *pmin = *pmax = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        double u = n[a] * lo[a], v = n[a] * hi[a];
+        *pmin += u < v ? u : v;
+        *pmax += u < v ? v : u;
+    }
+}
+
+static void unit_normal(const axx_substrate* s, int k, double n[3]) {
+    double l = sqrt(s->normal[3 * k] * s->normal[3 * k] + s->normal[3 * k + 1] * s->normal[3 * k + 1] +
+                    s->normal[3 * k + 2] * s->normal[3 * k + 2]);
+    for (int a = 0; a < 3; ++a) n[a] = s->normal[3 * k + a] / l;
+}
+
 static int prim_reach_hits_box(const axx_substrate* s, int k, double sx, double sy, double sz,
                                double lo[3], double hi[3], double reach) {
     /* Does primitive k shifted by (sx,sy,sz), grown by `reach`, touch the box? */
     double c[3] = {s->center[3 * k] + sx, s->center[3 * k + 1] + sy, s->center[3 * k + 2] + sz};
+    if (s->type[k] == AXX_PRIM_PLANE) {
+        double n[3], pmin, pmax;
+        unit_normal(s, k, n);
+        double d = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+        box_proj(n, lo, hi, &pmin, &pmax);
+        return d >= pmin - reach && d <= pmax + reach;
+    }
     double R = s->radius[k] + reach;
     int dims = s->type[k] == AXX_PRIM_SPHERE ? 3 : 2;
     double d2 = 0.0;
@@ -165,6 +190,11 @@ static int prim_reach_hits_box(const axx_substrate* s, int k, double sx, double 
 /* Boundary reachable from the box: the circle/sphere surface comes within
  * `reach` of some point of the box. */
 static int prim_surface_near_box(const oprim* p, double lo[3], double hi[3], double reach) {
+    if (p->type == AXX_PRIM_PLANE) {
+        double n[3] = {p->nx, p->ny, p->nz}, pmin, pmax;
+        box_proj(n, lo, hi, &pmin, &pmax);
+        return (double)p->d >= pmin - reach && (double)p->d <= pmax + reach;
+    }
     double c[3] = {p->cx, p->cy, p->cz};
     double R = sqrt((double)p->R2);
     int dims = p->type == AXX_PRIM_SPHERE ? 3 : 2;
@@ -196,6 +226,13 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
     if (!(s->D_lat >= 0)) return AXX_ERR_INVALID;
     double Lmax = sqrt(6.0 * s->D_lat * q->dt);
     for (int k = 0; k < s->n_prims; ++k) {
+        if (s->type[k] == AXX_PRIM_PLANE) {
+            if (!s->normal) return AXX_ERR_INVALID;
+            double l2 = s->normal[3 * k] * s->normal[3 * k] + s->normal[3 * k + 1] * s->normal[3 * k + 1] +
+                        s->normal[3 * k + 2] * s->normal[3 * k + 2];
+            if (!(l2 > 0)) return AXX_ERR_INVALID;
+            continue;
+        }
         if (s->type[k] != AXX_PRIM_CYLINDER_XY && s->type[k] != AXX_PRIM_SPHERE) return AXX_ERR_INVALID;
         if (!(s->radius[k] > 0) || !(s->D[k] >= 0)) return AXX_ERR_INVALID;
         double R = s->radius[k];
@@ -209,6 +246,7 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
      * RPSinitializer.h:191-219) */
     for (int i = 0; i < s->n_prims; ++i)
         for (int j = i + 1; j < s->n_prims; ++j) {
+            if (s->type[i] == AXX_PRIM_PLANE || s->type[j] == AXX_PRIM_PLANE) continue;
             double d[3];
             int dims = (s->type[i] == AXX_PRIM_SPHERE && s->type[j] == AXX_PRIM_SPHERE) ? 3 : 2;
             double L3[3] = {s->a, s->b, s->c};
@@ -233,6 +271,7 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
         double hc = 0.5 * fmin(s->a, s->b), hs = 0.5 * fmin(fmin(s->a, s->b), s->c);
         P->H2c = (float)(hc * hc);
         P->H2s = (float)(hs * hs);
+        P->plane_eps = (float)(64.0 * ldexp(1.0, -23) * fmax(fmax(s->a, s->b), s->c));
     }
     int n = s->n_prims;
     P->n_comp = n + 1;
@@ -245,7 +284,16 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
     P->L[0] = (float)sqrt(6.0 * s->D_lat * dt);
     P->rate[0] = (s->T2_lat > 0 && isfinite(s->T2_lat)) ? (float)(dt / s->T2_lat) : 0.0f;
     double v_lat = sqrt(6.0 * s->D_lat / dt);
+    int any_plane = 0;
     for (int k = 0; k < n; ++k) {
+        if (s->type[k] == AXX_PRIM_PLANE) {
+            /* planes bound no compartment: nothing diffuses "in" them */
+            P->L[k + 1] = 0.0f;
+            P->rate[k + 1] = 0.0f;
+            P->thr_out[k] = P->thr_in[k] = 0.0f;
+            any_plane = 1;
+            continue;
+        }
         double L = sqrt(6.0 * s->D[k] * dt);
         if (L > Lmax) Lmax = L;
         P->L[k + 1] = (float)L;
@@ -258,18 +306,38 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
     P->Lmax = (float)Lmax;
     /* periodic images: every shift in {-1,0,1}^2 (x{-1,0,1} for spheres)
      * whose grown shape touches the cell, in (original, shift) order */
-    double reach = Lmax * 1.001 + 1e-6 * fmax(s->a, s->b);
+    /* a plane counts as hit within plane_eps of it, so its reach is longer */
+    double reach = Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) + (any_plane ? 2.0 * P->plane_eps : 0.0);
     double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
     int cap = n * 27 + 1;
     P->img = malloc(sizeof(oprim) * cap);
     int m = 0;
     for (int k = 0; k < n; ++k) {
-        int zr = s->type[k] == AXX_PRIM_SPHERE ? 1 : 0;
+        int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
+        int first = m;
         for (int kz = -zr; kz <= zr; ++kz)
             for (int ky = -1; ky <= 1; ++ky)
                 for (int kx = -1; kx <= 1; ++kx) {
                     double sx = kx * s->a, sy = ky * s->b, sz = kz * s->c;
                     if (!prim_reach_hits_box(s, k, sx, sy, sz, lo, hi, reach)) continue;
+                    if (s->type[k] == AXX_PRIM_PLANE) {
+                        /* shifts along the plane give the same plane */
+                        double nn[3];
+                        unit_normal(s, k, nn);
+                        float dd = (float)(nn[0] * (s->center[3 * k] + sx) + nn[1] * (s->center[3 * k + 1] + sy) +
+                                           nn[2] * (s->center[3 * k + 2] + sz));
+                        int dup = 0;
+                        for (int q = first; q < m; ++q) dup |= P->img[q].d == dd;
+                        if (dup) continue;
+                        oprim* p = &P->img[m++];
+                        memset(p, 0, sizeof(*p));
+                        p->type = AXX_PRIM_PLANE;
+                        p->comp = k + 1;
+                        p->orig = k;
+                        p->nx = (float)nn[0]; p->ny = (float)nn[1]; p->nz = (float)nn[2];
+                        p->d = dd;
+                        continue;
+                    }
                     oprim* p = &P->img[m++];
                     p->cx = (float)(s->center[3 * k] + sx);
                     p->cy = (float)(s->center[3 * k + 1] + sy);
@@ -280,13 +348,14 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                     p->type = s->type[k];
                     p->comp = k + 1;
                     p->orig = k;
+                    p->nx = p->ny = p->nz = p->d = 0.0f;
                 }
     }
     P->n_img = m;
     /* cell bins (binned variant): surface within reach of the bin */
     if (bin_width > 0) {
         int any_sphere = 0;
-        for (int k = 0; k < n; ++k) any_sphere |= (s->type[k] == AXX_PRIM_SPHERE);
+        for (int k = 0; k < n; ++k) any_sphere |= (s->type[k] != AXX_PRIM_CYLINDER_XY);
         double Ls[3] = {s->a, s->b, s->c};
         for (int a = 0; a < 3; ++a) {
             int nb = (int)floor(Ls[a] / bin_width);
@@ -307,7 +376,11 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                 double c3[3] = {pi->cx, pi->cy, pi->cz};
                 int lo_b[3], hi_b[3];
                 for (int a = 0; a < 3; ++a) {
-                    if (a == 2 && pi->type != AXX_PRIM_SPHERE) { lo_b[a] = 0; hi_b[a] = P->nb[a] - 1; continue; }
+                    if ((a == 2 && pi->type == AXX_PRIM_CYLINDER_XY) || pi->type == AXX_PRIM_PLANE) {
+                        lo_b[a] = 0;
+                        hi_b[a] = P->nb[a] - 1;
+                        continue;
+                    }
                     double bw = Ls[a] / P->nb[a];
                     lo_b[a] = (int)floor((c3[a] - R) / bw) - 1;
                     hi_b[a] = (int)floor((c3[a] + R) / bw) + 1;
@@ -340,6 +413,7 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
 /* ------------------------------------------------------------------ */
 /* Geometry predicates (cylinderXY.cuh:69-80, Sphere.cuh:89-94). */
 static inline int inside_at(const oprim* p, float x, float y, float z) {
+    if (p->type == AXX_PRIM_PLANE) return 0;
     float ox = x - p->cx, oy = y - p->cy;
     if (p->type == AXX_PRIM_SPHERE) {
         float oz = z - p->cz;
@@ -361,6 +435,30 @@ static int region_of(const oprep* P, float x, float y, float z) {
  * of the surface is taken from its tracked compartment, and the start point
  * is clamped onto that side (C' = min(C,0) inside, max(C,0) outside), which
  * replaces the reference's epsilon test (real_equal(root*|dr|, 0, eps)). */
+/* Plane crossing (plane.cuh:18-28 restated): the walker's side is the side
+ * of the step's start point (x0), and the segment start is clamped onto it.
+ * The plane counts as hit when the segment comes closer than eps to it (not
+ * only when it crosses), so that, with the push applied after a plane
+ * reflection, every step ends at least about eps away on the walker's side and
+ * the side taken from the next start point is the tracked one.  The reported
+ * time is that of reaching the plane, at most ell. */
+static inline int plane_test(const oprim* p, float x0, float y0, float z0, float sx, float sy,
+                             float sz, float ux, float uy, float uz, float ell, float eps,
+                             float* t_out) {
+    float un = fmaf(p->nz, uz, fmaf(p->ny, uy, p->nx * ux));
+    float sd = fmaf(p->nz, sz, fmaf(p->ny, sy, p->nx * sx)) - p->d;
+    float s0 = fmaf(p->nz, z0, fmaf(p->ny, y0, p->nx * x0)) - p->d;
+    float sg = s0 >= 0.0f ? 1.0f : -1.0f;
+    float dd = sg * sd;
+    dd = dd > 0.0f ? dd : 0.0f;     /* clamped distance of the segment start */
+    float v = -(sg * un);           /* approach speed towards the plane */
+    if (!(v > 0.0f)) return 0;
+    if (!(fmaf(-ell, v, dd) < eps)) return 0;
+    float t = dd / v;
+    *t_out = t < ell ? t : ell;
+    return 1;
+}
+
 static inline int hit_test(const oprim* p, int same_comp, float H2c, float H2s, float sx,
                            float sy, float sz, float ux, float uy, float uz, float Axy, float A3,
                            float ell, float* t_out, int* ins_out) {
@@ -479,6 +577,9 @@ static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t k
         *comp = region_of(P, *x, *y, *z);
         return 0;
     }
+    if (R->init_mode == AXX_INIT_COMPARTMENT)
+        for (int i = 0; i < P->n_img; ++i)
+            if (P->img[i].comp == R->init_arg && P->img[i].type == AXX_PRIM_PLANE) return AXX_ERR_INVALID;
     for (uint32_t it = 0; it < (1u << 20); ++it) {
         draw(wid, it, STREAM_INIT << 24, inst, key, o);
         float px, py, pz;
@@ -495,7 +596,7 @@ static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t k
             const oprim* pk = NULL;
             for (int i = 0; i < P->n_img; ++i)
                 if (P->img[i].orig == k) { pk = &P->img[i]; break; }
-            if (!pk) return AXX_ERR_INVALID;
+            if (!pk || pk->type == AXX_PRIM_PLANE) return AXX_ERR_INVALID;
             float Rf = pk->R;
             px = pk->cx + (unif(o[0]) * 2.0f - 1.0f) * Rf;
             py = pk->cy + (unif(o[1]) * 2.0f - 1.0f) * Rf;
@@ -560,6 +661,7 @@ static void* job_run(void* arg) {
             orc_sincos_turn(wa, &sphi, &cphi);
             float ux = sth * cphi, uy = sth * sphi, uz = c;
             float ell = P->L[comp];
+            int lastplane = -1;
             float sx = x, sy = y, sz = z;
             const int* list;
             int cnt, dense;
@@ -569,13 +671,22 @@ static void* job_run(void* arg) {
                 int rej = 0;
                 for (int j = 0; j < cnt; ++j) {
                     const oprim* pk = &P->img[dense ? j : list[j]];
-                    if (inside_at(pk, sx, sy, sz) != inside_at(pk, ex, ey, ez)) { rej = 1; break; }
+                    float tp;
+                    /* a plane rejects the step that would come within
+                     * plane_eps of it (it bounds no region) */
+                    if (pk->type == AXX_PRIM_PLANE
+                            ? plane_test(pk, x, y, z, sx, sy, sz, ux, uy, uz, ell, P->plane_eps, &tp)
+                            : inside_at(pk, sx, sy, sz) != inside_at(pk, ex, ey, ez)) {
+                        rej = 1;
+                        break;
+                    }
                 }
                 if (rej) { ncoll++; ell = 0.0f; }
             } else {
                 float Axy = fmaf(uy, uy, ux * ux);
                 float A3 = fmaf(uz, uz, Axy);
                 int nb = 0, kc = 0;
+                lastplane = -1;
                 for (;;) {
                     float best = INFINITY;
                     int bk = -1, bins_ = 0;
@@ -583,10 +694,13 @@ static void* job_run(void* arg) {
                         int k = dense ? j : list[j];
                         const oprim* pk = &P->img[k];
                         float t;
-                        int ins;
-                        if (hit_test(pk, comp == pk->comp, P->H2c, P->H2s, sx, sy, sz, ux, uy, uz,
-                                     Axy, A3, ell, &t, &ins) &&
-                            t < best) {
+                        int ins = 0, hit;
+                        if (pk->type == AXX_PRIM_PLANE)
+                            hit = plane_test(pk, x, y, z, sx, sy, sz, ux, uy, uz, ell, P->plane_eps, &t);
+                        else
+                            hit = hit_test(pk, comp == pk->comp, P->H2c, P->H2s, sx, sy, sz, ux, uy, uz,
+                                           Axy, A3, ell, &t, &ins);
+                        if (hit && t < best) {
                             best = t;
                             bk = k;
                             bins_ = ins;
@@ -599,7 +713,7 @@ static void* job_run(void* arg) {
                     float px = fmaf(t, ux, sx), py = fmaf(t, uy, sy), pz = fmaf(t, uz, sz);
                     int inside = bins_;
                     int transmit = 0;
-                    if (mode == AXX_BOUNDARY_PERMEABLE) {
+                    if (mode == AXX_BOUNDARY_PERMEABLE && pk->type != AXX_PRIM_PLANE) {
                         uint32_t oc[4];
                         draw(wid, (uint32_t)i, (STREAM_COLL << 24) | ((uint32_t)kc >> 2), inst, key, oc);
                         float U = unif(oc[kc & 3]);
@@ -607,6 +721,7 @@ static void* job_run(void* arg) {
                         transmit = U <= thr;
                         kc++;
                     }
+                    lastplane = pk->type == AXX_PRIM_PLANE ? bk : -1;
                     if (transmit) {
                         int nc = inside ? 0 : pk->comp;
                         float rem = ell - t;
@@ -620,7 +735,10 @@ static void* job_run(void* arg) {
                         ntran++;
                     } else {
                         float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
-                        if (pk->type == AXX_PRIM_SPHERE) {
+                        if (pk->type == AXX_PRIM_PLANE) {
+                            nx = pk->nx; ny = pk->ny; nz = pk->nz;
+                            kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
+                        } else if (pk->type == AXX_PRIM_SPHERE) {
                             nz = pz - pk->cz;
                             float inv = 1.0f / pk->R;
                             nx = nx * inv; ny = ny * inv; nz = nz * inv;
@@ -643,6 +761,21 @@ static void* job_run(void* arg) {
                 }
             }
             float ex = fmaf(ell, ux, sx), ey = fmaf(ell, uy, sy), ez = fmaf(ell, uz, sz);
+            if (lastplane >= 0) {
+                /* after a plane reflection keep at least plane_eps from the
+                 * plane on the walker's side, so rounding cannot put the next
+                 * step's start on the far side */
+                const oprim* pl = &P->img[lastplane];
+                float s0 = fmaf(pl->nz, z, fmaf(pl->ny, y, pl->nx * x)) - pl->d;
+                float se = fmaf(pl->nz, ez, fmaf(pl->ny, ey, pl->nx * ex)) - pl->d;
+                float sg = s0 >= 0.0f ? 1.0f : -1.0f;
+                float gap = P->plane_eps - sg * se;
+                if (gap > 0.0f) {
+                    ex = fmaf(sg * gap, pl->nx, ex);
+                    ey = fmaf(sg * gap, pl->ny, ey);
+                    ez = fmaf(sg * gap, pl->nz, ez);
+                }
+            }
             dx = dx + (ex - x);
             dy = dy + (ey - y);
             dz = dz + (ez - z);
@@ -794,7 +927,7 @@ int orc_image_count(const axx_substrate* s, const axx_acq* q) {
 
 int orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
                  const float u3[3], float ell, float* t) {
-    oprim p = {cx, cy, cz, R * R, R, type, 1, 0};
+    oprim p = {cx, cy, cz, R * R, R, type, 1, 0, 0.0f, 0.0f, 0.0f, 0.0f};
     float Axy = fmaf(u3[1], u3[1], u3[0] * u3[0]);
     float A3 = fmaf(u3[2], u3[2], Axy);
     int ins;

@@ -54,3 +54,20 @@ def neuman_limit(gamma, g, delta, D, R, shape="cylinder"):
     else:
         k, c = 8.0 / 175.0, 581.0 / 840.0
     return math.exp(-k * gamma * gamma * g * g * R ** 4 / D * (2.0 * delta - c * R * R / D))
+
+
+def gpd_pgse_planes(gamma, g, delta, Delta, D, l, n_terms=400):
+    """GPD for PGSE normal to two parallel reflecting planes a distance l
+    apart: the position autocorrelation is sum_{n odd} 8 l^2/(n^4 pi^4)
+    exp(-D (n pi / l)^2 t), so
+      ln E = -gamma^2 g^2 sum_{n odd} [8 l^2/(n^4 pi^4)] F(a_n) / (D^2 a_n^4),
+    F(a) = 2 D a^2 d - 2 + 2e^{-D a^2 d} + 2e^{-D a^2 Delta}
+           - e^{-D a^2 (Delta - d)} - e^{-D a^2 (Delta + d)}, a_n = n pi / l
+    (the same form as the cylinder formula above with its coefficients)."""
+    n = np.arange(1, 2 * n_terms, 2, dtype=float)
+    a = n * math.pi / l
+    Da2 = D * a * a
+    F = (2 * Da2 * delta - 2 + 2 * np.exp(-Da2 * delta) + 2 * np.exp(-Da2 * Delta)
+         - np.exp(-Da2 * (Delta - delta)) - np.exp(-Da2 * (Delta + delta)))
+    B = 8 * l * l / (n ** 4 * math.pi ** 4)
+    return math.exp(-gamma * gamma * g * g * float(np.sum(B * F / (D * D * a ** 4))))

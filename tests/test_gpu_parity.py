@@ -137,3 +137,52 @@ def test_bounce_cap_counted():
     acq = _pgse(800, 1e-2)
     sig, _ = _run_both("bounce_cap", sub, acq, abi.RunSpec(n_walkers=2048, seed=2, bounce_cap=1))
     assert sig.cap_hits > 0
+
+
+def test_planes_slab_with_permeable_sphere():
+    # planes x = 4 and x = 12 (normals into the slab) around a permeable
+    # sphere; planes reflect, never transmit and draw no collision numbers
+    a = 16.0
+    sub = abi.SubstrateSpec(a=a, b=a, c=a, D_lat=2.0, T2_lat=60.0,
+                            types=[abi.PRIM_PLANE, abi.PRIM_PLANE, abi.PRIM_SPHERE],
+                            centers=[[4.0, 0, 0], [12.0, 0, 0], [8.0, 8.0, 8.0]],
+                            radii=[0.0, 0.0, 2.5], D=[0.0, 0.0, 1.0], T2=[np.inf, np.inf, 30.0],
+                            kappa=[0.0, 0.0, sb.kappa_for_probability(0.05, 1.0, 1e-2)],
+                            normals=[[1, 0, 0], [-1, 0, 0], [0, 0, 0]])
+    acq = _pgse(1500, 1e-2, axis=(1, 0, 0))
+    run = abi.RunSpec(n_walkers=2048, seed=17, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both("planes_slab_sphere", sub, acq, run)
+    assert sig.collisions > 1000 and sig.transmissions > 10
+
+
+def test_planes_at_cell_faces():
+    # the reference's plane validation geometry (planePhase.cu:114-117):
+    # planes at x = 0 and x = l, a cell of side l
+    l = 10.0
+    sub = abi.SubstrateSpec(a=l, b=l, c=l, D_lat=2.5, types=[abi.PRIM_PLANE] * 2,
+                            centers=[[0.0, 0, 0], [l, 0, 0]], radii=[0.0, 0.0],
+                            normals=[[1, 0, 0], [-1, 0, 0]])
+    acq = wf.build_acq([wf.CosGFunc(g * 1e-6, 10.0, 2.0, 3, (1, 0, 0)) for g in (0.0, 0.05, 0.1)],
+                       2200, 1e-2, GAMMA)
+    sig, _ = _run_both("planes_cell_faces", sub, acq, abi.RunSpec(n_walkers=2048, seed=23))
+    assert sig.collisions > 1000
+
+
+def test_oblique_plane_with_cylinders():
+    sub = sb.hex_lattice(2.0, 6.0, 2, 1, D_in=1.5, D_ex=2.0)
+    sub = abi.SubstrateSpec(a=sub.a, b=sub.b, c=sub.c, D_lat=2.0,
+                            types=list(sub.types) + [abi.PRIM_PLANE],
+                            centers=np.vstack([sub.centers, [[6.0, 5.0, 1.0]]]),
+                            radii=list(sub.radii) + [0.0], D=list(sub.D) + [0.0],
+                            normals=np.vstack([np.zeros((sub.n_prims, 3)), [[1.0, 2.0, 0.5]]]))
+    acq = _pgse(1500, 1e-2, axis=(0.6, 0.0, 0.8))
+    sig, _ = _run_both("oblique_plane", sub, acq, abi.RunSpec(n_walkers=2048, seed=31))
+    assert sig.collisions > 1000
+
+
+def test_planes_rejection_mode():
+    sub = sb.parallel_planes(10.0, 2.5, 7.5, 2.0)
+    acq = _pgse(1200, 1e-2)
+    sig, _ = _run_both("planes_rejection", sub, acq,
+                       abi.RunSpec(n_walkers=2048, seed=41, boundary_mode=abi.BOUNDARY_REJECTION))
+    assert sig.collisions > 100

@@ -168,3 +168,60 @@ def test_translation_of_the_substrate_shifts_walkers():
     a = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0, center=(10.0, 10.0, 0.0))
     b = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0, center=(30.0, -10.0, 0.0))
     assert np.array_equal(po.walk(a, acq, run), po.walk(b, acq, run))
+
+
+def test_parallel_planes_match_gpd():
+    # two reflecting planes x = a/4 and x = 3a/4 in a periodic cell: both
+    # slabs (inner, and outer across the periodic edge) have width l = a/2
+    a = 16.0
+    sub = abi.SubstrateSpec(a=a, b=a, c=a, D_lat=2.0, types=[abi.PRIM_PLANE] * 2,
+                            centers=[[a / 4, 0, 0], [3 * a / 4, 0, 0]], radii=[1.0, 1.0],
+                            normals=[[1, 0, 0], [-1, 0, 0]])
+    gs = np.linspace(0.0, 0.3, 7) * 1e-6
+    acq = wf.build_acq([wf.RectGrad(g, 5.0, 10.0) for g in gs], 20000, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=8000, seed=21)
+    sig, st = po.run_phase(sub, acq, run, bin_width=1.0)
+    v = abi.state_view(st, run.n_walkers, acq.n_profiles)
+    x0, x = v["r0"][:, 0], v["pos"][:, 0]
+    inner0, inner = (x0 > a / 4) & (x0 < 3 * a / 4), (x > a / 4) & (x < 3 * a / 4)
+    assert np.array_equal(inner0, inner)       # nobody crossed a plane
+    assert sig.collisions > 1000
+    for j, g in enumerate(gs):
+        e = an.gpd_pgse_planes(GAMMA, g, 5.0, 15.0, 2.0, a / 2)
+        assert abs(sig.E_re[j] - e) <= 3 * sig.sigma_re[j] + 0.02 * (1 - e) + 1e-9, (j, sig.E_re[j], e)
+
+
+def test_oblique_plane_bands_hold_walkers():
+    # plane x + y = a (normal (1,1,0)/sqrt 2) in a periodic square cell: its
+    # images x + y = k a cut the plane into bands that no walker may leave.
+    # The band is read off the unwrapped position r0 + displacement.
+    a = 8.0
+    sub = abi.SubstrateSpec(a=a, b=a, c=a, D_lat=2.0, types=[abi.PRIM_PLANE],
+                            centers=[[a / 2, a / 2, 0.0]], radii=[0.0], normals=[[1.0, 1.0, 0.0]])
+    acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 5000, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=4000, seed=5)
+    st = po.walk(sub, acq, run)
+    v = abi.state_view(st, run.n_walkers, acq.n_profiles)
+    r0 = v["r0"].astype(np.float64)
+    r1 = r0 + v["disp"].astype(np.float64)
+    b0 = np.floor((r0[:, 0] + r0[:, 1]) / a)
+    b1 = np.floor((r1[:, 0] + r1[:, 1]) / a)
+    assert np.array_equal(b0, b1)
+    assert int(v["ncoll"].sum()) > 1000
+    # the binned scan gives the same walk
+    assert np.array_equal(po.walk(sub, acq, run, bin_width=1.0), st)
+
+
+def test_plane_rejection_and_invalid_init():
+    sub = sb.parallel_planes(10.0, 2.5, 7.5, 2.0)
+    acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 2000, 1e-2, GAMMA)
+    run = abi.RunSpec(n_walkers=2000, seed=9, boundary_mode=abi.BOUNDARY_REJECTION)
+    st = po.walk(sub, acq, run)
+    v = abi.state_view(st, run.n_walkers, acq.n_profiles)
+    x0, x = v["r0"][:, 0], v["pos"][:, 0]
+    assert np.array_equal((x0 > 2.5) & (x0 < 7.5), (x > 2.5) & (x < 7.5))
+    assert int(v["ncoll"].sum()) > 100
+    with pytest.raises(RuntimeError):
+        po.walk(sub, acq, abi.RunSpec(n_walkers=4, seed=1, init_mode=abi.INIT_INSIDE_PRIM, init_arg=0))
+    with pytest.raises(RuntimeError):
+        po.walk(sub, acq, abi.RunSpec(n_walkers=4, seed=1, init_mode=abi.INIT_COMPARTMENT, init_arg=1))
