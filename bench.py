@@ -4,10 +4,14 @@ hexagonal-lattice cosine-OGSE workload (1000 permeable cylinders, 1e6
 walkers, 250,000 steps; see axonexa_amd/workloads.py::c2_hex), with the
 CPU oracle timed beside it on the host cores.
 
-One bench step = one full walk (init + 250,000 time steps of every walker)
+One bench step = one full walk (init + every time step of every walker)
 plus the fp64 signal reduction, inputs resident on the device.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+--workload c3_gamma measures the gamma-packing workload (400 cylinders,
+4e6 walkers x 80,000 steps, 64 profiles x 3 axes); c1_free / c1_cylinder the
+small PGSE workloads.
 
 Prints one JSON line.
 """
@@ -40,14 +44,32 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+PROFILE = os.path.join(ROOT, "profiles", "walk_kernel_c2_hex.txt")
+
+
+def _profiled_traffic(walkers: int, n_steps: int):
+    """HBM bytes per walk launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (calibrated against a plain copy) for the default workload
+    size; None for other sizes or when the summary is absent."""
+    if walkers != 1000000 or n_steps != 250000 or not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    return None
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--walkers", type=int, default=1000000)
-    ap.add_argument("--n-steps", type=int, default=250000, help="time steps per walk")
-    ap.add_argument("--cpu-walkers", type=int, default=4096,
+    ap.add_argument("--workload", default="c2_hex",
+                    choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder"])
+    ap.add_argument("--walkers", type=int, default=0, help="0: the workload's own count")
+    ap.add_argument("--n-steps", type=int, default=0, help="time steps per walk (0: the workload's)")
+    ap.add_argument("--cpu-walkers", type=int, default=32768,
                     help="walkers in the CPU-baseline sample (full step count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: plan default)")
@@ -56,7 +78,8 @@ def main(argv=None) -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(the path shards by walker id, see DESIGN.md)"}))
+                                   "(walkers are independent: a multi-GPU run would give each "
+                                   "device its own walker-id range)"}))
         return 2
 
     import numpy as np
@@ -64,7 +87,14 @@ def main(argv=None) -> int:
     from axonexa_amd import abi, engine
     from axonexa_amd import workloads as W
 
-    wl = W.c2_hex(n_walkers=args.walkers, n_steps=args.n_steps)
+    make = {"c2_hex": W.c2_hex, "c3_gamma": W.c3_gamma, "c1_free": W.c1_free,
+            "c1_cylinder": W.c1_cylinder}[args.workload]
+    kw = {}
+    if args.walkers:
+        kw["n_walkers"] = args.walkers
+    if args.n_steps:
+        kw["n_steps"] = args.n_steps
+    wl = make(**kw)
     n = wl.acq.n_steps
     plan = engine.Plan(wl.sub, wl.acq, wl.run)
     if args.block:
@@ -105,13 +135,13 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "c2_hex: " + wl.description,
+            "workload": wl.name + ": " + wl.description,
             "walkers": wl.run.n_walkers,
             "time_steps": n,
             "dt_ms": wl.acq.dt,
             "waveforms": wl.acq.n_waveforms,
             "profiles": wl.acq.n_profiles,
-            "cylinders": wl.sub.n_prims,
+            "primitives": wl.sub.n_prims,
             "block": cfg["block"],
             "bin_width_um": round(cfg["bin_width"], 4),
             "lds_bytes": cfg["lds_bytes"],
@@ -122,15 +152,18 @@ def main(argv=None) -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": (_profiled_traffic(wl.run.n_walkers, n) if args.workload == "c2_hex"
+                        else None),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload "
+                              "(profiles/walk_kernel_c2_hex.txt), not measured in this run",
             "flops_per_walker_step": F,
             "walker_steps_per_launch": ws,
             "kernel_ms": avg_ms,
             "collisions_per_walker_step": rho,
         },
         "signal": {
-            "E_perp_lowfreq_maxG": float(sig.E[14]),
-            "E_par_lowfreq_maxG": float(sig.E[15]),
+            "E_first": float(sig.E[0]),
+            "E_last": float(sig.E[-1]),
             "transmissions": sig.transmissions,
             "cap_hits": sig.cap_hits,
         },
@@ -139,8 +172,8 @@ def main(argv=None) -> int:
     if not args.no_cpu:
         from oracle import pyoracle as po
 
-        nw = min(args.cpu_walkers, args.walkers)
-        sample = W.c2_hex(n_walkers=nw, n_steps=args.n_steps)
+        nw = min(args.cpu_walkers, wl.run.n_walkers)
+        sample = make(**dict(kw, n_walkers=nw))
         threads = _cpu_threads()
         tc = time.perf_counter()
         ost = po.walk(sample.sub, sample.acq, sample.run, threads=threads,

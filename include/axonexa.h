@@ -218,7 +218,8 @@ int  axx_plan_sync(axx_plan* plan);
 /* Copy walker state to / from the host (n_walkers * AXX_STATE_FIELDS(P) words). */
 int  axx_plan_get_state(axx_plan* plan, uint32_t* host_state);
 int  axx_plan_set_state(axx_plan* plan, const uint32_t* host_state);
-/* Time a walk over [step_begin, step_end) with HIP events on the plan's stream. */
+/* Walk over [step_begin, step_end) and time the walk kernel with HIP events on
+ * the plan's stream (from step 0 the placement runs first, untimed). */
 int  axx_plan_time_walk(axx_plan* plan, int step_begin, int step_end, float* ms);
 /* Launch configuration the plan uses (threads per block, walkers per thread,
  * bin width, LDS bytes) and a setter used by the autotuner. */

@@ -129,7 +129,7 @@ class Sphere {
 };
 
 // Infinite reflecting plane through `center` with normal `normal`
-// (src/primitives/plane.cuh:1-45): bounds no region, never transmits.
+// (src/primitives/plane.cuh:1-40): bounds no region, never transmits.
 class Plane {
   public:
     axx_compat::Prim p;

@@ -12,12 +12,14 @@
  *   boundaryCheck.cuh:138-217         multi-reflection / permeable boundary
  *   lattice.cuh:31-247                periodic cell, wrap (lmod), region lookup
  *   cylinderXY.cuh:69-131, Sphere.cuh:89-149   inside / intersect / normal
+ *   plane.cuh:18-39                   plane crossing, normal, no region
  *   kernelSetup.cuh:1-5               per-walker RNG (replaced: Philox4x32-10)
  *   phaseAcquisitionStream.cuh:171-189 + reduction.cuh:76-195   signal sums
  * with the deliberate, documented deviations of the build (counter-based
  * Philox, sphere-point picking without acos, segment form of the reflection
  * loop, side-tracked crossing test, true T2 time split, continued boundary
- * checks after a transmission).  Every function cites the reference lines it
+ * checks after a transmission, planes hit within plane_eps with a push off
+ * the plane after reflecting).  Every function cites the reference lines it
  * follows.
  *
  * Pins (what checks this file, tests/): Philox known-answer vectors
