@@ -88,6 +88,10 @@ class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("bin_width", C.c_double), ("lds_budget", C.c_int)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("substrate", C.POINTER(Substrate)), ("acq", C.POINTER(Acq)), ("run", C.POINTER(Run))]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("sum_cos", _dp), ("sum_sin", _dp), ("sum_cos2", _dp), ("sum_sin2", _dp),

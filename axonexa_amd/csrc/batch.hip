@@ -1,0 +1,530 @@
+// batch.hip — many substrates and acquisitions of uneven walker counts in
+// one persistent launch, captured with the rest of the run into a hipGraph.
+//
+// No counterpart in the reference's kernels: its nearest relative is
+// phaseAcquisitionStream<F>::runAcquisitionStream / runAcquisitionStreamLattice
+// (src/acquisition/phaseAcquisitionStream.cuh:254-613), which loops over
+// acquisitions and launches one walk plus 2M reductions per acquisition.
+// Here every instance keeps its own plan (tables, walker state, reduction
+// buffers, exactly as a single run), and the batch adds:
+//   * a tile list: (instance, walker range) work items of block x wpt
+//     walkers, longest first;
+//   * batch_walk_kernel: a persistent grid (occupancy x CUs) whose blocks take
+//     tiles from an atomic counter, restage the instance's substrate tables
+//     in LDS when the instance changes, and walk the tile with the same
+//     per-lane step code as the single-run kernel (bitwise the same walkers);
+//   * batched placement, reduction and finalisation kernels over all
+//     instances;
+//   * an autotuner over (threads per block, walkers per thread) timed on a
+//     short probe walk, with a per-context cache (optionally a file).
+#include "internal.h"
+#include "init_kernel.h"
+
+#include <cstdlib>
+#include <map>
+
+namespace axx {
+
+// Instance kernel variants: padded profile width 1, 4 or 8, every gradient
+// axis, cylinders-only or general geometry.  class = 2 * width_index + gen.
+constexpr int kBatchMaxProfiles = 8;
+constexpr int kBatchBlockMax = 512;
+
+struct BatchWalkArgs {
+    const WalkArgs* __restrict__ args;   // [n_inst]
+    const int* __restrict__ cls;         // [n_inst]
+    const int4* __restrict__ tiles;      // [n_tiles] (instance, w0, w1, 0)
+    int n_tiles;
+    unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
+};
+
+template <int PT, bool GEN>
+__device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
+    for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x)
+        walk_lane<PT, 7, GEN, false>(A, T, A.prof, (uint32_t)w);
+}
+
+__global__ __launch_bounds__(kBatchBlockMax) void batch_walk_kernel(BatchWalkArgs B) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_tile;
+    int staged = -1;
+    LdsTables T{};
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(B.next, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t >= B.n_tiles) break;
+        const int4 tile = B.tiles[t];
+        const int i = __builtin_amdgcn_readfirstlane(tile.x);
+        const int w0 = __builtin_amdgcn_readfirstlane(tile.y);
+        const int w1 = __builtin_amdgcn_readfirstlane(tile.z);
+        const WalkArgs A = B.args[i];
+        const int c = __builtin_amdgcn_readfirstlane(B.cls[i]);
+        if (i != staged) {
+            // every lane finished the previous tile (barriers above)
+            T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
+            staged = i;
+        }
+        switch (c) {
+        case 0: batch_tile<1, false>(A, T, w0, w1); break;
+        case 1: batch_tile<1, true>(A, T, w0, w1); break;
+        case 2: batch_tile<4, false>(A, T, w0, w1); break;
+        case 3: batch_tile<4, true>(A, T, w0, w1); break;
+        case 4: batch_tile<8, false>(A, T, w0, w1); break;
+        default: batch_tile<8, true>(A, T, w0, w1); break;
+        }
+    }
+}
+
+// Placement: one block per tile.
+__global__ __launch_bounds__(256) void batch_init_kernel(const WalkArgs* __restrict__ args,
+                                                         const int* __restrict__ n_mom,
+                                                         const int4* __restrict__ tiles) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int4 tile = tiles[blockIdx.x];
+    const int i = __builtin_amdgcn_readfirstlane(tile.x);
+    const WalkArgs A = args[i];
+    float4* geom;
+    int* meta;
+    stage_images(A, lds, geom, meta);
+    const int nm = n_mom[i];
+    for (int w = tile.y + (int)threadIdx.x; w < tile.z; w += (int)blockDim.x) init_lane(A, geom, meta, w, nm);
+}
+
+// Zero the tile counter and every instance's event counters (a kernel, not
+// a memset node, so the graph replays it reliably).
+__global__ void batch_zero_kernel(unsigned int* next, unsigned long long* const* __restrict__ cnt, int n_inst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *next = 0u;
+    if (t < 4 * n_inst) cnt[t >> 2][t & 3] = 0ull;
+}
+
+// Reduction: one block per (instance, chunk, waveform group).
+__global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const RedArgs* __restrict__ red,
+                                                                   const int4* __restrict__ blocks) {
+    const int4 b = blocks[blockIdx.x];
+    const RedArgs R = red[__builtin_amdgcn_readfirstlane(b.x)];
+    reduce_chunk(R, __builtin_amdgcn_readfirstlane(b.y), __builtin_amdgcn_readfirstlane(b.z));
+}
+
+// Chunk-ordered sums: one block per instance.
+__global__ void batch_finalize_kernel(const RedArgs* __restrict__ red, double* const* __restrict__ out) {
+    const RedArgs R = red[blockIdx.x];
+    double* o = out[blockIdx.x];
+    for (int t = threadIdx.x; t < R.M * kRedVals; t += blockDim.x) {
+        const int j = t / kRedVals, v = t % kRedVals;
+        double s = 0.0;
+        for (int c = 0; c < R.n_chunks; ++c) s += R.partial[((size_t)j * R.n_chunks + c) * kRedVals + v];
+        o[t] = s;
+    }
+}
+
+}  // namespace axx
+
+struct axx_batch {
+    axx_ctx* ctx = nullptr;
+    std::vector<axx_plan*> plans;
+    std::vector<int> cls;
+    int block = 256, wpt = 1, grid = 0, n_tiles = 0, n_red = 0;
+    size_t lds_walk = 0, lds_init = 0;
+    std::string signature;
+    // device tables
+    WalkArgs* d_args = nullptr;
+    WalkArgs* d_probe = nullptr;
+    int* d_cls = nullptr;
+    int* d_nmom = nullptr;
+    int4* d_tiles = nullptr;
+    unsigned int* d_next = nullptr;
+    RedArgs* d_red = nullptr;
+    int4* d_red_blocks = nullptr;
+    unsigned long long** d_cnt = nullptr;
+    double** d_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+};
+
+namespace {
+
+std::map<std::string, std::pair<int, int>>& tune_cache() {
+    static std::map<std::string, std::pair<int, int>> m;
+    return m;
+}
+
+int class_of(const axx_plan* p) {
+    const int wi = p->Pt <= 1 ? 0 : (p->Pt <= 4 ? 1 : 2);
+    return 2 * wi + (p->gen ? 1 : 0);
+}
+
+int min_pt_for(int P) { return P <= 1 ? 1 : (P <= 4 ? 4 : 8); }
+
+void batch_free_graph(axx_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void batch_free(axx_batch* b) {
+    if (!b) return;
+    batch_free_graph(b);
+    void* bufs[] = {b->d_args, b->d_probe, b->d_cls, b->d_nmom, b->d_tiles, b->d_next,
+                    b->d_red, b->d_red_blocks, b->d_cnt, b->d_out};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (axx_plan* p : b->plans) plan_free(p);
+    delete b;
+}
+
+template <typename T>
+int upload(axx_ctx* ctx, T** dst, const std::vector<T>& v) {
+    if (*dst) {
+        (void)hipFree(*dst);
+        *dst = nullptr;
+    }
+    HIP_TRY(ctx, hipMalloc((void**)dst, std::max<size_t>(v.size(), 1) * sizeof(T)));
+    if (!v.empty()) HIP_TRY(ctx, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return AXX_OK;
+}
+
+// Relative cost of one walker-step of a class (moments dominate for wide
+// profile sets).
+double class_weight(int c) {
+    static const double w[6] = {1.0, 1.2, 1.2, 1.4, 1.45, 1.65};
+    return w[c];
+}
+
+// Tiles of block * wpt walkers, longest (walker-steps x class weight) first;
+// equal costs keep instance then walker order.
+int build_tiles(axx_batch* b, int block, int wpt) {
+    axx_ctx* ctx = b->ctx;
+    const long long T = (long long)block * wpt;
+    struct Tile { double cost; int inst, w0, w1; };
+    std::vector<Tile> tiles;
+    for (int i = 0; i < (int)b->plans.size(); ++i) {
+        const axx_plan* p = b->plans[i];
+        for (long long w0 = 0; w0 < p->N; w0 += T) {
+            const long long w1 = std::min<long long>(p->N, w0 + T);
+            tiles.push_back({(double)(w1 - w0) * (p->n_steps - 1) * class_weight(b->cls[i]), i, (int)w0,
+                             (int)w1});
+        }
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const Tile& x, const Tile& y) { return x.cost > y.cost; });
+    std::vector<int4> h(tiles.size());
+    for (size_t k = 0; k < tiles.size(); ++k) h[k] = make_int4(tiles[k].inst, tiles[k].w0, tiles[k].w1, 0);
+    b->n_tiles = (int)h.size();
+    int rc = upload(ctx, &b->d_tiles, h);
+    if (rc) return rc;
+    int per_cu = 0;
+    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, b->lds_walk));
+    if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "batch kernel does not fit a CU at block %d", block);
+    b->grid = std::max(1, std::min(per_cu * std::max(ctx->n_cu, 1), b->n_tiles));
+    b->block = block;
+    b->wpt = wpt;
+    return AXX_OK;
+}
+
+int launch_walk_part(axx_batch* b, const WalkArgs* args) {
+    axx_ctx* ctx = b->ctx;
+    const int n = (int)b->plans.size();
+    hipLaunchKernelGGL(batch_zero_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, ctx->stream, b->d_next,
+                       b->d_cnt, n);
+    HIP_TRY(ctx, hipGetLastError());
+    hipLaunchKernelGGL(batch_init_kernel, dim3(b->n_tiles), dim3(256), b->lds_init, ctx->stream,
+                       (const WalkArgs*)b->d_args, (const int*)b->d_nmom, (const int4*)b->d_tiles);
+    HIP_TRY(ctx, hipGetLastError());
+    BatchWalkArgs B{args, b->d_cls, b->d_tiles, b->n_tiles, b->d_next};
+    hipLaunchKernelGGL(batch_walk_kernel, dim3(b->grid), dim3(b->block), b->lds_walk, ctx->stream, B);
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+int launch_reduce_part(axx_batch* b) {
+    axx_ctx* ctx = b->ctx;
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(b->n_red), dim3(kRedThreads), 0, ctx->stream,
+                       (const RedArgs*)b->d_red, (const int4*)b->d_red_blocks);
+    HIP_TRY(ctx, hipGetLastError());
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)b->plans.size()), dim3(256), 0, ctx->stream,
+                       (const RedArgs*)b->d_red, b->d_out);
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+// Capture zero + placement + walk + reduction into one graph.
+int capture(axx_batch* b) {
+    axx_ctx* ctx = b->ctx;
+    batch_free_graph(b);
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_walk_part(b, b->d_args);
+    if (!rc) rc = launch_reduce_part(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, AXX_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return AXX_OK;
+}
+
+void load_file_cache() {
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const char* path = std::getenv("AXX_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char key[4096];
+    int blk, wpt;
+    while (std::fscanf(f, "%4095s %d %d", key, &blk, &wpt) == 3) tune_cache()[key] = {blk, wpt};
+    std::fclose(f);
+}
+
+void store_file_cache(const std::string& key, int blk, int wpt) {
+    const char* path = std::getenv("AXX_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d\n", key.c_str(), blk, wpt);
+    std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" void axx_batch_destroy(axx_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_free(b);
+}
+
+extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* inst, const axx_tuning* tune,
+                                axx_batch** out) {
+    if (!ctx || !inst || !out || n_inst < 1) return fail(ctx, AXX_ERR_INVALID, "batch: bad arguments");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    axx_batch* b = new (std::nothrow) axx_batch();
+    if (!b) return AXX_ERR_NOMEM;
+    b->ctx = ctx;
+    axx_tuning t{};
+    if (tune) t = *tune;
+    if (t.lds_budget <= 0) t.lds_budget = 72 * 1024;   // room for two workgroups per CU
+    t.block = 0;
+    char sig[64];
+    std::snprintf(sig, sizeof(sig), "n%d", n_inst);
+    b->signature = sig;
+    std::vector<WalkArgs> args;
+    std::vector<int> nmom;
+    std::vector<RedArgs> red;
+    std::vector<int4> rblocks;
+    std::vector<unsigned long long*> cnt;
+    std::vector<double*> outs;
+    for (int i = 0; i < n_inst; ++i) {
+        const axx_instance& I = inst[i];
+        if (!I.substrate || !I.acq || !I.run) {
+            batch_free(b);
+            return fail(ctx, AXX_ERR_INVALID, "instance %d: null descriptor", i);
+        }
+        if (I.run->boundary_mode == AXX_BOUNDARY_REJECTION) {
+            batch_free(b);
+            return fail(ctx, AXX_ERR_INVALID, "instance %d: rejection runs are not batched", i);
+        }
+        if (I.acq->n_profiles > kBatchMaxProfiles) {
+            batch_free(b);
+            return fail(ctx, AXX_ERR_LIMIT, "instance %d: %d profiles (batch limit %d)", i, I.acq->n_profiles,
+                        kBatchMaxProfiles);
+        }
+        if (I.run->n_walkers >= (1ll << 31)) {
+            batch_free(b);
+            return fail(ctx, AXX_ERR_LIMIT, "instance %d: n_walkers >= 2^31", i);
+        }
+        axx_plan* p = new (std::nothrow) axx_plan();
+        if (!p) {
+            batch_free(b);
+            return AXX_ERR_NOMEM;
+        }
+        p->ctx = ctx;
+        const int rc = plan_create_impl(ctx, I.substrate, I.acq, I.run, &t, p, min_pt_for(I.acq->n_profiles));
+        if (rc) {
+            std::string msg = "instance " + std::to_string(i) + ": " + ctx->err;
+            plan_free(p);
+            batch_free(b);
+            return fail(ctx, rc, "%s", msg.c_str());
+        }
+        b->plans.push_back(p);
+        b->cls.push_back(class_of(p));
+        WalkArgs A = plan_walk_args(p, 1, p->n_steps);
+        args.push_back(A);
+        nmom.push_back(3 * p->Pt);
+        b->lds_walk = std::max(b->lds_walk, p->lds_bytes);
+        b->lds_init = std::max(b->lds_init, p->init_lds);
+        RedArgs R = plan_red_args(p);
+        red.push_back(R);
+        const int groups = (p->M + kRedJB - 1) / kRedJB;
+        for (int c = 0; c < p->n_chunks; ++c)
+            for (int g = 0; g < groups; ++g) rblocks.push_back(make_int4(i, c, g, 0));
+        cnt.push_back(p->d_cnt);
+        outs.push_back(p->d_out);
+        char buf[96];
+        std::snprintf(buf, sizeof(buf), ";c%dN%lldn%di%dm%d", b->cls.back(), (long long)p->N, p->n_steps,
+                      p->n_img, p->M);
+        b->signature += buf;
+    }
+    b->n_red = (int)rblocks.size();
+    int rc;
+    if ((rc = upload(ctx, &b->d_args, args)) || (rc = upload(ctx, &b->d_cls, b->cls)) ||
+        (rc = upload(ctx, &b->d_nmom, nmom)) || (rc = upload(ctx, &b->d_red, red)) ||
+        (rc = upload(ctx, &b->d_red_blocks, rblocks)) || (rc = upload(ctx, &b->d_cnt, cnt)) ||
+        (rc = upload(ctx, &b->d_out, outs))) {
+        batch_free(b);
+        return rc;
+    }
+    if (hipMalloc((void**)&b->d_next, sizeof(unsigned int)) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        batch_free(b);
+        return fail(ctx, AXX_ERR_DEVICE, "batch: device allocation");
+    }
+    if (b->lds_walk > ctx->lds_max || b->lds_init > ctx->lds_max) {
+        batch_free(b);
+        return fail(ctx, AXX_ERR_LIMIT, "batch: substrate tables exceed LDS");
+    }
+    // configuration: explicit, cached, or a default; the autotuner may change it
+    int blk = 256, wpt = 1;
+    load_file_cache();
+    auto it = tune_cache().find(b->signature);
+    if (tune && tune->block) blk = tune->block;
+    else if (it != tune_cache().end()) { blk = it->second.first; wpt = it->second.second; }
+    if (blk < 64 || blk > kBatchBlockMax || blk % 64) {
+        batch_free(b);
+        return fail(ctx, AXX_ERR_INVALID, "batch block must be a multiple of 64 in [64, %d]", kBatchBlockMax);
+    }
+    if ((rc = build_tiles(b, blk, wpt)) || (rc = capture(b))) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return AXX_OK;
+}
+
+extern "C" int axx_batch_run(axx_batch* b) {
+    if (!b) return AXX_ERR_INVALID;
+    axx_ctx* ctx = b->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    for (axx_plan* p : b->plans) {
+        p->next_step = p->n_steps;
+        p->reduced = true;
+    }
+    b->ran = true;
+    return AXX_OK;
+}
+
+extern "C" int axx_batch_time(axx_batch* b, float* ms) {
+    if (!b || !ms) return AXX_ERR_INVALID;
+    axx_ctx* ctx = b->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
+    int rc = axx_batch_run(b);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(b->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return AXX_OK;
+}
+
+extern "C" int axx_batch_sync(axx_batch* b) {
+    if (!b) return AXX_ERR_INVALID;
+    HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    return AXX_OK;
+}
+
+extern "C" int axx_batch_results(axx_batch* b, int i, axx_result* out) {
+    if (!b || !out || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch results: bad index");
+    if (!b->ran) return fail(b->ctx, AXX_ERR_STATE, "batch results requested before a run");
+    return axx_plan_results(b->plans[i], out);
+}
+
+extern "C" int axx_batch_get_state(axx_batch* b, int i, uint32_t* host) {
+    if (!b || !host || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch state: bad index");
+    return axx_plan_get_state(b->plans[i], host);
+}
+
+extern "C" int axx_batch_set_config(axx_batch* b, int block, int wpt) {
+    if (!b) return AXX_ERR_INVALID;
+    if (block < 64 || block > kBatchBlockMax || block % 64 || wpt < 1 || wpt > 64)
+        return fail(b->ctx, AXX_ERR_INVALID, "batch config: block multiple of 64 in [64, %d], wpt in [1, 64]",
+                    kBatchBlockMax);
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    int rc = build_tiles(b, block, wpt);
+    if (rc) return rc;
+    return capture(b);
+}
+
+extern "C" int axx_batch_get_config(const axx_batch* b, int* block, int* wpt, int* grid, int* n_tiles) {
+    if (!b) return AXX_ERR_INVALID;
+    if (block) *block = b->block;
+    if (wpt) *wpt = b->wpt;
+    if (grid) *grid = b->grid;
+    if (n_tiles) *n_tiles = b->n_tiles;
+    return AXX_OK;
+}
+
+extern "C" int axx_batch_autotune(axx_batch* b, int probe_steps, int* best_block, int* best_wpt, float* best_ms) {
+    if (!b || probe_steps < 1) return AXX_ERR_INVALID;
+    axx_ctx* ctx = b->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // probe arguments: every instance walks steps [1, min(n, probe + 1))
+    std::vector<WalkArgs> probe;
+    for (axx_plan* p : b->plans) probe.push_back(plan_walk_args(p, 1, std::min(p->n_steps, probe_steps + 1)));
+    int rc = upload(ctx, &b->d_probe, probe);
+    if (rc) return rc;
+    static const int blocks[] = {256, 512};
+    static const int wpts[] = {1, 2, 4};
+    float best = 1e30f;
+    int bb = b->block, bw = b->wpt;
+    for (int blk : blocks)
+        for (int w : wpts) {
+            if ((rc = build_tiles(b, blk, w))) return rc;
+            if ((rc = launch_walk_part(b, b->d_probe))) return rc;   // warm
+            HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
+            if ((rc = launch_walk_part(b, b->d_probe))) return rc;
+            HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
+            HIP_TRY(ctx, hipEventSynchronize(b->ev1));
+            float ms = 0;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best) {
+                best = ms;
+                bb = blk;
+                bw = w;
+            }
+        }
+    if ((rc = build_tiles(b, bb, bw)) || (rc = capture(b))) return rc;
+    for (axx_plan* p : b->plans) p->reduced = false;   // the probe overwrote the walker state
+    b->ran = false;
+    tune_cache()[b->signature] = {bb, bw};
+    store_file_cache(b->signature, bb, bw);
+    if (best_block) *best_block = bb;
+    if (best_wpt) *best_wpt = bw;
+    if (best_ms) *best_ms = best;
+    return AXX_OK;
+}
+
+extern "C" int axx_run_batch(axx_ctx* ctx, int n_inst, const axx_instance* inst, axx_result* out) {
+    if (!out) return AXX_ERR_INVALID;
+    axx_batch* b = nullptr;
+    int rc = axx_batch_create(ctx, n_inst, inst, nullptr, &b);
+    if (rc) return rc;
+    rc = axx_batch_run(b);
+    for (int i = 0; !rc && i < n_inst; ++i) rc = axx_batch_results(b, i, &out[i]);
+    axx_batch_destroy(b);
+    return rc;
+}

@@ -19,85 +19,8 @@
 #include <string>
 #include <vector>
 
-#include "../../include/axonexa.h"
-#include "reduce_kernel.h"
+#include "internal.h"
 #include "init_kernel.h"
-#include "walk_kernel.h"
-
-using namespace axx;
-
-struct axx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int n_cu = 0;
-    size_t lds_max = 65536;
-    std::string err;
-};
-
-struct axx_plan {
-    axx_ctx* ctx = nullptr;
-    // host copies
-    int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7;
-    bool gen = false;       // spheres or planes present: general kernel path
-    long long N = 0;
-    axx_run run{};
-    double dt = 0;
-    std::vector<double> S, wf_scale, wf_dir;
-    std::vector<int> wf_p;
-    // geometry sizes
-    int n_img = 0, n_bins = 0, n_cand = 0, nb[3] = {1, 1, 1};
-    float ib[3] = {1, 1, 1}, ib4[2] = {4, 4};
-    size_t init_lds = 0;
-    double bin_width = 0;
-    size_t lds_bytes = 0;
-    int init_img = 0;
-    int block = 256, wpt = 1;
-    int next_step = 0;
-    bool reduced = false;
-    // device buffers
-    float4* d_geom = nullptr;
-    int* d_meta = nullptr;
-    float4* d_cyl = nullptr;
-    uint4* d_bins = nullptr;
-    uint16_t* d_cand = nullptr;
-    float* d_L = nullptr;
-    float* d_rate = nullptr;
-    float* d_thr_out = nullptr;
-    float* d_thr_in = nullptr;
-    float* d_prof = nullptr;
-    uint32_t* d_state = nullptr;
-    int* d_err = nullptr;
-    int* d_wfp = nullptr;
-    double* d_wfdir = nullptr;
-    double* d_wfscale = nullptr;
-    double* d_S = nullptr;
-    double* d_partial = nullptr;
-    double* d_out = nullptr;
-    unsigned long long* d_cnt = nullptr;
-    int n_chunks = 0, chunk = 0;
-    float cell[3] = {0, 0, 0};
-    float H2c = 0, H2s = 0, plane_eps = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
-
-static int fail(axx_ctx* ctx, int code, const char* fmt, ...) {
-    if (ctx) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        ctx->err = buf;
-    }
-    return code;
-}
-
-#define HIP_TRY(ctx, call)                                                                  \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail((ctx), AXX_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 extern "C" const char* axx_version(void) { return "axonexa_amd 0.1.0 (gfx950)"; }
 
@@ -493,7 +416,7 @@ static int dupload(axx_ctx* ctx, T** dst, const T* src, size_t n) {
     return AXX_OK;
 }
 
-static void plan_free(axx_plan* p) {
+void plan_free(axx_plan* p) {
     if (!p) return;
     void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_bins, p->d_cand, p->d_L, p->d_rate,
                     p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
@@ -517,8 +440,8 @@ extern "C" void axx_plan_destroy(axx_plan* plan) {
     plan_free(plan);
 }
 
-static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
-                            const axx_run* r, const axx_tuning* tune, axx_plan* p) {
+int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
+                     const axx_tuning* tune, axx_plan* p, int min_pt) {
     int rc = validate(ctx, s, q);
     if (rc) return rc;
     if (r->n_walkers < 1) return fail(ctx, AXX_ERR_INVALID, "n_walkers must be >= 1");
@@ -548,7 +471,7 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     p->n_steps = q->n_steps;
     p->dt = q->dt;
     p->P = q->n_profiles;
-    p->Pt = 1;
+    p->Pt = std::max(1, min_pt);
     while (p->Pt < p->P) p->Pt *= 2;
     p->M = q->n_waveforms;
     p->ax = axes_mask(q);
@@ -616,7 +539,14 @@ static int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq*
     if ((rc = dupload<unsigned long long>(ctx, &p->d_cnt, nullptr, 4))) return rc;
     HIP_TRY(ctx, hipEventCreate(&p->ev0));
     HIP_TRY(ctx, hipEventCreate(&p->ev1));
-    p->block = (tune && tune->block) ? tune->block : 1024;
+    if (tune && tune->block) {
+        p->block = tune->block;
+    } else {
+        // largest block that still gives every CU at least two workgroups
+        // (small walker counts would otherwise leave CUs idle)
+        const long long per = std::max<long long>(64, p->N / (2LL * std::max(ctx->n_cu, 1)));
+        p->block = (int)std::min<long long>(1024, per / 64 * 64);
+    }
     p->wpt = 1;
     HIP_TRY(ctx, hipDeviceSynchronize());
     return AXX_OK;
@@ -629,7 +559,7 @@ extern "C" int axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const ax
     axx_plan* p = new (std::nothrow) axx_plan();
     if (!p) return AXX_ERR_NOMEM;
     p->ctx = ctx;
-    int rc = plan_create_impl(ctx, s, q, r, tune, p);
+    int rc = plan_create_impl(ctx, s, q, r, tune, p, 1);
     if (rc) {
         plan_free(p);
         return rc;
@@ -638,8 +568,7 @@ extern "C" int axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const ax
     return AXX_OK;
 }
 
-static int launch_walk(axx_plan* p, int i0, int i1) {
-    axx_ctx* ctx = p->ctx;
+WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     WalkArgs A{};
     A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
     A.H2c = p->H2c; A.H2s = p->H2s; A.plane_eps = p->plane_eps;
@@ -649,6 +578,7 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
     A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_cyl = p->d_cyl; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
     A.compL = p->d_L; A.rate = p->d_rate; A.thr_out = p->d_thr_out; A.thr_in = p->d_thr_in;
+    A.prof = p->d_prof;
     A.state = p->d_state;
     A.N = p->N;
     A.k0 = (uint32_t)p->run.seed; A.k1 = (uint32_t)(p->run.seed >> 32);
@@ -658,6 +588,12 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
     A.i0 = i0; A.i1 = i1;
     A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
     A.err = p->d_err;
+    return A;
+}
+
+static int launch_walk(axx_plan* p, int i0, int i1) {
+    axx_ctx* ctx = p->ctx;
+    WalkArgs A = plan_walk_args(p, i0, i1);
     walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
     const int block = std::min(p->block, walk_max_block(p->Pt));
     const long long grid = (p->N + block - 1) / block;
@@ -712,12 +648,7 @@ extern "C" int axx_plan_time_walk(axx_plan* p, int step_begin, int step_end, flo
     return AXX_OK;
 }
 
-extern "C" int axx_plan_reduce(axx_plan* p) {
-    if (!p) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    if (p->next_step != p->n_steps)
-        return fail(ctx, AXX_ERR_STATE, "walk not finished (at step %d of %d)", p->next_step, p->n_steps);
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+RedArgs plan_red_args(const axx_plan* p) {
     RedArgs R{};
     R.state = p->d_state;
     R.N = p->N;
@@ -734,6 +665,16 @@ extern "C" int axx_plan_reduce(axx_plan* p) {
     R.S = p->d_S;
     R.partial = p->d_partial;
     R.counters = p->d_cnt;
+    return R;
+}
+
+extern "C" int axx_plan_reduce(axx_plan* p) {
+    if (!p) return AXX_ERR_INVALID;
+    axx_ctx* ctx = p->ctx;
+    if (p->next_step != p->n_steps)
+        return fail(ctx, AXX_ERR_STATE, "walk not finished (at step %d of %d)", p->next_step, p->n_steps);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const RedArgs R = plan_red_args(p);
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_cnt, 4);
     const int groups = (p->M + kRedJB - 1) / kRedJB;
     hipLaunchKernelGGL(reduce_kernel, dim3(p->n_chunks, groups), dim3(kRedThreads), 0, ctx->stream, R);
@@ -812,7 +753,7 @@ extern "C" int axx_plan_set_state(axx_plan* p, const uint32_t* host) {
 
 extern "C" int axx_plan_get_config(const axx_plan* p, int* block, int* wpt, double* bin_width, int* lds) {
     if (!p) return AXX_ERR_INVALID;
-    if (block) *block = p->block;
+    if (block) *block = std::min(p->block, walk_max_block(p->Pt));   // as launched
     if (wpt) *wpt = p->wpt;
     if (bin_width) *bin_width = p->bin_width;
     if (lds) *lds = (int)p->lds_bytes;

@@ -1,25 +1,28 @@
-// init_kernel.h — initial walker placement (included by one translation
-// unit only).
+// init_kernel.h — initial walker placement.
 #pragma once
 #include "walk_kernel.h"
 
 namespace axx {
 
-// Initial placement (kernelLattice.cuh:29-37, lattice.cuh:31-58,
-// cylinderXY.cuh:176-183), all draws on the init stream; writes the full
-// walker state (n_mom moment fields zeroed).
-__global__ __launch_bounds__(1024) void init_kernel(WalkArgs A, int n_mom) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    float4* geom = reinterpret_cast<float4*>(lds);
-    int* meta = reinterpret_cast<int*>(geom + A.n_img);
+// Images of A staged into LDS as geom[n_img] then meta[n_img]
+// (20 * n_img bytes).
+__device__ __forceinline__ void stage_images(const WalkArgs& A, unsigned char* lds, float4*& geom,
+                                             int*& meta) {
+    geom = reinterpret_cast<float4*>(lds);
+    meta = reinterpret_cast<int*>(geom + A.n_img);
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
         geom[i] = A.g_geom[i];
         meta[i] = A.g_meta[i];
     }
     __syncthreads();
+}
+
+// Initial placement of walker w (kernelLattice.cuh:29-37, lattice.cuh:31-58,
+// cylinderXY.cuh:176-183), all draws on the init stream; writes the full
+// walker state (n_mom moment fields zeroed).
+__device__ __forceinline__ void init_lane(const WalkArgs& A, const float4* geom, const int* meta,
+                                          long long w, int n_mom) {
     const long long N = A.N;
-    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= N) return;
     uint32_t* st = A.state;
     float* fs = reinterpret_cast<float*>(st);
     int* is = reinterpret_cast<int*>(st);
@@ -78,6 +81,16 @@ __global__ __launch_bounds__(1024) void init_kernel(WalkArgs A, int n_mom) {
     st[(size_t)F_NCAP * N + w] = 0u;
     is[(size_t)F_STEP * N + w] = 1;
     for (int f = 0; f < n_mom; ++f) fs[(size_t)(F_MOM + f) * N + w] = 0.0f;
+}
+
+static __global__ __launch_bounds__(1024) void init_kernel(WalkArgs A, int n_mom) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float4* geom;
+    int* meta;
+    stage_images(A, lds, geom, meta);
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= A.N) return;
+    init_lane(A, geom, meta, w, n_mom);
 }
 
 }  // namespace axx

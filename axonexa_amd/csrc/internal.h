@@ -1,0 +1,101 @@
+// internal.h — host-side objects shared by the C ABI translation units
+// (capi.hip: contexts and plans; batch.hip: batched runs).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/axonexa.h"
+#include "reduce_kernel.h"
+#include "walk_kernel.h"
+
+using namespace axx;
+
+struct axx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t lds_max = 65536;
+    std::string err;
+};
+
+struct axx_plan {
+    axx_ctx* ctx = nullptr;
+    // host copies
+    int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7;
+    bool gen = false;       // spheres or planes present: general kernel path
+    long long N = 0;
+    axx_run run{};
+    double dt = 0;
+    std::vector<double> S, wf_scale, wf_dir;
+    std::vector<int> wf_p;
+    // geometry sizes
+    int n_img = 0, n_bins = 0, n_cand = 0, nb[3] = {1, 1, 1};
+    float ib[3] = {1, 1, 1}, ib4[2] = {4, 4};
+    size_t init_lds = 0;
+    double bin_width = 0;
+    size_t lds_bytes = 0;
+    int init_img = 0;
+    int block = 256, wpt = 1;
+    int next_step = 0;
+    bool reduced = false;
+    // device buffers
+    float4* d_geom = nullptr;
+    int* d_meta = nullptr;
+    float4* d_cyl = nullptr;
+    uint4* d_bins = nullptr;
+    uint16_t* d_cand = nullptr;
+    float* d_L = nullptr;
+    float* d_rate = nullptr;
+    float* d_thr_out = nullptr;
+    float* d_thr_in = nullptr;
+    float* d_prof = nullptr;
+    uint32_t* d_state = nullptr;
+    int* d_err = nullptr;
+    int* d_wfp = nullptr;
+    double* d_wfdir = nullptr;
+    double* d_wfscale = nullptr;
+    double* d_S = nullptr;
+    double* d_partial = nullptr;
+    double* d_out = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    int n_chunks = 0, chunk = 0;
+    float cell[3] = {0, 0, 0};
+    float H2c = 0, H2s = 0, plane_eps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static int fail(axx_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), AXX_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+
+// Plan internals used by the batched runs.  min_pt: smallest padded
+// profile width to allocate moments for (1 for a plain plan).
+int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
+                     const axx_tuning* tune, axx_plan* p, int min_pt);
+void plan_free(axx_plan* p);
+WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1);
+RedArgs plan_red_args(const axx_plan* p);

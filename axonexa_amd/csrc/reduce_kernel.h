@@ -48,11 +48,12 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(kRedThreads) void reduce_kernel(RedArgs R) {
+// One block of kRedThreads reduces walker chunk chunk_id for waveform group
+// gy (waveforms gy*kRedJB ...); group 0 also adds the counters.
+__device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_id, const int gy) {
     __shared__ double sh[kRedThreads / 64][kRedJB * kRedVals];
     __shared__ unsigned long long shc[kRedThreads / 64][4];
-    const int chunk_id = blockIdx.x;
-    const int j0 = blockIdx.y * kRedJB;
+    const int j0 = gy * kRedJB;
     const long long N = R.N;
     const float* fs = reinterpret_cast<const float*>(R.state);
     const int* is = reinterpret_cast<const int*>(R.state);
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(RedArgs R) {
         const double wt = R.relaxation ? exp(-Ef) : 1.0;
         const bool intra = is[12 * N + w] != 0;
         const double r0[3] = {(double)fs[6 * N + w], (double)fs[7 * N + w], (double)fs[8 * N + w]};
-        if (blockIdx.y == 0) {
+        if (gy == 0) {
             cc[0] += R.state[14 * N + w];
             cc[1] += R.state[15 * N + w];
             cc[2] += R.state[16 * N + w];
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(RedArgs R) {
             const double s = wave_sum(acc[jj][v]);
             if (lane == 0) sh[wave][jj * kRedVals + v] = s;
         }
-    if (blockIdx.y == 0) {
+    if (gy == 0) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const unsigned long long s = wave_sum_u64(cc[k]);
@@ -130,15 +131,19 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(RedArgs R) {
             R.partial[((size_t)j * R.n_chunks + chunk_id) * kRedVals + v] = s;
         }
     }
-    if (blockIdx.y == 0 && threadIdx.x < 4) {
+    if (gy == 0 && threadIdx.x < 4) {
         unsigned long long s = 0;
         for (int wv = 0; wv < kRedThreads / 64; ++wv) s += shc[wv][threadIdx.x];
         atomicAdd(&R.counters[threadIdx.x], s);
     }
 }
 
+static __global__ __launch_bounds__(kRedThreads) void reduce_kernel(RedArgs R) {
+    reduce_chunk(R, blockIdx.x, blockIdx.y);
+}
+
 // Sum the per-chunk partials in chunk order: one thread per (waveform, value).
-__global__ void finalize_kernel(const double* __restrict__ partial, int M, int n_chunks,
+static __global__ void finalize_kernel(const double* __restrict__ partial, int M, int n_chunks,
                                 double* __restrict__ out /*[M][6]*/) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M * kRedVals) return;
@@ -148,7 +153,7 @@ __global__ void finalize_kernel(const double* __restrict__ partial, int M, int n
     out[t] = s;
 }
 
-__global__ void zero_u64(unsigned long long* p, int n) {
+static __global__ void zero_u64(unsigned long long* p, int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0ull;
 }

@@ -32,6 +32,7 @@ struct WalkArgs {
     const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
     const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
     const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
+    const float* __restrict__ prof;         // [n_steps][Pt] unit profile samples
     const float* __restrict__ compL;        // [n_comp] step length
     const float* __restrict__ rate;         // [n_comp] dt / T2
     const float* __restrict__ thr_out;      // [n_prims] 4 kappa / v_inside
@@ -167,24 +168,20 @@ walk_fn pick_walk_p64(int ax, bool gen, bool rej);
 // limited to 512 threads per block so that each lane may use 256 VGPRs.
 constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 
-// GEN: substrates with spheres or planes (3D bins, per-image meta words);
-// otherwise cylinders only.
+// One lane walks walker wo of A over steps [max(A.i0, 1), A.i1) with the
+// substrate tables T in LDS.  GEN: substrates with spheres or planes (3D
+// bins, per-image meta words); otherwise cylinders only.
 template <int P, int AX, bool GEN, bool REJ>
-__global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const LdsTables T = stage_tables<GEN>(A, lds);
-
+__device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
+                                          const float* __restrict__ prof_g, const uint32_t wo) {
     const long long N = A.N;
-    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= N) return;
     uint32_t* st = A.state;
     float* fs = reinterpret_cast<float*>(st);
     int* is = reinterpret_cast<int*>(st);
-    const uint32_t wid = A.woff + (uint32_t)w;
+    const uint32_t wid = A.woff + wo;
     const_fptr prof = (const_fptr)prof_g;
 
-    // field f of this walker: base + f*N (wave-uniform) + w (32-bit lane offset)
-    const uint32_t wo = (uint32_t)w;
+    // field f of this walker: base + f*N (wave-uniform) + wo (32-bit lane offset)
     float x, y, z, dx, dy, dz, E, lf;
     int comp, li;
     uint32_t ncoll, ntran, ncap;
@@ -424,6 +421,15 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
         if (AX & 2) (fso + (F_MOM + 3 * p + 1) * N)[wo2] = m[p][1];
         if (AX & 4) (fso + (F_MOM + 3 * p + 2) * N)[wo2] = m[p][2];
     }
+}
+
+template <int P, int AX, bool GEN, bool REJ>
+__global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsTables T = stage_tables<GEN>(A, lds);
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= A.N) return;
+    walk_lane<P, AX, GEN, REJ>(A, T, prof_g, (uint32_t)w);
 }
 
 }  // namespace axx

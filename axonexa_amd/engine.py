@@ -67,6 +67,20 @@ def lib():
     L.axx_plan_set_config.argtypes = [vp, C.c_int, C.c_int]
     L.axx_plan_profile_count.argtypes = [vp]
     L.axx_plan_destroy.argtypes = [vp]
+    IP = C.POINTER(abi.Instance)
+    L.axx_batch_create.argtypes = [vp, C.c_int, IP, C.POINTER(abi.Tuning), C.POINTER(vp)]
+    L.axx_batch_run.argtypes = [vp]
+    L.axx_batch_time.argtypes = [vp, C.POINTER(C.c_float)]
+    L.axx_batch_sync.argtypes = [vp]
+    L.axx_batch_results.argtypes = [vp, C.c_int, Res]
+    L.axx_batch_get_state.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32)]
+    L.axx_batch_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float)]
+    L.axx_batch_set_config.argtypes = [vp, C.c_int, C.c_int]
+    L.axx_batch_get_config.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.axx_batch_destroy.argtypes = [vp]
+    L.axx_run_batch.argtypes = [vp, C.c_int, IP, Res]
     _LIB = L
     return L
 
@@ -204,3 +218,87 @@ def run_phase(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
     ctx.check(lib().axx_run_phase(ctx.handle, C.byref(s), C.byref(q), C.byref(r), C.byref(res)),
               "axx_run_phase")
     return abi.signals_from(res, bufs)
+
+
+class Batch:
+    """Many (substrate, acquisition, run) instances in one persistent,
+    graph-captured launch (axx_batch_*).  Instance i gives bitwise the walkers
+    and sums of ``run_phase(*instances[i])``."""
+
+    def __init__(self, instances, ctx: Optional[Context] = None, block: int = 0,
+                 lds_budget: int = 0):
+        self.ctx = ctx or default_context()
+        self.instances = list(instances)
+        self._structs = [(s.struct(), q.struct(), r.struct()) for s, q, r in self.instances]
+        arr = (abi.Instance * len(self._structs))()
+        for k, (s, q, r) in enumerate(self._structs):
+            arr[k].substrate = C.pointer(s)
+            arr[k].acq = C.pointer(q)
+            arr[k].run = C.pointer(r)
+        self._arr = arr
+        self._t = abi.Tuning(int(block), 0.0, int(lds_budget))
+        self._h = C.c_void_p()
+        rc = lib().axx_batch_create(self.ctx.handle, len(self._structs), arr, C.byref(self._t),
+                                    C.byref(self._h))
+        self.ctx.check(rc, "axx_batch_create")
+
+    def __len__(self):
+        return len(self.instances)
+
+    def run(self):
+        self.ctx.check(lib().axx_batch_run(self._h), "axx_batch_run")
+        return self
+
+    def time(self) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().axx_batch_time(self._h, C.byref(ms)), "axx_batch_time")
+        return ms.value
+
+    def sync(self):
+        self.ctx.check(lib().axx_batch_sync(self._h), "axx_batch_sync")
+
+    def results(self, i: int) -> abi.Signals:
+        res, bufs = abi.alloc_result(self.instances[i][1].n_waveforms)
+        self.ctx.check(lib().axx_batch_results(self._h, int(i), C.byref(res)), "axx_batch_results")
+        return abi.signals_from(res, bufs)
+
+    def get_state(self, i: int) -> np.ndarray:
+        _, q, r = self.instances[i]
+        st = np.zeros(abi.state_fields(q.n_profiles) * r.n_walkers, np.uint32)
+        self.ctx.check(lib().axx_batch_get_state(self._h, int(i),
+                                                 st.ctypes.data_as(C.POINTER(C.c_uint32))),
+                       "axx_batch_get_state")
+        return st
+
+    def autotune(self, probe_steps: int = 200):
+        b, w, ms = C.c_int(), C.c_int(), C.c_float()
+        self.ctx.check(lib().axx_batch_autotune(self._h, int(probe_steps), C.byref(b), C.byref(w),
+                                                C.byref(ms)), "axx_batch_autotune")
+        return {"block": b.value, "wpt": w.value, "probe_ms": ms.value}
+
+    def set_config(self, block: int, wpt: int = 1):
+        self.ctx.check(lib().axx_batch_set_config(self._h, int(block), int(wpt)),
+                       "axx_batch_set_config")
+
+    def config(self):
+        b, w, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(lib().axx_batch_get_config(self._h, C.byref(b), C.byref(w), C.byref(g),
+                                                  C.byref(t)), "axx_batch_get_config")
+        return {"block": b.value, "wpt": w.value, "grid": g.value, "tiles": t.value}
+
+    def close(self):
+        if self._h:
+            lib().axx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

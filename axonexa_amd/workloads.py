@@ -18,6 +18,13 @@ Engine units: um, ms, T; gamma = 2.675e5 rad/(ms T); gradients in T/um
                64 profiles (PGSE, cos, sin, square wave x 16 timings) x
                30 directions x 3 amplitudes to 0.3 T/m
 
+  c4_batch     256 instances from a seeded Philox sampler: 128 single spheres
+               R ~ U(1, 8) um (walkers inside), 64 hexagonal cylinder
+               lattices R ~ U(1, 5) um, 64 gamma packings (40 cylinders);
+               walkers log-uniform in [1e4, 1e6]; each instance its own PGSE
+               or cosine-OGSE set of 4-32 waveforms; dt = 2e-3 ms, 10,000
+               steps
+
 The reference drivers these restate: examples/Cos_OneCyl.cu (one cylinder,
 cosine OGSE), examples/hexPhase.cu (hexagonal lattice), examples/gammadist.cu
 (gamma packing), examples/planePhase.cu / spherePhase.cu (validation).
@@ -139,3 +146,72 @@ def c3_gamma(n_walkers: int = 4000000, n_steps: int = 80000, dt: float = 5e-4,
     run = abi.RunSpec(n_walkers=n_walkers, seed=1234, relaxation=1)
     return Workload("c3_gamma", sub, acq, run,
                     "400 gamma-radius cylinders at 65% packing, 64 profiles x 30 dirs x 3 amps")
+
+
+def _c4_waveforms(rng: np.random.Generator, n_steps: int, dt: float):
+    """4..32 waveforms: PGSE (one timing, random directions, linear
+    amplitudes) or cosine OGSE (2, 4 or 8 frequencies x amplitudes along a
+    random axis)."""
+    T = (n_steps - 1) * dt
+    M = int(rng.integers(4, 33))
+    gmax = rng.uniform(0.05, 0.3) * T_PER_M
+    fs = []
+    if rng.random() < 0.5:
+        delta = rng.uniform(0.1, 0.4) * T
+        gap = rng.uniform(0.02, 1.0) * (T - 2 * delta - 2 * dt)
+        dirs = rng.normal(size=(M, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        for j in range(M):
+            fs.append(wf.RectGrad(gmax * (j + 1) / M, delta, gap, dirs[j]))
+    else:
+        nf = int(rng.choice([2, 4, 8]))
+        lobe = rng.uniform(0.3, 0.48) * T
+        gap = rng.uniform(0.0, 1.0) * (T - 2 * lobe - 2 * dt)
+        axis = np.zeros(3)
+        axis[int(rng.integers(0, 3))] = 1.0
+        for j in range(M):
+            periods = 1.0 + (j % nf)
+            fs.append(wf.CosGFunc(gmax * (1 + j // nf) / math.ceil(M / nf), lobe, gap, periods, axis))
+    return wf.build_acq(fs, n_steps, dt, GAMMA)
+
+
+def c4_batch(seed: int = 4, n_inst: int = 256, scale: float = 1.0, n_steps: int = 10000,
+             dt: float = 2e-3) -> List[Workload]:
+    """The batched uneven workload: instance parameters from a Philox
+    generator (numpy's Philox bit generator, key = seed); walker counts
+    log-uniform in [1e4, 1e6] times `scale`."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    n_sph, n_hex = n_inst // 2, n_inst // 4
+    for k in range(n_inst):
+        N = max(64, int(round(10.0 ** rng.uniform(4.0, 6.0) * scale)))
+        acq = _c4_waveforms(rng, n_steps, dt)
+        D_in, D_ex = rng.uniform(1.0, 2.5), rng.uniform(1.5, 3.0)
+        if k < n_sph:
+            R = rng.uniform(1.0, 8.0)
+            sub = sb.single_sphere(2.5 * R, R, D_in=D_in, D_ex=D_ex)
+            run = abi.RunSpec(n_walkers=N, seed=1234, instance=k, init_mode=abi.INIT_INSIDE_PRIM,
+                              init_arg=0)
+            name = f"sphere R={R:.2f}"
+        elif k < n_sph + n_hex:
+            R = rng.uniform(1.0, 5.0)
+            s = 2.0 * R * rng.uniform(1.05, 1.3)
+            p = rng.uniform(0.0, 1e-2)
+            sub = sb.hex_lattice(R, s, 2, 2, D_in=D_in, D_ex=D_ex, T2_in=rng.uniform(50, 100),
+                                 T2_ex=rng.uniform(40, 80), kappa=sb.kappa_for_probability(p, D_in, dt))
+            run = abi.RunSpec(n_walkers=N, seed=1234, instance=k,
+                              boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+            name = f"hex R={R:.2f} s={s:.2f} p={p:.1e}"
+        else:
+            theta = rng.uniform(0.25, 0.5)
+            frac = rng.uniform(0.4, 0.6)
+            p = rng.uniform(0.0, 1e-2)
+            sub = sb.gamma_packing(40, 5.0, theta, frac, D_in=D_in, D_ex=D_ex,
+                                   seed=int(rng.integers(1, 2 ** 31)), T2_in=rng.uniform(50, 100),
+                                   T2_ex=rng.uniform(40, 80),
+                                   kappa=sb.kappa_for_probability(p, D_in, dt), rmin=0.5)
+            run = abi.RunSpec(n_walkers=N, seed=1234, instance=k,
+                              boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+            name = f"gamma theta={theta:.2f} f={frac:.2f} p={p:.1e}"
+        out.append(Workload(f"c4_{k}", sub, acq, run, name))
+    return out

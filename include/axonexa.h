@@ -229,6 +229,44 @@ int  axx_plan_set_config(axx_plan* plan, int block, int wpt);
 int  axx_plan_profile_count(const axx_plan* plan);
 void axx_plan_destroy(axx_plan* plan);
 
+/*
+ * Batched runs: many substrates and acquisitions of uneven walker counts in
+ * one persistent launch, captured together with placement and reduction into
+ * one hipGraph.  No reference counterpart beyond the loop over acquisitions
+ * of phaseAcquisitionStream<F>::runAcquisitionStreamLattice
+ * (phaseAcquisitionStream.cuh:425-613); every instance gives bitwise the
+ * walkers and sums of its own axx_run_phase.  Limits: at most 8 profiles per
+ * instance, no rejection policy, n_walkers < 2^31.  Instances keep their own
+ * seed / instance id / walker offset, so replicas stay independent.
+ */
+typedef struct axx_instance {
+    const axx_substrate* substrate;
+    const axx_acq* acq;
+    const axx_run* run;
+} axx_instance;
+typedef struct axx_batch axx_batch;
+/* Build every instance's tables and state, the tile list and the graph.
+ * tune: block (threads, <= 512) and lds_budget (default 72 KiB per instance)
+ * are honoured; block 0 takes the cached autotuned choice or 256. */
+int  axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* inst,
+                      const axx_tuning* tune, axx_batch** out);
+int  axx_batch_run(axx_batch* b);                    /* launch the graph (async)   */
+int  axx_batch_time(axx_batch* b, float* ms);        /* run once, HIP-event timed  */
+int  axx_batch_sync(axx_batch* b);
+int  axx_batch_results(axx_batch* b, int i, axx_result* out);   /* synchronises */
+int  axx_batch_get_state(axx_batch* b, int i, uint32_t* host_state);
+/* Time a probe walk of probe_steps steps for each (threads per block,
+ * walkers per thread) candidate, keep the fastest, re-capture the graph and
+ * cache the choice (per process; also appended to the file named by the
+ * AXX_TUNE_CACHE environment variable, read at the first batch). */
+int  axx_batch_autotune(axx_batch* b, int probe_steps, int* best_block, int* best_wpt,
+                        float* best_ms);
+int  axx_batch_set_config(axx_batch* b, int block, int wpt);
+int  axx_batch_get_config(const axx_batch* b, int* block, int* wpt, int* grid, int* n_tiles);
+void axx_batch_destroy(axx_batch* b);
+/* One-shot: create, run, copy every instance's results (out[n_inst]), destroy. */
+int  axx_run_batch(axx_ctx* ctx, int n_inst, const axx_instance* inst, axx_result* out);
+
 /* Library version string. */
 const char* axx_version(void);
 

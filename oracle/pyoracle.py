@@ -40,6 +40,7 @@ def lib():
         L.orc_reduce.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                                  u32p, C.POINTER(abi.Result)]
         L.orc_image_count.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
+        L.orc_axes_mask.argtypes = [C.POINTER(abi.Acq)]
         L.orc_hit_test.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                    C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                    C.POINTER(C.c_float)]
@@ -82,6 +83,12 @@ def validate(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:
 def image_count(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:
     s, q = sub.struct(), acq.struct()
     return lib().orc_image_count(C.byref(s), C.byref(q))
+
+
+def axes_mask(acq: abi.AcqSpec) -> int:
+    """Moment axes the walk keeps (bit 0 = x, 1 = y, 2 = z)."""
+    q = acq.struct()
+    return lib().orc_axes_mask(C.byref(q))
 
 
 def hit_test(ptype, center, R, inside, s3, u3, ell):

@@ -7,6 +7,17 @@ FLOAT_FIELDS = ("pos", "disp", "r0", "E", "lf", "mom")
 INT_FIELDS = ("comp", "comp0", "li", "ncoll", "ntran", "ncap", "step")
 
 
+def mask_moments(st, n_walkers, n_profiles, axes):
+    """Copy of a state with the moments of axes outside the bit mask zeroed
+    (a walk that keeps every axis vs one that keeps only those in use)."""
+    st = np.array(st, copy=True)
+    m = abi.state_view(st, n_walkers, n_profiles)["mom"]
+    for a in range(3):
+        if not (axes >> a) & 1:
+            m[:, :, a] = 0.0
+    return st
+
+
 def compare_states(gst, ost, n_walkers, n_profiles, rtol=1e-5):
     """Integer fields must be identical; float fields within rtol relative
     (scale: max(|x|, 1)).  Returns a dict of diagnostics."""

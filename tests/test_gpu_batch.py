@@ -1,0 +1,106 @@
+"""Batched persistent launch (axx_batch_*) vs the CPU oracle and vs single
+runs.
+
+Every instance of a batch keeps its own seed, instance id and walker ids, so
+its walkers must be bitwise those of the oracle (and of its own single run):
+the batch differs only in scheduling (tiles taken from a work counter by a
+persistent grid) and in keeping moments for every axis (the oracle keeps the
+axes in use; the others are masked before comparing).
+"""
+import numpy as np
+import pytest
+
+from axonexa_amd import abi
+from axonexa_amd import waveforms as wf
+from axonexa_amd import workloads as W
+from oracle import pyoracle as po
+from tests.helpers import compare_states, mask_moments
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from axonexa_amd import engine
+    return engine
+
+
+def _check_instance(b, i, w, label):
+    gsig = b.results(i)
+    gst = b.get_state(i)
+    ost = po.walk(w.sub, w.acq, w.run)
+    osig = po.reduce(w.sub, w.acq, w.run, ost)
+    n, P = w.run.n_walkers, w.acq.n_profiles
+    info = compare_states(mask_moments(gst, n, P, po.axes_mask(w.acq)), ost, n, P)
+    assert info["bitwise_fraction"] == 1.0, (label, info)
+    for k in ("sum_cos", "sum_sin", "sum_cos2", "sum_sin2", "intra_cos", "intra_sin"):
+        assert np.allclose(getattr(gsig, k), getattr(osig, k), rtol=1e-12, atol=1e-9 * n), (label, k)
+    assert (gsig.collisions, gsig.transmissions, gsig.cap_hits, gsig.n_intra) == \
+        (osig.collisions, osig.transmissions, osig.cap_hits, osig.n_intra), label
+    return gsig
+
+
+def test_batch_of_256_instances_matches_oracle():
+    ws = W.c4_batch(scale=0.01, n_steps=300)
+    print(f"[batch] 256 instances, {sum(w.run.n_walkers for w in ws)} walkers x 300 steps", flush=True)
+    eng = _engine()
+    with eng.Batch([(w.sub, w.acq, w.run) for w in ws]) as b:
+        print(f"[batch] config {b.config()}", flush=True)
+        b.run()
+        coll = 0
+        for i, w in enumerate(ws):
+            coll += _check_instance(b, i, w, f"{w.name} {w.description}").collisions
+    assert coll > 10000
+
+
+def test_batch_equals_single_runs_and_graph_replays():
+    ws = W.c4_batch(n_inst=12, scale=0.02, n_steps=500, seed=11)
+    eng = _engine()
+    single = [eng.run_phase(w.sub, w.acq, w.run) for w in ws]
+    print("[batch] 12 instances: single runs done", flush=True)
+    with eng.Batch([(w.sub, w.acq, w.run) for w in ws]) as b:
+        for rep in range(3):
+            b.run()
+            for i, w in enumerate(ws):
+                g = b.results(i)
+                assert np.array_equal(g.sum_cos, single[i].sum_cos), (rep, i)
+                assert np.array_equal(g.sum_sin, single[i].sum_sin), (rep, i)
+                assert g.collisions == single[i].collisions
+            # eager work between graph replays
+            eng.run_phase(ws[0].sub, ws[0].acq, ws[0].run)
+            print(f"[batch] replay {rep} identical", flush=True)
+
+
+def test_batch_autotune_keeps_results():
+    ws = W.c4_batch(n_inst=16, scale=0.02, n_steps=400, seed=5)
+    eng = _engine()
+    with eng.Batch([(w.sub, w.acq, w.run) for w in ws]) as b:
+        b.run()
+        before = [b.results(i).sum_cos for i in range(len(ws))]
+        cfg = b.autotune(probe_steps=50)
+        print(f"[batch] autotune {cfg} -> {b.config()}", flush=True)
+        assert cfg["block"] in (256, 512) and cfg["wpt"] in (1, 2, 4)
+        b.run()
+        for i in range(len(ws)):
+            assert np.array_equal(b.results(i).sum_cos, before[i]), i
+        for blk, wpt in ((64, 1), (512, 3)):
+            b.set_config(blk, wpt)
+            b.run()
+            for i in range(len(ws)):
+                assert np.array_equal(b.results(i).sum_cos, before[i]), (blk, wpt, i)
+
+
+def test_batch_rejects_what_it_cannot_run():
+    ws = W.c4_batch(n_inst=2, scale=0.01, n_steps=100, seed=3)
+    eng = _engine()
+    w = ws[1]
+    rej = abi.RunSpec(n_walkers=100, seed=1, boundary_mode=abi.BOUNDARY_REJECTION)
+    with pytest.raises(eng.AxxError) as e:
+        eng.Batch([(w.sub, w.acq, rej)])
+    assert e.value.code == -1
+    # nine distinct profiles: above the batch limit of eight
+    fs = [wf.CosGFunc(1e-7, 0.08, 0.02, 1.0 + k, (1, 0, 0)) for k in range(9)]
+    wide = wf.build_acq(fs, 100, 2e-3, W.GAMMA)
+    assert wide.n_profiles == 9
+    with pytest.raises(eng.AxxError) as e:
+        eng.Batch([(w.sub, wide, w.run)])
+    assert e.value.code == -4
