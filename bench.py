@@ -16,6 +16,7 @@ small PGSE workloads.
 Prints one JSON line.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -66,7 +67,10 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2_hex",
-                    choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder"])
+                    choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder", "c4_batch"])
+    ap.add_argument("--no-unbatched", action="store_true",
+                    help="c4_batch: skip timing the instances one plan at a time")
+    ap.add_argument("--scale", type=float, default=1.0, help="c4_batch: walker-count scale")
     ap.add_argument("--walkers", type=int, default=0, help="0: the workload's own count")
     ap.add_argument("--n-steps", type=int, default=0, help="time steps per walk (0: the workload's)")
     ap.add_argument("--cpu-walkers", type=int, default=32768,
@@ -81,6 +85,9 @@ def main(argv=None) -> int:
                                    "(walkers are independent: a multi-GPU run would give each "
                                    "device its own walker-id range)"}))
         return 2
+
+    if args.workload == "c4_batch":
+        return bench_batch(args)
 
     import numpy as np
 
@@ -208,6 +215,132 @@ def main(argv=None) -> int:
     print(json.dumps(out))
     plan.close()
     return 0
+
+
+def bench_batch(args) -> int:
+    """The batched uneven workload: 256 instances in one graph launch
+    (persistent walk kernel + batched placement and reduction)."""
+    import numpy as np
+
+    from axonexa_amd import abi, engine
+    from axonexa_amd import workloads as W
+
+    kw = {"scale": args.scale}
+    if args.n_steps:
+        kw["n_steps"] = args.n_steps
+    ws = W.c4_batch(**kw)
+    n = ws[0].acq.n_steps
+    insts = [(w.sub, w.acq, w.run) for w in ws]
+    t0 = time.perf_counter()
+    b = engine.Batch(insts, block=args.block)
+    t_create = time.perf_counter() - t0
+    tune = b.autotune(probe_steps=200) if not args.block else {}
+    cfg = b.config()
+    for _ in range(args.warmup):
+        b.run().sync()
+    graph_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph_ms.append(b.time())
+    elapsed = time.perf_counter() - t0
+    total_ws = sum(w.walker_steps for w in ws)
+    value = total_ws * args.steps / elapsed
+    sigs = [b.results(i) for i in range(len(ws))]
+    flops = 0.0
+    coll = 0
+    for w, sg in zip(ws, sigs):
+        flops += w.flops_per_walker_step(sg.collisions / w.walker_steps) * w.walker_steps
+        coll += sg.collisions
+    avg_ms = float(np.mean(graph_ms))
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": "walker-steps/s",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "c4_batch: 128 single spheres R~U(1,8), 64 hex lattices R~U(1,5), "
+                        "64 gamma packings; walkers log-uniform 1e4..1e6; 4-32 PGSE/OGSE "
+                        "waveforms each; one hipGraph launch",
+            "instances": len(ws),
+            "walkers": int(sum(w.run.n_walkers for w in ws)),
+            "time_steps": n,
+            "block": cfg["block"], "walkers_per_thread": cfg["wpt"], "grid": cfg["grid"],
+            "tiles": cfg["tiles"], "autotune": tune, "create_s": t_create,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "graph_ms": avg_ms,
+            "note": "graph time (placement + persistent walk + reduction) as the duration",
+            "collisions_per_walker_step": coll / total_ws,
+        },
+    }
+    if not args.no_unbatched:
+        # the same instances one plan at a time (one walk + reduction launch each)
+        tu = 0.0
+        for w in ws:
+            with engine.Plan(w.sub, w.acq, w.run) as p:
+                p.sync()
+                tp = time.perf_counter()
+                p.walk().reduce().sync()
+                tu += time.perf_counter() - tp
+        out["unbatched"] = {"value": total_ws / tu, "seconds": tu,
+                            "speedup_of_batch": value / (total_ws / tu)}
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        k_per = max(1, args.cpu_walkers // 64)
+        threads = _cpu_threads()
+        tc = 0.0
+        ws_cpu = 0
+        bit_all = True
+        for i, w in enumerate(ws):
+            k = min(k_per, w.run.n_walkers)
+            run_k = dataclasses.replace(w.run, n_walkers=k)
+            t1 = time.perf_counter()
+            ost = po.walk(w.sub, w.acq, run_k, threads=threads)
+            tc += time.perf_counter() - t1
+            ws_cpu += k * (n - 1)
+            gst = b.get_state(i).reshape(-1, w.run.n_walkers)[:, :k].reshape(-1)
+            gst = _mask_axes(gst, k, w.acq.n_profiles, po.axes_mask(w.acq))
+            bit_all &= bool(np.array_equal(gst, ost))
+        out["cpu_baseline"] = {
+            "value": ws_cpu / tc, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {k_per} walkers of each of the {len(ws)} instances x {n} steps, "
+                      f"oracle, {threads} threads on {_cpu_model()}",
+            "seconds": tc,
+        }
+        out["vs_cpu"] = value / (ws_cpu / tc)
+        out["signal_error_vs_oracle"] = {"sample_bitwise_equal": bit_all}
+    print(json.dumps(out))
+    b.close()
+    return 0
+
+
+def _mask_axes(st, n_walkers, n_profiles, axes):
+    """Zero the moments of axes outside `axes` (the batch keeps every axis,
+    the oracle only those in use)."""
+    from axonexa_amd import abi
+
+    st = st.copy()
+    m = abi.state_view(st, n_walkers, n_profiles)["mom"]
+    for a in range(3):
+        if not (axes >> a) & 1:
+            m[:, :, a] = 0.0
+    return st
 
 
 if __name__ == "__main__":
