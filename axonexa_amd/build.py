@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = ["capi.hip", "batch.hip"] + [f"walk_p{p}.hip" for p in (1, 2, 4, 8, 16, 32, 64)]
+SOURCES = ["capi.hip", "batch.hip", "mlp.hip"] + [f"walk_p{p}.hip" for p in (1, 2, 4, 8, 16, 32, 64)]
 LIBNAME = "libaxonexa.so"
 
 

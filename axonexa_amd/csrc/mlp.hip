@@ -1,0 +1,742 @@
+// mlp.hip — the learned surrogate: a BF16 multilayer perceptron on MFMA with
+// FP32 accumulation, and its hand-written training kernels.
+//
+// No counterpart in the reference (it has no learned component).  The
+// network maps geometry / tissue scalars plus a sampled gradient waveform to
+// the normalised signal of several gradient directions; hidden layers use
+// GELU (tanh form), the last layer is linear.
+//
+// Layout.  Every feature dimension is padded to a multiple of 128 (the GEMM
+// tile) and batches to a multiple of 512 (tiles x split-K slices of 64);
+// padded weights, biases and activations are zero.  Activations are row
+// major [batch][features] in bf16.  Each layer keeps fp32 master weights
+// Wt[out][in] (+ bias) for Adam and two bf16 copies: Wt[out][in] (forward:
+// both operands contiguous along the reduction) and W[in][out] (input
+// gradient).
+//
+// Kernels.
+//   gemm_nt<EPI>  C[m][n] = sum_k A[m][k] * B[n][k], 128 x 128 x 64 block
+//                 tiles through LDS, 4 waves of 64 x 64, v_mfma_f32_32x32x16_bf16;
+//                 fused epilogues (bias + GELU, bias, GELU backward, split-K
+//                 fp32 partial);
+//   transpose_bf16, mse_grad, bias_grad, adam (sums the split-K partials in
+//   a fixed order: deterministic), gather (minibatch rows by Philox index).
+#include "internal.h"
+
+namespace axx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBM = 128, kBN = 128, kBK = 64, kLDK = kBK + 8;
+constexpr int kSplitK = 8;   // weight-gradient GEMMs split the batch reduction
+
+enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_BWD = 2, EPI_PARTIAL = 3 };
+
+struct EpiArgs {
+    const float* __restrict__ bias;    // [N] (EPI_BIAS*)
+    __bf16* __restrict__ out_bf;       // bf16 output [M][ldc]
+    __bf16* __restrict__ out_z;        // pre-activation [M][ldc] (EPI_BIAS_GELU)
+    float* __restrict__ out_f;         // fp32 output [M][ldc] (EPI_BIAS, EPI_PARTIAL: + slice * M * ldc)
+    const __bf16* __restrict__ zprev;  // pre-activation for GELU backward [M][ldc]
+    int ldc;
+};
+
+__device__ __forceinline__ float gelu_f(float x) {
+    const float c = 0.7978845608028654f;   // sqrt(2/pi)
+    const float u = c * (x + 0.044715f * x * x * x);
+    return 0.5f * x * (1.0f + tanhf(u));
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    const float x2 = x * x;
+    const float t = tanhf(c * (x + 0.044715f * x2 * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x2);
+}
+
+// C = A . B^T over k in [kbeg, kend); A [M][lda], B [N][ldb] bf16, M % 128,
+// N % 128 and the k range % 64 are zero (checked by the host).  Grid
+// (N / 128, M / 128, slices).
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_nt(const __bf16* __restrict__ A, int lda,
+                                               const __bf16* __restrict__ B, int ldb, int M, int K_slice,
+                                               EpiArgs E) {
+    __shared__ __align__(16) __bf16 As[kBM][kLDK];
+    __shared__ __align__(16) __bf16 Bs[kBN][kLDK];
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    const int kbeg = blockIdx.z * K_slice, kend = kbeg + K_slice;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+#pragma unroll
+        for (int c = tid; c < kBM * kBK / 8; c += 256) {
+            const int r = c >> 3, kc = (c & 7) * 8;
+            *reinterpret_cast<uint4*>(&As[r][kc]) =
+                *reinterpret_cast<const uint4*>(&A[(size_t)(m0 + r) * lda + k0 + kc]);
+            *reinterpret_cast<uint4*>(&Bs[r][kc]) =
+                *reinterpret_cast<const uint4*>(&B[(size_t)(n0 + r) * ldb + k0 + kc]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + i * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + j * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // acc[i][j][4 * q + r]: row 8q + 4 (lane / 32) + r, column lane % 32 of the
+    // 32 x 32 tile (i, j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+            float bcol = 0.0f;
+            if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS) bcol = E.bias[col];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5) + r;
+                    const size_t o = (size_t)row * E.ldc + col;
+                    const float v = acc[i][j][4 * q + r];
+                    if (EPI == EPI_BIAS_GELU) {
+                        const float z = v + bcol;
+                        E.out_z[o] = (__bf16)z;
+                        E.out_bf[o] = (__bf16)gelu_f(z);
+                    } else if (EPI == EPI_BIAS) {
+                        E.out_f[o] = v + bcol;
+                    } else if (EPI == EPI_GELU_BWD) {
+                        E.out_bf[o] = (__bf16)(v * gelu_grad((float)E.zprev[o]));
+                    } else {
+                        E.out_f[(size_t)blockIdx.z * M * E.ldc + o] = v;
+                    }
+                }
+        }
+}
+
+// out[c][r] = in[r][c]; R and C multiples of 64.  Grid (C / 64, R / 64), 256 threads.
+__global__ __launch_bounds__(256) void transpose_bf16(const __bf16* __restrict__ in, int ldi,
+                                                      __bf16* __restrict__ out, int ldo) {
+    __shared__ __bf16 t[64][66];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        t[r][c] = in[(size_t)(r0 + r) * ldi + c0 + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        out[(size_t)(c0 + c) * ldo + r0 + r] = t[r][c];
+    }
+}
+
+// Mean-squared error over the first n_out of ldy outputs of n rows:
+// loss_sum[0] += sum (y - t)^2 (one partial per block, added in block order
+// by the host-side copy of `partial`), dy = 2 (y - t) / (n * n_out) in bf16,
+// zero in the padding.
+__global__ __launch_bounds__(256) void mse_grad(const float* __restrict__ y, const float* __restrict__ t,
+                                                int n, int n_out, int ldy, float scale,
+                                                __bf16* __restrict__ dy, float* __restrict__ partial) {
+    __shared__ float sh[256];
+    float s = 0.0f;
+    const long long total = (long long)n * ldy;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int r = (int)(e / ldy), c = (int)(e % ldy);
+        float g = 0.0f;
+        if (c < n_out) {
+            const float d = y[e] - t[(size_t)r * n_out + c];
+            s += d * d;
+            g = 2.0f * d * scale;
+        }
+        dy[e] = (__bf16)g;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// db[c] = sum over n rows of dz[r][c]: one thread per column, rows in order.
+__global__ void bias_grad(const __bf16* __restrict__ dz, int n, int ld, float* __restrict__ db) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ld) return;
+    float s = 0.0f;
+    for (int r = 0; r < n; ++r) s += (float)dz[(size_t)r * ld + c];
+    db[c] = s;
+}
+
+// Adam on the fp32 master weights Wt[out][in] (gradient = sum of kSplitK
+// partials in slice order) and biases; writes the bf16 copies Wt and W.
+struct AdamArgs {
+    float* __restrict__ w;          // [outp][inp] master
+    float* __restrict__ mw;
+    float* __restrict__ vw;
+    const float* __restrict__ gpart;   // [kSplitK][outp][inp]
+    float* __restrict__ b;          // [outp]
+    float* __restrict__ mb;
+    float* __restrict__ vb;
+    const float* __restrict__ gb;   // [outp]
+    __bf16* __restrict__ wt_bf;     // [outp][inp]
+    __bf16* __restrict__ w_bf;      // [inp][outp]
+    int outp, inp, out_real, in_real;
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+};
+
+__global__ __launch_bounds__(256) void adam_kernel(AdamArgs P) {
+    const long long nw = (long long)P.outp * P.inp;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e < nw) {
+        const int o = (int)(e / P.inp), i = (int)(e % P.inp);
+        float w = P.w[e];
+        if (o < P.out_real && i < P.in_real) {
+            float g = 0.0f;
+            for (int s = 0; s < kSplitK; ++s) g += P.gpart[(size_t)s * nw + e];
+            const float m = P.b1 * P.mw[e] + (1.0f - P.b1) * g;
+            const float v = P.b2 * P.vw[e] + (1.0f - P.b2) * g * g;
+            P.mw[e] = m;
+            P.vw[e] = v;
+            w -= P.lr * (m / P.c1) / (sqrtf(v / P.c2) + P.eps);
+            P.w[e] = w;
+        }
+        P.wt_bf[e] = (__bf16)w;
+        P.w_bf[(size_t)i * P.outp + o] = (__bf16)w;
+    }
+    if (e < P.outp && e < P.out_real) {
+        const float g = P.gb[e];
+        const float m = P.b1 * P.mb[e] + (1.0f - P.b1) * g;
+        const float v = P.b2 * P.vb[e] + (1.0f - P.b2) * g * g;
+        P.mb[e] = m;
+        P.vb[e] = v;
+        P.b[e] -= P.lr * (m / P.c1) / (sqrtf(v / P.c2) + P.eps);
+    }
+}
+
+// Minibatch rows: row j of the batch is dataset row idx_j, idx_j = Philox
+// word mod n (counter (j, step, 3 << 24, 0), key = seed).
+__global__ void gather_rows(const __bf16* __restrict__ X, int ldx, const float* __restrict__ T, int n_out,
+                            int n_data, int batch, uint32_t step, uint32_t k0, uint32_t k1,
+                            __bf16* __restrict__ xb, float* __restrict__ tb) {
+    const int j = blockIdx.x;
+    if (j >= batch) return;
+    const u32x4 o = philox10((uint32_t)j, step, 3u << 24, 0u, k0, k1);
+    const int r = (int)(o.x % (uint32_t)n_data);
+    for (int c = threadIdx.x; c < ldx; c += blockDim.x) xb[(size_t)j * ldx + c] = X[(size_t)r * ldx + c];
+    for (int c = threadIdx.x; c < n_out; c += blockDim.x) tb[(size_t)j * n_out + c] = T[(size_t)r * n_out + c];
+}
+
+// fp32 [n][in_real] -> bf16 [n][ldx] with zero padding.
+__global__ void to_bf16_padded(const float* __restrict__ x, int n, int in_real, int ldx, __bf16* __restrict__ out) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * ldx) return;
+    const int r = (int)(e / ldx), c = (int)(e % ldx);
+    out[e] = (__bf16)(c < in_real ? x[(size_t)r * in_real + c] : 0.0f);
+}
+
+// Copy the first n_out columns of y [n][ldy] to out [n][n_out].
+__global__ void unpad_f32(const float* __restrict__ y, int n, int ldy, int n_out, float* __restrict__ out) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n_out) return;
+    const int r = (int)(e / n_out), c = (int)(e % n_out);
+    out[e] = y[(size_t)r * ldy + c];
+}
+
+}  // namespace axx
+
+// ---------------------------------------------------------------------------
+// Host side
+
+namespace {
+
+int pad_to(int v, int m) { return (v + m - 1) / m * m; }
+
+struct Layer {
+    int in = 0, out = 0, inp = 0, outp = 0;   // real and padded sizes
+    bool gelu = true;
+    float *w = nullptr, *mw = nullptr, *vw = nullptr, *b = nullptr, *mb = nullptr, *vb = nullptr;
+    float *gpart = nullptr, *gb = nullptr;
+    __bf16 *wt_bf = nullptr, *w_bf = nullptr;
+};
+
+}  // namespace
+
+struct axx_mlp {
+    axx_ctx* ctx = nullptr;
+    std::vector<Layer> L;
+    int max_batch = 0;
+    int adam_t = 0;
+    // activations: H[l] = input of layer l ([max_batch][L[l].inp]); Z[l] = pre-activation
+    // of hidden layer l ([max_batch][L[l].outp]); Y = output fp32
+    std::vector<__bf16*> H, Z;
+    float* Y = nullptr;
+    __bf16 *dA = nullptr, *dB = nullptr;        // gradient ping-pong [max_batch][maxdim]
+    __bf16 *HT = nullptr, *dZT = nullptr;       // transposes [maxdim][max_batch]
+    float* T = nullptr;                         // targets of a batch [max_batch][out]
+    float* loss_part = nullptr;                 // [kLossBlocks]
+    float* xin = nullptr;                       // fp32 staging [max_batch][in]
+    // resident dataset
+    __bf16* dX = nullptr;
+    float* dT = nullptr;
+    int n_data = 0;
+};
+
+namespace {
+
+constexpr int kLossBlocks = 256;
+
+void mlp_free(axx_mlp* m) {
+    if (!m) return;
+    for (Layer& l : m->L) {
+        void* bufs[] = {l.w, l.mw, l.vw, l.b, l.mb, l.vb, l.gpart, l.gb, l.wt_bf, l.w_bf};
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
+    }
+    for (__bf16* p : m->H)
+        if (p) (void)hipFree(p);
+    for (__bf16* p : m->Z)
+        if (p) (void)hipFree(p);
+    void* bufs[] = {m->Y, m->dA, m->dB, m->HT, m->dZT, m->T, m->loss_part, m->xin, m->dX, m->dT};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    delete m;
+}
+
+template <typename T>
+int dalloc(axx_ctx* ctx, T** p, size_t n, bool zero = true) {
+    HIP_TRY(ctx, hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+    if (zero) HIP_TRY(ctx, hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return AXX_OK;
+}
+
+int launch_gemm(axx_ctx* ctx, int epi, const __bf16* A, int lda, const __bf16* B, int ldb, int M, int N, int K,
+                int slices, const EpiArgs& E) {
+    if (M % kBM || N % kBN || K % (kBK * slices))
+        return fail(ctx, AXX_ERR_INVALID, "gemm shape %d x %d x %d (slices %d) not tiled", M, N, K, slices);
+    const dim3 grid(N / kBN, M / kBM, slices);
+    const int ks = K / slices;
+    switch (epi) {
+    case EPI_BIAS_GELU: hipLaunchKernelGGL(gemm_nt<EPI_BIAS_GELU>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
+    case EPI_BIAS: hipLaunchKernelGGL(gemm_nt<EPI_BIAS>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
+    case EPI_GELU_BWD: hipLaunchKernelGGL(gemm_nt<EPI_GELU_BWD>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
+    default: hipLaunchKernelGGL(gemm_nt<EPI_PARTIAL>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+int transpose(axx_ctx* ctx, const __bf16* in, int R, int Cc, int ldi, __bf16* out, int ldo) {
+    hipLaunchKernelGGL(transpose_bf16, dim3(Cc / 64, R / 64), dim3(256), 0, ctx->stream, in, ldi, out, ldo);
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+// Forward over n rows already in H[0] (n % 128 == 0).
+int forward(axx_mlp* m, int n) {
+    axx_ctx* ctx = m->ctx;
+    const int nl = (int)m->L.size();
+    for (int l = 0; l < nl; ++l) {
+        const Layer& y = m->L[l];
+        EpiArgs E{};
+        E.bias = y.b;
+        E.ldc = y.outp;
+        int rc;
+        if (y.gelu) {
+            E.out_bf = m->H[l + 1];
+            E.out_z = m->Z[l];
+            rc = launch_gemm(ctx, EPI_BIAS_GELU, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.inp, 1, E);
+        } else {
+            E.out_f = m->Y;
+            rc = launch_gemm(ctx, EPI_BIAS, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.inp, 1, E);
+        }
+        if (rc) return rc;
+    }
+    return AXX_OK;
+}
+
+// Loss and gradients of the batch in H[0] / T (n rows): fills every layer's
+// gpart (split-K partials) and gb; returns the loss partials in loss_part.
+int backward(axx_mlp* m, int n) {
+    axx_ctx* ctx = m->ctx;
+    const int nl = (int)m->L.size();
+    const Layer& last = m->L[nl - 1];
+    hipLaunchKernelGGL(mse_grad, dim3(kLossBlocks), dim3(256), 0, ctx->stream, (const float*)m->Y,
+                       (const float*)m->T, n, last.out, last.outp, 1.0f / ((float)n * last.out), m->dA,
+                       m->loss_part);
+    HIP_TRY(ctx, hipGetLastError());
+    __bf16* dcur = m->dA;   // gradient w.r.t. the output of layer l (pre-activation for hidden)
+    __bf16* dnext = m->dB;
+    for (int l = nl - 1; l >= 0; --l) {
+        const Layer& y = m->L[l];
+        int rc;
+        // bias gradient
+        hipLaunchKernelGGL(bias_grad, dim3((y.outp + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const __bf16*)dcur, n, y.outp, y.gb);
+        HIP_TRY(ctx, hipGetLastError());
+        // weight gradient: gWt[o][i] = sum_r dZ[r][o] H[r][i] = (dZ^T)(H^T)^T
+        if ((rc = transpose(ctx, dcur, n, y.outp, y.outp, m->dZT, n))) return rc;
+        if ((rc = transpose(ctx, m->H[l], n, y.inp, y.inp, m->HT, n))) return rc;
+        EpiArgs E{};
+        E.out_f = y.gpart;
+        E.ldc = y.inp;
+        if ((rc = launch_gemm(ctx, EPI_PARTIAL, m->dZT, n, m->HT, n, y.outp, y.inp, n, kSplitK, E))) return rc;
+        // input gradient through the previous layer's GELU
+        if (l > 0) {
+            EpiArgs G{};
+            G.out_bf = dnext;
+            G.zprev = m->Z[l - 1];
+            G.ldc = y.inp;
+            if ((rc = launch_gemm(ctx, EPI_GELU_BWD, dcur, y.outp, y.w_bf, y.outp, n, y.inp, y.outp, 1, G)))
+                return rc;
+            std::swap(dcur, dnext);
+        }
+    }
+    return AXX_OK;
+}
+
+int adam_step(axx_mlp* m, float lr) {
+    axx_ctx* ctx = m->ctx;
+    m->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (Layer& y : m->L) {
+        AdamArgs P{};
+        P.w = y.w; P.mw = y.mw; P.vw = y.vw; P.gpart = y.gpart;
+        P.b = y.b; P.mb = y.mb; P.vb = y.vb; P.gb = y.gb;
+        P.wt_bf = y.wt_bf; P.w_bf = y.w_bf;
+        P.outp = y.outp; P.inp = y.inp; P.out_real = y.out; P.in_real = y.in;
+        P.lr = lr; P.b1 = b1; P.b2 = b2; P.eps = 1e-8f;
+        P.c1 = 1.0f - std::pow(b1, (float)m->adam_t);
+        P.c2 = 1.0f - std::pow(b2, (float)m->adam_t);
+        const long long nw = (long long)y.outp * y.inp;
+        hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, ctx->stream, P);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    return AXX_OK;
+}
+
+// bf16 copies of the master weights (after set_params / init): an Adam
+// kernel with lr = 0 rewrites them without changing the masters.
+int refresh_bf16(axx_mlp* m) {
+    axx_ctx* ctx = m->ctx;
+    for (Layer& y : m->L) {
+        AdamArgs P{};
+        P.w = y.w; P.mw = y.mw; P.vw = y.vw; P.gpart = y.gpart;
+        P.b = y.b; P.mb = y.mb; P.vb = y.vb; P.gb = y.gb;
+        P.wt_bf = y.wt_bf; P.w_bf = y.w_bf;
+        P.outp = y.outp; P.inp = y.inp; P.out_real = 0; P.in_real = 0;   // no update
+        P.lr = 0.0f; P.b1 = 0.9f; P.b2 = 0.999f; P.eps = 1e-8f; P.c1 = 1.0f; P.c2 = 1.0f;
+        const long long nw = (long long)y.outp * y.inp;
+        hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, ctx->stream, P);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    return AXX_OK;
+}
+
+int load_inputs(axx_mlp* m, const float* x_host, int n, int n_pad) {
+    axx_ctx* ctx = m->ctx;
+    const Layer& f = m->L[0];
+    HIP_TRY(ctx, hipMemsetAsync(m->xin, 0, (size_t)n_pad * f.in * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(m->xin, x_host, (size_t)n * f.in * sizeof(float), hipMemcpyHostToDevice,
+                                ctx->stream));
+    const long long tot = (long long)n_pad * f.inp;
+    hipLaunchKernelGGL(to_bf16_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float*)m->xin, n_pad, f.in, f.inp, m->H[0]);
+    HIP_TRY(ctx, hipGetLastError());
+    return AXX_OK;
+}
+
+}  // namespace
+
+extern "C" void axx_mlp_destroy(axx_mlp* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    mlp_free(m);
+}
+
+extern "C" int axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int max_batch, axx_mlp** out) {
+    if (!ctx || !dims || !out || n_layers < 1 || n_layers > 64 || max_batch < 1)
+        return fail(ctx, AXX_ERR_INVALID, "mlp: bad arguments");
+    *out = nullptr;
+    for (int l = 0; l <= n_layers; ++l)
+        if (dims[l] < 1 || dims[l] > 16384) return fail(ctx, AXX_ERR_INVALID, "mlp: dims[%d] = %d", l, dims[l]);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    axx_mlp* m = new (std::nothrow) axx_mlp();
+    if (!m) return AXX_ERR_NOMEM;
+    m->ctx = ctx;
+    m->max_batch = pad_to(max_batch, kBK * kSplitK);
+    int maxdim = 0;
+    for (int l = 0; l < n_layers; ++l) {
+        Layer y;
+        y.in = dims[l];
+        y.out = dims[l + 1];
+        // inputs of layer l are the padded outputs of layer l - 1
+        y.inp = l == 0 ? pad_to(y.in, kBN) : m->L[l - 1].outp;
+        y.outp = pad_to(y.out, kBN);
+        y.gelu = l + 1 < n_layers;
+        maxdim = std::max(maxdim, std::max(y.inp, y.outp));
+        m->L.push_back(y);
+    }
+    const size_t B = (size_t)m->max_batch;
+    int rc = AXX_OK;
+    for (Layer& y : m->L) {
+        const size_t nw = (size_t)y.outp * y.inp;
+        if ((rc = dalloc(ctx, &y.w, nw)) || (rc = dalloc(ctx, &y.mw, nw)) || (rc = dalloc(ctx, &y.vw, nw)) ||
+            (rc = dalloc(ctx, &y.b, y.outp)) || (rc = dalloc(ctx, &y.mb, y.outp)) ||
+            (rc = dalloc(ctx, &y.vb, y.outp)) || (rc = dalloc(ctx, &y.gpart, nw * kSplitK)) ||
+            (rc = dalloc(ctx, &y.gb, y.outp)) || (rc = dalloc(ctx, &y.wt_bf, nw)) ||
+            (rc = dalloc(ctx, &y.w_bf, nw)))
+            break;
+    }
+    m->H.assign(n_layers + 1, nullptr);
+    m->Z.assign(n_layers, nullptr);
+    for (int l = 0; !rc && l < n_layers; ++l) {
+        rc = dalloc(ctx, &m->H[l], B * m->L[l].inp);
+        if (!rc && m->L[l].gelu) {
+            rc = dalloc(ctx, &m->Z[l], B * m->L[l].outp);
+            if (!rc) rc = dalloc(ctx, &m->H[l + 1], B * m->L[l].outp);
+        }
+    }
+    const Layer& last = m->L.back();
+    if (!rc) rc = dalloc(ctx, &m->Y, B * last.outp);
+    if (!rc) rc = dalloc(ctx, &m->dA, B * maxdim);
+    if (!rc) rc = dalloc(ctx, &m->dB, B * maxdim);
+    if (!rc) rc = dalloc(ctx, &m->HT, B * maxdim);
+    if (!rc) rc = dalloc(ctx, &m->dZT, B * maxdim);
+    if (!rc) rc = dalloc(ctx, &m->T, B * last.out);
+    if (!rc) rc = dalloc(ctx, &m->loss_part, kLossBlocks);
+    if (!rc) rc = dalloc(ctx, &m->xin, B * m->L[0].in);
+    if (rc) {
+        mlp_free(m);
+        return rc;
+    }
+    *out = m;
+    return AXX_OK;
+}
+
+// Parameters, packed per layer: W[out][in] row major, then b[out] (fp32).
+extern "C" int axx_mlp_param_count(const axx_mlp* m, int64_t* n) {
+    if (!m || !n) return AXX_ERR_INVALID;
+    int64_t s = 0;
+    for (const Layer& y : m->L) s += (int64_t)y.out * y.in + y.out;
+    *n = s;
+    return AXX_OK;
+}
+
+extern "C" int axx_mlp_set_params(axx_mlp* m, const float* p) {
+    if (!m || !p) return AXX_ERR_INVALID;
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (Layer& y : m->L) {
+        std::vector<float> w((size_t)y.outp * y.inp, 0.0f), b(y.outp, 0.0f);
+        for (int o = 0; o < y.out; ++o)
+            for (int i = 0; i < y.in; ++i) w[(size_t)o * y.inp + i] = *p++;
+        for (int o = 0; o < y.out; ++o) b[o] = *p++;
+        HIP_TRY(ctx, hipMemcpy(y.w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(y.b, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemset(y.mw, 0, w.size() * sizeof(float)));
+        HIP_TRY(ctx, hipMemset(y.vw, 0, w.size() * sizeof(float)));
+        HIP_TRY(ctx, hipMemset(y.mb, 0, b.size() * sizeof(float)));
+        HIP_TRY(ctx, hipMemset(y.vb, 0, b.size() * sizeof(float)));
+    }
+    m->adam_t = 0;
+    int rc = refresh_bf16(m);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return AXX_OK;
+}
+
+extern "C" int axx_mlp_get_params(axx_mlp* m, float* p) {
+    if (!m || !p) return AXX_ERR_INVALID;
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (Layer& y : m->L) {
+        std::vector<float> w((size_t)y.outp * y.inp), b(y.outp);
+        HIP_TRY(ctx, hipMemcpy(w.data(), y.w, w.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(b.data(), y.b, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int o = 0; o < y.out; ++o)
+            for (int i = 0; i < y.in; ++i) *p++ = w[(size_t)o * y.inp + i];
+        for (int o = 0; o < y.out; ++o) *p++ = b[o];
+    }
+    return AXX_OK;
+}
+
+// y[n][out] = network(x[n][in]), host buffers; n <= max_batch.
+extern "C" int axx_mlp_forward(axx_mlp* m, const float* x, int n, float* y) {
+    if (!m || !x || !y || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp forward: n");
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int n_pad = pad_to(n, kBM);
+    int rc = load_inputs(m, x, n, n_pad);
+    if (!rc) rc = forward(m, n_pad);
+    if (rc) return rc;
+    const Layer& last = m->L.back();
+    float* tmp = m->xin;   // reuse the staging buffer for the unpadded output
+    if ((size_t)n * last.out > (size_t)m->max_batch * m->L[0].in) {
+        std::vector<float> full((size_t)n_pad * last.outp);
+        HIP_TRY(ctx, hipMemcpyAsync(full.data(), m->Y, full.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < last.out; ++c) y[(size_t)r * last.out + c] = full[(size_t)r * last.outp + c];
+        return AXX_OK;
+    }
+    const long long tot = (long long)n * last.out;
+    hipLaunchKernelGGL(unpad_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float*)m->Y, n, last.outp, last.out, tmp);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(y, tmp, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return AXX_OK;
+}
+
+// Forward over the n_pad rows already resident in the input buffer, timed
+// with HIP events (inputs staged by a previous axx_mlp_forward).
+extern "C" int axx_mlp_time_forward(axx_mlp* m, int n, float* ms) {
+    if (!m || !ms || n < 1 || n > m->max_batch) return AXX_ERR_INVALID;
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+    int rc = forward(m, pad_to(n, kBM));
+    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(e1));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Loss (mean squared error over the n x out targets) and gradients of the
+// mean loss w.r.t. the packed parameters, for the batch (x, t).
+extern "C" int axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int n, float* loss, float* grad) {
+    if (!m || !x || !t || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp grad: n");
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int n_pad = pad_to(n, kBK * kSplitK);   // M tiles and split-K slices
+    const Layer& last = m->L.back();
+    int rc = load_inputs(m, x, n, n_pad);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemsetAsync(m->T, 0, (size_t)n_pad * last.out * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(m->T, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = forward(m, n_pad))) return rc;
+    if (n_pad != n) {
+        // padded rows must not contribute: their targets equal their outputs
+        HIP_TRY(ctx, hipMemcpy2DAsync(m->T + (size_t)n * last.out, last.out * sizeof(float),
+                                      m->Y + (size_t)n * last.outp, last.outp * sizeof(float),
+                                      last.out * sizeof(float), n_pad - n, hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+    }
+    if ((rc = backward(m, n_pad))) return rc;
+    std::vector<float> part(kLossBlocks);
+    HIP_TRY(ctx, hipMemcpyAsync(part.data(), m->loss_part, kLossBlocks * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (float v : part) s += v;
+    if (loss) *loss = (float)(s / ((double)n * last.out));
+    // the scale of dY used n_pad rows; rescale to the mean over n rows
+    const float rescale = (float)n_pad / (float)n;
+    if (grad) {
+        for (Layer& y : m->L) {
+            const size_t nw = (size_t)y.outp * y.inp;
+            std::vector<float> gp(nw * kSplitK), gb(y.outp);
+            HIP_TRY(ctx, hipMemcpy(gp.data(), y.gpart, gp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_TRY(ctx, hipMemcpy(gb.data(), y.gb, gb.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int o = 0; o < y.out; ++o)
+                for (int i = 0; i < y.in; ++i) {
+                    float g = 0.0f;
+                    for (int sl = 0; sl < kSplitK; ++sl) g += gp[(size_t)sl * nw + (size_t)o * y.inp + i];
+                    *grad++ = g * rescale;
+                }
+            for (int o = 0; o < y.out; ++o) *grad++ = gb[o] * rescale;
+        }
+    }
+    return AXX_OK;
+}
+
+// Keep a dataset on the device for training: x[n][in], t[n][out].
+extern "C" int axx_mlp_set_data(axx_mlp* m, const float* x, const float* t, int n) {
+    if (!m || !x || !t || n < 1) return AXX_ERR_INVALID;
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (m->dX) (void)hipFree(m->dX);
+    if (m->dT) (void)hipFree(m->dT);
+    m->dX = nullptr;
+    m->dT = nullptr;
+    const Layer& f = m->L[0];
+    const Layer& last = m->L.back();
+    int rc = dalloc(ctx, &m->dX, (size_t)n * f.inp, false);
+    if (!rc) rc = dalloc(ctx, &m->dT, (size_t)n * last.out, false);
+    if (rc) return rc;
+    float* tmp = nullptr;
+    if ((rc = dalloc(ctx, &tmp, (size_t)n * f.in, false))) return rc;
+    HIP_TRY(ctx, hipMemcpy(tmp, x, (size_t)n * f.in * sizeof(float), hipMemcpyHostToDevice));
+    const long long tot = (long long)n * f.inp;
+    hipLaunchKernelGGL(to_bf16_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float*)tmp, n, f.in, f.inp, m->dX);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpy(m->dT, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(tmp);
+    m->n_data = n;
+    return AXX_OK;
+}
+
+// Adam training on the resident dataset: `steps` minibatches of `batch`
+// rows drawn with Philox (key = seed, counter = (row, global step)).
+// loss_trace[s] (optional) receives each step's minibatch loss.
+extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_t seed, float* loss_trace) {
+    if (!m || steps < 1 || batch < 1 || batch > m->max_batch || !m->dX)
+        return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp train: arguments or no data");
+    if (batch % (kBK * kSplitK) || batch % kBM)
+        return fail(m->ctx, AXX_ERR_INVALID, "mlp train: batch must be a multiple of %d", kBK * kSplitK);
+    axx_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Layer& f = m->L[0];
+    const Layer& last = m->L.back();
+    std::vector<float> part(kLossBlocks);
+    for (int s = 0; s < steps; ++s) {
+        hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(128), 0, ctx->stream, (const __bf16*)m->dX, f.inp,
+                           (const float*)m->dT, last.out, m->n_data, batch, (uint32_t)(m->adam_t + 1),
+                           (uint32_t)seed, (uint32_t)(seed >> 32), m->H[0], m->T);
+        HIP_TRY(ctx, hipGetLastError());
+        int rc = forward(m, batch);
+        if (!rc) rc = backward(m, batch);
+        if (!rc) rc = adam_step(m, lr);
+        if (rc) return rc;
+        if (loss_trace) {
+            HIP_TRY(ctx, hipMemcpyAsync(part.data(), m->loss_part, kLossBlocks * sizeof(float),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            double sum = 0.0;
+            for (float v : part) sum += v;
+            loss_trace[s] = (float)(sum / ((double)batch * last.out));
+        }
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return AXX_OK;
+}

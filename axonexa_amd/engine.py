@@ -81,6 +81,17 @@ def lib():
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.axx_batch_destroy.argtypes = [vp]
     L.axx_run_batch.argtypes = [vp, C.c_int, IP, Res]
+    fp = C.POINTER(C.c_float)
+    L.axx_mlp_create.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(vp)]
+    L.axx_mlp_param_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.axx_mlp_set_params.argtypes = [vp, fp]
+    L.axx_mlp_get_params.argtypes = [vp, fp]
+    L.axx_mlp_forward.argtypes = [vp, fp, C.c_int, fp]
+    L.axx_mlp_time_forward.argtypes = [vp, C.c_int, fp]
+    L.axx_mlp_loss_grad.argtypes = [vp, fp, fp, C.c_int, fp, fp]
+    L.axx_mlp_set_data.argtypes = [vp, fp, fp, C.c_int]
+    L.axx_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, fp]
+    L.axx_mlp_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -289,6 +300,87 @@ class Batch:
     def close(self):
         if self._h:
             lib().axx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """BF16 MLP on MFMA with hand-written training kernels (axx_mlp_*)."""
+
+    def __init__(self, dims, max_batch: int = 4096, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.dims = [int(d) for d in dims]
+        arr = (C.c_int * len(self.dims))(*self.dims)
+        self._h = C.c_void_p()
+        rc = lib().axx_mlp_create(self.ctx.handle, len(self.dims) - 1, arr, int(max_batch),
+                                  C.byref(self._h))
+        self.ctx.check(rc, "axx_mlp_create")
+        n = C.c_int64()
+        self.ctx.check(lib().axx_mlp_param_count(self._h, C.byref(n)), "axx_mlp_param_count")
+        self.n_params = n.value
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        self.ctx.check(lib().axx_mlp_set_params(self._h, _fptr(p)), "axx_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        self.ctx.check(lib().axx_mlp_get_params(self._h, _fptr(p)), "axx_mlp_get_params")
+        return p
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        n = x.shape[0]
+        y = np.empty((n, self.dims[-1]), np.float32)
+        self.ctx.check(lib().axx_mlp_forward(self._h, _fptr(x), n, _fptr(y)), "axx_mlp_forward")
+        return y
+
+    def time_forward(self, n: int) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().axx_mlp_time_forward(self._h, int(n), C.byref(ms)), "axx_mlp_time_forward")
+        return ms.value
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x), _f32(t)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        self.ctx.check(lib().axx_mlp_loss_grad(self._h, _fptr(x), _fptr(t), x.shape[0], C.byref(loss),
+                                               _fptr(g)), "axx_mlp_loss_grad")
+        return loss.value, g
+
+    def set_data(self, x, t):
+        x, t = _f32(x), _f32(t)
+        self.ctx.check(lib().axx_mlp_set_data(self._h, _fptr(x), _fptr(t), x.shape[0]), "axx_mlp_set_data")
+
+    def train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+        trace = np.empty(steps, np.float32)
+        self.ctx.check(lib().axx_mlp_train(self._h, int(steps), int(batch), float(lr), int(seed),
+                                           _fptr(trace)), "axx_mlp_train")
+        return trace
+
+    def close(self):
+        if self._h:
+            lib().axx_mlp_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

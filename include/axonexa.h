@@ -267,6 +267,31 @@ void axx_batch_destroy(axx_batch* b);
 /* One-shot: create, run, copy every instance's results (out[n_inst]), destroy. */
 int  axx_run_batch(axx_ctx* ctx, int n_inst, const axx_instance* inst, axx_result* out);
 
+/*
+ * Learned surrogate: a BF16 multilayer perceptron on MFMA (FP32 accumulate)
+ * with hand-written training kernels.  No reference counterpart.  dims[0]
+ * inputs, dims[n_layers] outputs; hidden layers use GELU (tanh form), the
+ * last is linear.  Parameters are packed per layer as W[out][in] (row
+ * major) then b[out], fp32.  Loss: mean squared error over all outputs.
+ */
+typedef struct axx_mlp axx_mlp;
+int  axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int max_batch, axx_mlp** out);
+int  axx_mlp_param_count(const axx_mlp* m, int64_t* n);
+int  axx_mlp_set_params(axx_mlp* m, const float* params);    /* also resets Adam */
+int  axx_mlp_get_params(axx_mlp* m, float* params);
+/* y[n][out] = f(x[n][in]) (host buffers, n <= max_batch). */
+int  axx_mlp_forward(axx_mlp* m, const float* x, int n, float* y);
+/* Re-run the forward pass over the inputs already on the device (the last
+ * axx_mlp_forward's) and time it with HIP events. */
+int  axx_mlp_time_forward(axx_mlp* m, int n, float* ms);
+/* Mean loss of (x, t) and its gradient w.r.t. the packed parameters. */
+int  axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int n, float* loss, float* grad);
+/* Training set kept on the device; Adam (b1 0.9, b2 0.999, eps 1e-8) over
+ * `steps` minibatches of `batch` rows (a multiple of 512) drawn with Philox. */
+int  axx_mlp_set_data(axx_mlp* m, const float* x, const float* t, int n);
+int  axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_t seed, float* loss_trace);
+void axx_mlp_destroy(axx_mlp* m);
+
 /* Library version string. */
 const char* axx_version(void);
 
