@@ -151,7 +151,8 @@ def pack_cylinders(radii: np.ndarray, box: float, rng: np.random.Generator,
         mask = ov > 0
         if not mask.any():
             return pos % box, r
-        f = np.where(mask, ov / np.maximum(dist, 1e-12), 0.0)
+        with np.errstate(invalid="ignore"):
+            f = np.where(mask, ov / np.maximum(dist, 1e-12), 0.0)
         pos += 0.5 * np.einsum("ij,ijk->ik", f, d) * 0.5
         pos %= box
     raise RuntimeError("packing did not converge to a non-overlapping state")
