@@ -30,6 +30,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBM = 128, kBN = 128, kBK = 64, kLDK = kBK + 8;
 constexpr int kSplitK = 8;   // weight-gradient GEMMs split the batch reduction
+constexpr int kBiasSlices = 32;   // bias gradients: row slices summed in order by Adam
 
 enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_BWD = 2, EPI_PARTIAL = 3 };
 
@@ -178,13 +179,30 @@ __global__ __launch_bounds__(256) void mse_grad(const float* __restrict__ y, con
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// db[c] = sum over n rows of dz[r][c]: one thread per column, rows in order.
+// out[0] = scale * sum of the n partial losses (fixed tree order).
+__global__ void sum_partials(const float* __restrict__ part, int n, float scale, float* __restrict__ out) {
+    __shared__ float sh[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] * scale;
+}
+
+// db[s][c] = sum over the rows of slice s (of kBiasSlices) of dz[r][c]:
+// adjacent threads take adjacent columns.  Grid (ld / 256, kBiasSlices).
 __global__ void bias_grad(const __bf16* __restrict__ dz, int n, int ld, float* __restrict__ db) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ld) return;
+    const int per = (n + kBiasSlices - 1) / kBiasSlices;
+    const int r0 = blockIdx.y * per, r1 = min(n, r0 + per);
     float s = 0.0f;
-    for (int r = 0; r < n; ++r) s += (float)dz[(size_t)r * ld + c];
-    db[c] = s;
+    for (int r = r0; r < r1; ++r) s += (float)dz[(size_t)r * ld + c];
+    db[(size_t)blockIdx.y * ld + c] = s;
 }
 
 // Adam on the fp32 master weights Wt[out][in] (gradient = sum of kSplitK
@@ -197,7 +215,7 @@ struct AdamArgs {
     float* __restrict__ b;          // [outp]
     float* __restrict__ mb;
     float* __restrict__ vb;
-    const float* __restrict__ gb;   // [outp]
+    const float* __restrict__ gb;   // [kBiasSlices][outp]
     __bf16* __restrict__ wt_bf;     // [outp][inp]
     __bf16* __restrict__ w_bf;      // [inp][outp]
     int outp, inp, out_real, in_real;
@@ -224,7 +242,8 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs P) {
         P.w_bf[(size_t)i * P.outp + o] = (__bf16)w;
     }
     if (e < P.outp && e < P.out_real) {
-        const float g = P.gb[e];
+        float g = 0.0f;
+        for (int s = 0; s < kBiasSlices; ++s) g += P.gb[(size_t)s * P.outp + e];
         const float m = P.b1 * P.mb[e] + (1.0f - P.b1) * g;
         const float v = P.b2 * P.vb[e] + (1.0f - P.b2) * g * g;
         P.mb[e] = m;
@@ -390,7 +409,7 @@ int backward(axx_mlp* m, int n) {
         const Layer& y = m->L[l];
         int rc;
         // bias gradient
-        hipLaunchKernelGGL(bias_grad, dim3((y.outp + 255) / 256), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(bias_grad, dim3((y.outp + 255) / 256, kBiasSlices), dim3(256), 0, ctx->stream,
                            (const __bf16*)dcur, n, y.outp, y.gb);
         HIP_TRY(ctx, hipGetLastError());
         // weight gradient: gWt[o][i] = sum_r dZ[r][o] H[r][i] = (dZ^T)(H^T)^T
@@ -504,7 +523,7 @@ extern "C" int axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int m
         if ((rc = dalloc(ctx, &y.w, nw)) || (rc = dalloc(ctx, &y.mw, nw)) || (rc = dalloc(ctx, &y.vw, nw)) ||
             (rc = dalloc(ctx, &y.b, y.outp)) || (rc = dalloc(ctx, &y.mb, y.outp)) ||
             (rc = dalloc(ctx, &y.vb, y.outp)) || (rc = dalloc(ctx, &y.gpart, nw * kSplitK)) ||
-            (rc = dalloc(ctx, &y.gb, y.outp)) || (rc = dalloc(ctx, &y.wt_bf, nw)) ||
+            (rc = dalloc(ctx, &y.gb, (size_t)y.outp * kBiasSlices)) || (rc = dalloc(ctx, &y.wt_bf, nw)) ||
             (rc = dalloc(ctx, &y.w_bf, nw)))
             break;
     }
@@ -662,7 +681,7 @@ extern "C" int axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int
     if (grad) {
         for (Layer& y : m->L) {
             const size_t nw = (size_t)y.outp * y.inp;
-            std::vector<float> gp(nw * kSplitK), gb(y.outp);
+            std::vector<float> gp(nw * kSplitK), gb((size_t)y.outp * kBiasSlices);
             HIP_TRY(ctx, hipMemcpy(gp.data(), y.gpart, gp.size() * sizeof(float), hipMemcpyDeviceToHost));
             HIP_TRY(ctx, hipMemcpy(gb.data(), y.gb, gb.size() * sizeof(float), hipMemcpyDeviceToHost));
             for (int o = 0; o < y.out; ++o)
@@ -671,7 +690,11 @@ extern "C" int axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int
                     for (int sl = 0; sl < kSplitK; ++sl) g += gp[(size_t)sl * nw + (size_t)o * y.inp + i];
                     *grad++ = g * rescale;
                 }
-            for (int o = 0; o < y.out; ++o) *grad++ = gb[o] * rescale;
+            for (int o = 0; o < y.out; ++o) {
+                float g = 0.0f;
+                for (int sl = 0; sl < kBiasSlices; ++sl) g += gb[(size_t)sl * y.outp + o];
+                *grad++ = g * rescale;
+            }
         }
     }
     return AXX_OK;
@@ -718,7 +741,9 @@ extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const Layer& f = m->L[0];
     const Layer& last = m->L.back();
-    std::vector<float> part(kLossBlocks);
+    float* d_loss = nullptr;
+    int rc0 = dalloc(ctx, &d_loss, (size_t)steps);
+    if (rc0) return rc0;
     for (int s = 0; s < steps; ++s) {
         hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(128), 0, ctx->stream, (const __bf16*)m->dX, f.inp,
                            (const float*)m->dT, last.out, m->n_data, batch, (uint32_t)(m->adam_t + 1),
@@ -727,16 +752,17 @@ extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_
         int rc = forward(m, batch);
         if (!rc) rc = backward(m, batch);
         if (!rc) rc = adam_step(m, lr);
-        if (rc) return rc;
-        if (loss_trace) {
-            HIP_TRY(ctx, hipMemcpyAsync(part.data(), m->loss_part, kLossBlocks * sizeof(float),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            double sum = 0.0;
-            for (float v : part) sum += v;
-            loss_trace[s] = (float)(sum / ((double)batch * last.out));
+        if (rc) {
+            (void)hipFree(d_loss);
+            return rc;
         }
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, ctx->stream, (const float*)m->loss_part,
+                           kLossBlocks, 1.0f / ((float)batch * last.out), d_loss + s);
     }
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && loss_trace)
+        e = hipMemcpy(loss_trace, d_loss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d_loss);
+    if (e != hipSuccess) return fail(ctx, AXX_ERR_DEVICE, "mlp train: %s", hipGetErrorString(e));
     return AXX_OK;
 }

@@ -4,6 +4,11 @@
         one device-to-device copy of a known size (calibrates the HBM byte
         counters FETCH_SIZE / WRITE_SIZE against a plain copy kernel)
 
+    python -m axonexa_amd.profiling mlp [--batch 65536] [--train-steps 20]
+        the surrogate network (268 -> 1024 x 6 -> 30): 5 forward passes at
+        the batch, then Adam steps at batch 4096 on random data (a short
+        program for a kernel trace)
+
     python -m axonexa_amd.profiling summarize <results.db> [--kernel NAME]
         print per-kernel dispatch count, mean duration and the mean of every
         collected counter per dispatch from a rocprofv3 SQLite database
@@ -26,6 +31,28 @@ def copy(mib: int) -> None:
     b.copy_(a)
     torch.cuda.synchronize()
     print(f"copied {mib} MiB ({n * 4} bytes read, {n * 4} bytes written)")
+
+
+def mlp(batch: int, train_steps: int) -> None:
+    import numpy as np
+
+    from . import engine
+
+    dims = [268] + [1024] * 6 + [30]
+    rng = np.random.default_rng(0)
+    with engine.MLP(dims, max_batch=max(batch, 4096)) as m:
+        p = np.concatenate([np.concatenate([rng.uniform(-1, 1, a * b) / np.sqrt(a), np.zeros(b)])
+                            for a, b in zip(dims[:-1], dims[1:])]).astype(np.float32)
+        m.set_params(p)
+        x = rng.uniform(-1, 1, size=(batch, 268)).astype(np.float32)
+        m.forward(x)
+        ms = [m.time_forward(batch) for _ in range(5)]
+        flops = 2.0 * batch * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+        print(f"forward batch {batch}: {np.median(ms):.3f} ms, {flops / np.median(ms) / 1e9:.1f} TFLOP/s")
+        t = rng.uniform(0, 1, size=(8192, 30)).astype(np.float32)
+        m.set_data(x[:8192], t)
+        tr = m.train(train_steps, batch=4096, lr=1e-3, seed=1)
+        print(f"{train_steps} Adam steps at batch 4096: loss {tr[0]:.4f} -> {tr[-1]:.4f}")
 
 
 def _kernels(con):
@@ -81,12 +108,17 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
+    ml = sub.add_parser("mlp")
+    ml.add_argument("--batch", type=int, default=65536)
+    ml.add_argument("--train-steps", type=int, default=20)
     s = sub.add_parser("summarize")
     s.add_argument("db")
     s.add_argument("--kernel", default="")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy(args.mib)
+    elif args.cmd == "mlp":
+        mlp(args.batch, args.train_steps)
     else:
         print(summarize(args.db, args.kernel))
 
