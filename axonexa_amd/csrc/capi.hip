@@ -252,8 +252,11 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         if (s->type[k] == AXX_PRIM_SPHERE) H.gen = true;
     }
     H.Lmax = Lmax;
-    // a plane counts as hit within plane_eps of it, so its reach is longer
-    const double reach = Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) + (any_plane ? 2.0 * H.plane_eps : 0.0);
+    // reach of a pair of steps (one lookup serves both; the first may end up
+    // to one step outside the cell); a plane counts as hit within plane_eps
+    // of it, so its reach is longer
+    const double reach = 2.0 * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
+                         (any_plane ? 4.0 * H.plane_eps : 0.0);
     for (int k = 0; k < n; ++k) {
         const int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
         const size_t first = H.geom.size();

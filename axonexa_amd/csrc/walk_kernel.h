@@ -203,9 +203,36 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     const bool permeable = A.mode == 1;
     float Lc = A.compL[comp];
 
+    // Surfaces within reach of the current start point: the bin's list,
+    // narrowed by the sub-bin nibble to one entry (or the whole list when
+    // several).  The lists cover two step lengths, so one lookup serves both
+    // steps of a pair; the start may lie up to one step outside the cell
+    // (clamped to the edge sub-bin, still within reach).
+    struct Cands {
+        int nrel;
+        const uint16_t* cl;
+    };
+    auto lookup = [&]() {
+        const int sbx = min(max((int)(x * A.ibx4), 0), 4 * A.nbx - 1);
+        const int sby = min(max((int)(y * A.iby4), 0), 4 * A.nby - 1);
+        const int bz = GEN ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
+        const uint32_t row = __umul24((uint32_t)bz, (uint32_t)A.nby) + ((uint32_t)sby >> 2);
+        const uint4 rec = T.bins[__umul24(row, (uint32_t)A.nbx) + ((uint32_t)sbx >> 2)];
+        const int sbi = (sbx & 3) | ((sby & 3) << 2);
+        const uint32_t nib = ((sbi < 8 ? rec.y : rec.z) >> ((sbi & 7) << 2)) & 0xFu;
+        const bool many = (nib & kNibMany) != 0u;
+        Cands c;
+        c.nrel = many ? (int)(rec.x & 0xFFu) : (nib == kNibNone ? 0 : 1);
+        c.cl = T.cand + (rec.x >> 8) + (many ? 0u : nib);
+        return c;
+    };
+
     // One step i: direction from the two Philox words, boundary handling,
-    // wrap, displacement and phase moments (kernelLattice.cuh:46-99).
-    auto step = [&](const int i, const uint32_t wa, const uint32_t wb) {
+    // displacement, phase moments; the position is wrapped into the cell
+    // after the second step of each pair (i even) (kernelLattice.cuh:46-99).
+    auto step = [&](const int i, const uint32_t wa, const uint32_t wb, const Cands C, const bool wrap) {
+        const int nrel = C.nrel;
+        const uint16_t* cl = C.cl;
         // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
         const float ct = unif(wb) * 2.0f - 1.0f;
         const float sth = sqrt_unit((1.0f - ct) * (1.0f + ct));
@@ -214,17 +241,6 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         float ux = sth * cphi, uy = sth * sphi, uz = ct;
         float ell = Lc;
         float sx = x, sy = y, sz = z;
-        // surfaces within reach of the start: the bin's list, narrowed by the
-        // sub-bin nibble to one entry (or the whole list when several)
-        const int sbx = min(max((int)(x * A.ibx4), 0), 4 * A.nbx - 1);
-        const int sby = min(max((int)(y * A.iby4), 0), 4 * A.nby - 1);
-        const int bz = GEN ? min(max((int)(z * A.ibz), 0), A.nbz - 1) : 0;
-        const uint4 rec = T.bins[(bz * A.nby + (sby >> 2)) * A.nbx + (sbx >> 2)];
-        const int sbi = (sbx & 3) | ((sby & 3) << 2);
-        const uint32_t nib = ((sbi < 8 ? rec.y : rec.z) >> ((sbi & 7) << 2)) & 0xFu;
-        const bool many = (nib & kNibMany) != 0u;
-        const int nrel = many ? (int)(rec.x & 0xFFu) : (nib == kNibNone ? 0 : 1);
-        const uint16_t* cl = T.cand + (rec.x >> 8) + (many ? 0u : nib);
 
         int lastpl = -1;     // image of the plane of the step's last reflection
         if (REJ) {
@@ -367,9 +383,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         dx = dx + (ex - x);
         dy = dy + (ey - y);
         dz = dz + (ez - z);
-        x = wrap1(ex, A.a);
-        y = wrap1(ey, A.b);
-        z = wrap1(ez, A.c);
+        x = wrap ? wrap1(ex, A.a) : ex;
+        y = wrap ? wrap1(ey, A.b) : ey;
+        z = wrap ? wrap1(ez, A.c) : ez;
         // phase moments (kernelLattice.cuh:81-85): wave-uniform profile samples
         const_fptr g = prof + (size_t)i * P;
 #pragma unroll
@@ -386,17 +402,18 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     int i = A.i0 < 1 ? 1 : A.i0;
     if (((i - 1) & 1) && i < A.i1) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        step(i, o.z, o.w);
+        step(i, o.z, o.w, lookup(), true);
         ++i;
     }
     for (; i + 1 < A.i1; i += 2) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        step(i, o.x, o.y);
-        step(i + 1, o.z, o.w);
+        const Cands c = lookup();
+        step(i, o.x, o.y, c, false);
+        step(i + 1, o.z, o.w, c, true);
     }
     if (i < A.i1) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        step(i, o.x, o.y);
+        step(i, o.x, o.y, lookup(), false);
     }
 
     // Re-derive the state addresses after the loop (the empty asm hides

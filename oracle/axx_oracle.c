@@ -308,8 +308,10 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
     P->Lmax = (float)Lmax;
     /* periodic images: every shift in {-1,0,1}^2 (x{-1,0,1} for spheres)
      * whose grown shape touches the cell, in (original, shift) order */
-    /* a plane counts as hit within plane_eps of it, so its reach is longer */
-    double reach = Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) + (any_plane ? 2.0 * P->plane_eps : 0.0);
+    /* positions wrap after every second step, so a step may start up to one
+     * step length outside the cell: images within two step lengths; a plane
+     * counts as hit within plane_eps of it, so its reach is longer */
+    double reach = 2.0 * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) + (any_plane ? 4.0 * P->plane_eps : 0.0);
     double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
     int cap = n * 27 + 1;
     P->img = malloc(sizeof(oprim) * cap);
@@ -781,9 +783,16 @@ static void* job_run(void* arg) {
             dx = dx + (ex - x);
             dy = dy + (ey - y);
             dz = dz + (ez - z);
-            x = wrap1(ex, P->a);
-            y = wrap1(ey, P->b);
-            z = wrap1(ez, P->c);
+            /* wrap after the second step of each pair (i even) */
+            if ((i & 1) == 0) {
+                x = wrap1(ex, P->a);
+                y = wrap1(ey, P->b);
+                z = wrap1(ez, P->c);
+            } else {
+                x = ex;
+                y = ey;
+                z = ez;
+            }
             /* phase moments (kernelLattice.cuh:81-85, factored by profile) */
             const float* g = q->profiles + (size_t)i * Pn;
             for (int p = 0; p < Pn; ++p) {
