@@ -79,6 +79,7 @@ struct HostPrep {
     std::vector<int> orig;
     std::vector<float> L, rate, thr_out, thr_in;
     std::vector<float4> cyl;
+    std::vector<float> inv;      // 1 / R per image (float division), 0 for planes
     std::vector<uint4> bins;
     std::vector<uint16_t> cand;
     int nb[3] = {1, 1, 1};
@@ -211,7 +212,7 @@ static bool surface_near_box(const float4& g, int meta, const double lo[3], cons
 }
 
 static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
-    return n_bins * 16 + n_img * (gen ? 20 : 16) + n_cand * 2;
+    return n_bins * 16 + n_img * (gen ? 24 : 20) + n_cand * 2;
 }
 
 // Images, compartments and bins.  The float32 image centres are
@@ -295,6 +296,8 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         float mf;
         std::memcpy(&mf, &H.meta[i], sizeof(mf));
         H.cyl.push_back(make_float4(H.geom[i].x, H.geom[i].y, H.geom[i].w, mf));
+        const float Rf = H.geom[i].w;
+        H.inv.push_back((H.meta[i] & kPlaneBit) ? 0.0f : 1.0f / Rf);
     }
     // bin width: at least 4 step lengths; widen until the tables fit in LDS.
     // Each candidate carries a 4 x 4 mask of the sub-bins (xy) whose
@@ -421,7 +424,7 @@ static int dupload(axx_ctx* ctx, T** dst, const T* src, size_t n) {
 
 void plan_free(axx_plan* p) {
     if (!p) return;
-    void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_bins, p->d_cand, p->d_L, p->d_rate,
+    void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_bins, p->d_cand, p->d_L, p->d_rate,
                     p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
                     p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
     for (void* b : bufs)
@@ -518,6 +521,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if ((rc = dupload(ctx, &p->d_geom, H.geom.data(), H.geom.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_cyl, H.cyl.data(), H.cyl.size()))) return rc;
+    if ((rc = dupload(ctx, &p->d_inv, H.inv.data(), H.inv.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
     if ((rc = dupload(ctx, &p->d_L, H.L.data(), H.L.size()))) return rc;
@@ -580,6 +584,7 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     A.ibx4 = p->ib4[0]; A.iby4 = p->ib4[1];
     A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
     A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_cyl = p->d_cyl; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
+    A.g_inv = p->d_inv;
     A.compL = p->d_L; A.rate = p->d_rate; A.thr_out = p->d_thr_out; A.thr_in = p->d_thr_in;
     A.prof = p->d_prof;
     A.state = p->d_state;

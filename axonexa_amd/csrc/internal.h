@@ -50,6 +50,7 @@ struct axx_plan {
     float4* d_geom = nullptr;
     int* d_meta = nullptr;
     float4* d_cyl = nullptr;
+    float* d_inv = nullptr;
     uint4* d_bins = nullptr;
     uint16_t* d_cand = nullptr;
     float* d_L = nullptr;

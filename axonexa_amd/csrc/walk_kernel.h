@@ -30,6 +30,7 @@ struct WalkArgs {
     const float4* __restrict__ g_geom;      // [n_img] cx, cy, cz, R (planes: nx, ny, nz, d)
     const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit | kPlaneBit
     const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
+    const float* __restrict__ g_inv;        // [n_img] 1 / R (float division; 0 for planes)
     const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
     const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
     const float* __restrict__ prof;         // [n_steps][Pt] unit profile samples
@@ -110,11 +111,12 @@ __device__ __forceinline__ bool plane_hit(float4 g, float x0, float y0, float z0
 
 // Substrate tables in LDS: bins (16 B each), then one float4 per image
 // (cylinder-only: cx, cy, R, meta; otherwise cx, cy, cz, R plus a meta
-// array), then the u16 candidate lists.
+// array), the inverse radii, then the u16 candidate lists.
 struct LdsTables {
     uint4* bins;
     float4* geom;
     int* meta;
+    float* inv;
     uint16_t* cand;
 };
 
@@ -124,7 +126,8 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
     T.bins = reinterpret_cast<uint4*>(lds);
     T.geom = reinterpret_cast<float4*>(T.bins + A.n_bins);
     T.meta = reinterpret_cast<int*>(T.geom + A.n_img);
-    T.cand = GEN ? reinterpret_cast<uint16_t*>(T.meta + A.n_img) : reinterpret_cast<uint16_t*>(T.meta);
+    T.inv = GEN ? reinterpret_cast<float*>(T.meta + A.n_img) : reinterpret_cast<float*>(T.meta);
+    T.cand = reinterpret_cast<uint16_t*>(T.inv + A.n_img);
     for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
         if (GEN) {
@@ -133,6 +136,7 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
         } else {
             T.geom[i] = A.g_cyl[i];
         }
+        T.inv[i] = A.g_inv[i];
     }
     for (int i = threadIdx.x; i < A.n_cand; i += blockDim.x) T.cand[i] = A.g_cand[i];
     __syncthreads();
@@ -263,14 +267,16 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
             float Axy = fmaf(uy, uy, ux * ux);
             float A3 = fmaf(uz, uz, Axy);
             int nb = 0, kc = 0;
+            int excl = -1;   // image just left outward: a convex surface cannot be hit again
             bool live = nrel > 0;
+            // no lane of the wave has two or more candidates: no list loop
+            const bool single = __ballot(nrel > 1) == 0ull;
             while (live) {
                 // nearest crossing over the candidates (lattice.cuh:205-247)
                 float best = __builtin_inff();
                 int bk = -1;
                 bool bins_ = false;
-                for (int r = 0; r < nrel; ++r) {
-                    const int k = cl[r];
+                auto test = [&](const int k) {
                     int mt;
                     const float4 g = image_at<GEN>(T, k, mt);
                     if (GEN && (mt & kPlaneBit)) {
@@ -280,7 +286,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                             bk = k;
                             bins_ = false;
                         }
-                        continue;
+                        return;
                     }
                     const bool sk = GEN && (mt & kSphereBit);
                     const bool same = comp == (mt & kCompMask);
@@ -295,6 +301,14 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                             bk = k;
                             bins_ = Q.inside;
                         }
+                    }
+                };
+                if (single) {
+                    if (nrel > 0 && (int)cl[0] != excl) test(cl[0]);
+                } else {
+                    for (int r = 0; r < nrel; ++r) {
+                        const int k = cl[r];
+                        if (k != excl) test(k);
                     }
                 }
                 live = bk >= 0;
@@ -320,6 +334,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     kc++;
                 }
                 lastpl = pl ? bk : -1;
+                // left outward (outside reflection, transmission from inside,
+                // plane reflection): the next scan skips this surface
+                excl = (pl || ins == transmit) ? bk : -1;
                 if (transmit) {
                     // continue on the far side with its speed (boundaryCheck.cuh:211-215)
                     const int nc = ins ? 0 : pcomp;
@@ -341,7 +358,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                         nx = g.x; ny = g.y; nz = g.z;
                         kd = fmaf(nz, uz, fmaf(ny, uy, nx * ux));
                     } else {
-                        const float inv = 1.0f / g.w;
+                        const float inv = T.inv[bk];   // 1.0f / R
                         nx = (px - g.x) * inv;
                         ny = (py - g.y) * inv;
                         nz = sk ? (pz - g.z) * inv : 0.0f;
@@ -360,6 +377,8 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     ncap++;
                     ell = 0.0f;
                     live = false;
+                } else if (nrel == 1 && excl >= 0) {
+                    live = false;   // the only surface within reach was just left
                 }
             }
         }

@@ -690,12 +690,15 @@ static void* job_run(void* arg) {
                 float Axy = fmaf(uy, uy, ux * ux);
                 float A3 = fmaf(uz, uz, Axy);
                 int nb = 0, kc = 0;
+                int excl = -1;   /* image just left outward: a convex surface
+                                  * (or a plane) cannot be hit again */
                 lastplane = -1;
                 for (;;) {
                     float best = INFINITY;
                     int bk = -1, bins_ = 0;
                     for (int j = 0; j < cnt; ++j) {
                         int k = dense ? j : list[j];
+                        if (k == excl) continue;
                         const oprim* pk = &P->img[k];
                         float t;
                         int ins = 0, hit;
@@ -726,6 +729,9 @@ static void* job_run(void* arg) {
                         kc++;
                     }
                     lastplane = pk->type == AXX_PRIM_PLANE ? bk : -1;
+                    /* left outward: outside reflection, transmission from
+                     * inside, or a plane reflection */
+                    excl = (pk->type == AXX_PRIM_PLANE || inside == transmit) ? bk : -1;
                     if (transmit) {
                         int nc = inside ? 0 : pk->comp;
                         float rem = ell - t;
