@@ -77,6 +77,69 @@ def parallel_planes(cell: float, lo: float, hi: float, D: float, T2: float = flo
                          centers=cs, radii=[0.0, 0.0], normals=n)
 
 
+_TYPE_NAMES = {PRIM_CYLINDER_XY: "cylinder_xy", PRIM_SPHERE: "sphere", PRIM_PLANE: "plane"}
+
+
+def _num(v: float):
+    v = float(v)
+    return v if math.isfinite(v) else None
+
+
+def save_json(sub: SubstrateSpec, path: str, provenance: Optional[dict] = None) -> None:
+    """Write a substrate file (the format of include/axonexa_compat.hpp's
+    write_substrate_json: cell, lattice D / T2, one record per primitive;
+    an infinite T2 is null; floats round-trip exactly)."""
+    import json
+
+    prims = []
+    for k in range(sub.n_prims):
+        prims.append({
+            "type": _TYPE_NAMES[int(sub.types[k])],
+            "center": [float(x) for x in sub.centers[k]],
+            "radius": float(sub.radii[k]),
+            "D": float(sub.D[k]),
+            "T2": _num(sub.T2[k]),
+            "T1": 0.0,
+            "kappa": float(sub.kappa[k]),
+            "region": k,
+            "normal": [float(x) for x in sub.normals[k]],
+        })
+    doc = {"format": "axonexa-substrate", "version": 1,
+           "cell": [float(sub.a), float(sub.b), float(sub.c)],
+           "lattice": {"D": float(sub.D_lat), "T2": _num(sub.T2_lat), "T1": 0.0},
+           "primitives": prims, "provenance": provenance or {}}
+    with open(path, "w") as f:
+        json.dump(doc, f, indent=1, allow_nan=False)
+
+
+def load_json(path: str) -> SubstrateSpec:
+    """Read a substrate file written by save_json or by the C++ header."""
+    import json
+
+    with open(path) as f:
+        doc = json.load(f)
+    if doc.get("format") != "axonexa-substrate":
+        raise ValueError(f"{path}: not an axonexa substrate file")
+    names = {v: k for k, v in _TYPE_NAMES.items()}
+    prims = doc.get("primitives", [])
+    inf = float("inf")
+
+    def real(v, dflt):
+        return dflt if v is None else float(v)
+
+    lat = doc.get("lattice", {})
+    return SubstrateSpec(
+        a=float(doc["cell"][0]), b=float(doc["cell"][1]), c=float(doc["cell"][2]),
+        D_lat=real(lat.get("D"), 0.0), T2_lat=real(lat.get("T2"), inf),
+        types=[names[p.get("type", "cylinder_xy")] for p in prims],
+        centers=np.array([p["center"] for p in prims], np.float64).reshape(-1, 3),
+        radii=[real(p.get("radius"), 0.0) for p in prims],
+        D=np.array([real(p.get("D"), 0.0) for p in prims]),
+        T2=np.array([real(p.get("T2"), inf) for p in prims]),
+        kappa=np.array([real(p.get("kappa"), 0.0) for p in prims]),
+        normals=np.array([p.get("normal", [0.0, 0.0, 0.0]) for p in prims], np.float64).reshape(-1, 3))
+
+
 def hex_intra_fraction(R: float, spacing: float) -> float:
     return 2.0 * math.pi * R * R / (math.sqrt(3.0) * spacing * spacing)
 

@@ -24,7 +24,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
+#include <limits>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -162,6 +164,200 @@ class Lattice {
         return !(r.x > a || r.x < 0 || r.y > b || r.y < 0 || r.z > c || r.z < 0);
     }
 };
+
+// ------------------------------------------------ substrate file (JSON)
+// {"format": "axonexa-substrate", "version": 1, "cell": [a, b, c],
+//  "lattice": {"D": D, "T2": T2, "T1": T1},
+//  "primitives": [{"type": "cylinder_xy" | "sphere" | "plane",
+//                  "center": [x, y, z], "radius": R, "D": D, "T2": T2, "T1": T1,
+//                  "kappa": k, "region": r, "normal": [nx, ny, nz]}, ...],
+//  "provenance": {...}}
+// An infinite T2 is written as null.  Numbers carry 17 significant digits,
+// so a file round-trips doubles exactly.  The same format is read and
+// written by axonexa_amd/substrates.py (save_json / load_json).
+namespace axx_compat {
+struct Json {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    double num = 0;
+    bool b = false;
+    std::string str;
+    std::vector<Json> arr;
+    std::vector<std::pair<std::string, Json>> obj;
+    const Json* get(const std::string& k) const {
+        for (auto& kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+class JsonParser {
+  public:
+    explicit JsonParser(const std::string& t) : s(t) {}
+    Json parse() {
+        Json v = value();
+        ws();
+        if (i != s.size()) fail("trailing characters");
+        return v;
+    }
+
+  private:
+    const std::string& s;
+    size_t i = 0;
+    [[noreturn]] void fail(const char* what) const {
+        throw std::runtime_error(std::string("substrate JSON: ") + what + " at offset " + std::to_string(i));
+    }
+    void ws() {
+        while (i < s.size() && (s[i] == ' ' || s[i] == '\n' || s[i] == '\t' || s[i] == '\r')) ++i;
+    }
+    bool lit(const char* w) {
+        const size_t n = std::strlen(w);
+        if (s.compare(i, n, w) == 0) { i += n; return true; }
+        return false;
+    }
+    std::string string_() {
+        if (s[i] != '"') fail("expected a string");
+        ++i;
+        std::string out;
+        while (i < s.size() && s[i] != '"') {
+            if (s[i] == '\\') {
+                ++i;
+                if (i >= s.size()) fail("bad escape");
+                const char c = s[i];
+                out += c == 'n' ? '\n' : c == 't' ? '\t' : c;
+            } else {
+                out += s[i];
+            }
+            ++i;
+        }
+        if (i >= s.size()) fail("unterminated string");
+        ++i;
+        return out;
+    }
+    Json value() {
+        ws();
+        if (i >= s.size()) fail("unexpected end");
+        Json v;
+        const char c = s[i];
+        if (c == '{') {
+            v.kind = Json::OBJ;
+            ++i;
+            ws();
+            if (s[i] == '}') { ++i; return v; }
+            for (;;) {
+                ws();
+                std::string k = string_();
+                ws();
+                if (s[i] != ':') fail("expected ':'");
+                ++i;
+                v.obj.emplace_back(k, value());
+                ws();
+                if (s[i] == ',') { ++i; continue; }
+                if (s[i] == '}') { ++i; return v; }
+                fail("expected ',' or '}'");
+            }
+        }
+        if (c == '[') {
+            v.kind = Json::ARR;
+            ++i;
+            ws();
+            if (s[i] == ']') { ++i; return v; }
+            for (;;) {
+                v.arr.push_back(value());
+                ws();
+                if (s[i] == ',') { ++i; continue; }
+                if (s[i] == ']') { ++i; return v; }
+                fail("expected ',' or ']'");
+            }
+        }
+        if (c == '"') { v.kind = Json::STR; v.str = string_(); return v; }
+        if (lit("null")) return v;
+        if (lit("true")) { v.kind = Json::BOOL; v.b = true; return v; }
+        if (lit("false")) { v.kind = Json::BOOL; return v; }
+        char* end = nullptr;
+        v.num = std::strtod(s.c_str() + i, &end);
+        if (end == s.c_str() + i) fail("bad value");
+        i = (size_t)(end - s.c_str());
+        v.kind = Json::NUM;
+        return v;
+    }
+};
+
+inline std::string json_num(double v) {
+    if (!std::isfinite(v)) return "null";
+    char buf[40];
+    std::snprintf(buf, sizeof(buf), "%.17g", v);
+    return buf;
+}
+
+inline double json_real(const Json* v, double dflt) {
+    if (!v || v->kind == Json::NUL) return dflt;
+    if (v->kind != Json::NUM) throw std::runtime_error("substrate JSON: expected a number");
+    return v->num;
+}
+
+inline void write_substrate_json(const std::string& path, const Lattice& lat, const std::vector<Prim>& prims,
+                                 const std::string& provenance_json = "{}") {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    std::fprintf(f, "{\"format\": \"axonexa-substrate\", \"version\": 1,\n \"cell\": [%s, %s, %s],\n",
+                 json_num(lat.a).c_str(), json_num(lat.b).c_str(), json_num(lat.c).c_str());
+    std::fprintf(f, " \"lattice\": {\"D\": %s, \"T2\": %s, \"T1\": %s},\n \"primitives\": [",
+                 json_num(lat.D).c_str(), json_num(lat.T2).c_str(), json_num(lat.T1).c_str());
+    for (size_t k = 0; k < prims.size(); ++k) {
+        const Prim& p = prims[k];
+        const char* type = p.type == AXX_PRIM_SPHERE ? "sphere" : p.type == AXX_PRIM_PLANE ? "plane" : "cylinder_xy";
+        std::fprintf(f, "%s\n  {\"type\": \"%s\", \"center\": [%s, %s, %s], \"radius\": %s, \"D\": %s, "
+                        "\"T2\": %s, \"T1\": %s, \"kappa\": %s, \"region\": %d, \"normal\": [%s, %s, %s]}",
+                     k ? "," : "", type, json_num(p.cx).c_str(), json_num(p.cy).c_str(), json_num(p.cz).c_str(),
+                     json_num(p.R).c_str(), json_num(p.D).c_str(), json_num(p.T2).c_str(), json_num(p.T1).c_str(),
+                     json_num(p.kappa).c_str(), p.region, json_num(p.nx).c_str(), json_num(p.ny).c_str(),
+                     json_num(p.nz).c_str());
+    }
+    std::fprintf(f, "\n ],\n \"provenance\": %s}\n", provenance_json.c_str());
+    std::fclose(f);
+}
+
+inline void read_substrate_json(const std::string& path, Lattice& lat, std::vector<Prim>& prims) {
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) throw std::runtime_error("cannot read " + path);
+    std::string text;
+    char buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    const Json root = JsonParser(text).parse();
+    const Json* fmt = root.get("format");
+    if (!fmt || fmt->str != "axonexa-substrate") throw std::runtime_error(path + ": not an axonexa substrate file");
+    const Json* cell = root.get("cell");
+    if (!cell || cell->arr.size() != 3) throw std::runtime_error(path + ": cell must have 3 lengths");
+    const Json* L = root.get("lattice");
+    const double inf = std::numeric_limits<double>::infinity();
+    lat = Lattice(cell->arr[0].num, cell->arr[1].num, cell->arr[2].num, L ? json_real(L->get("T2"), inf) : inf,
+                  L ? json_real(L->get("T1"), 0.0) : 0.0, L ? json_real(L->get("D"), 0.0) : 0.0, 0);
+    prims.clear();
+    const Json* P = root.get("primitives");
+    if (P)
+        for (const Json& e : P->arr) {
+            Prim p;
+            const Json* t = e.get("type");
+            const std::string ty = t ? t->str : "cylinder_xy";
+            p.type = ty == "sphere" ? AXX_PRIM_SPHERE : ty == "plane" ? AXX_PRIM_PLANE : AXX_PRIM_CYLINDER_XY;
+            const Json* c = e.get("center");
+            if (!c || c->arr.size() != 3) throw std::runtime_error(path + ": primitive without a 3-d center");
+            p.cx = c->arr[0].num; p.cy = c->arr[1].num; p.cz = c->arr[2].num;
+            p.R = json_real(e.get("radius"), 0.0);
+            p.D = json_real(e.get("D"), 0.0);
+            p.T2 = json_real(e.get("T2"), inf);
+            p.T1 = json_real(e.get("T1"), 0.0);
+            p.kappa = json_real(e.get("kappa"), 0.0);
+            p.region = (int)json_real(e.get("region"), 0.0);
+            const Json* nrm = e.get("normal");
+            if (nrm && nrm->arr.size() == 3) { p.nx = nrm->arr[0].num; p.ny = nrm->arr[1].num; p.nz = nrm->arr[2].num; }
+            prims.push_back(p);
+        }
+    lat.setBasisSize((int)prims.size());
+}
+}  // namespace axx_compat
 
 // --------------------------------------------------------------- functors
 // Two-lobe waveform G(t) = G m(t) ghat on [0, T), -G m(t - gap - T) ghat on
@@ -507,6 +703,20 @@ class RPSLatticeInitializer {
         return true;
     }
     void setRegions() { for (size_t i = 0; i < basis.size(); ++i) basis[i].setRegion((int)i); }
+    // The packing as a substrate file (format above) and back; reading
+    // replaces the basis (its primitives must be of this class's type).
+    void writeJSON(const std::string& path, const std::string& provenance_json = "{}") const {
+        std::vector<axx_compat::Prim> prims;
+        for (auto& b : basis) prims.push_back(b.p);
+        axx_compat::write_substrate_json(path, lat, prims, provenance_json);
+    }
+    void readJSON(const std::string& path) {
+        std::vector<axx_compat::Prim> prims;
+        axx_compat::read_substrate_json(path, lat, prims);
+        basis.assign(prims.size(), B());
+        for (size_t i = 0; i < prims.size(); ++i) basis[i].p = prims[i];
+    }
+    Lattice& getLattice() { return lat; }
     void correctEdges() { lat.setBasisSize((int)basis.size()); }
     real getVI() {
         real v = 0;

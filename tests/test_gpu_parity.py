@@ -186,3 +186,63 @@ def test_planes_rejection_mode():
     sig, _ = _run_both("planes_rejection", sub, acq,
                        abi.RunSpec(n_walkers=2048, seed=41, boundary_mode=abi.BOUNDARY_REJECTION))
     assert sig.collisions > 100
+
+
+def test_wide_profile_set_gamma_packing():
+    # 64 profiles x 3 axes: the 512-thread, 256-VGPR instantiation
+    from axonexa_amd import workloads as W
+    sub = sb.gamma_packing(30, 5.0, 0.4, 0.6, D_in=1.7, D_ex=2.0, seed=4, T2_in=70.0, T2_ex=50.0,
+                           rmin=0.25)
+    fs = W.c3_functors(n_dirs=2, amps=(0.3,))
+    acq = wf.build_acq(fs, 600, 5e-3, GAMMA)
+    assert acq.n_profiles == 64
+    run = abi.RunSpec(n_walkers=1500, seed=77, relaxation=1)
+    sig, _ = _run_both("wide_profiles", sub, acq, run)
+    assert sig.collisions > 1000
+
+
+def test_single_walker_large_ids():
+    sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(900, 1e-2)
+    _run_both("one_walker", sub, acq,
+              abi.RunSpec(n_walkers=1, seed=2 ** 63 + 12345, walker_offset=2 ** 32 - 1, instance=2 ** 32 - 2))
+
+
+def test_spheres_rejection_and_planes_cap():
+    cs = [[2.5, 2.5, 2.5], [7.5, 7.5, 7.5]]
+    sub = abi.SubstrateSpec(a=10.0, b=10.0, c=10.0, D_lat=2.0,
+                            types=[abi.PRIM_SPHERE, abi.PRIM_SPHERE, abi.PRIM_PLANE],
+                            centers=cs + [[0.0, 0.0, 5.0]], radii=[2.0, 2.0, 0.0], D=2.0,
+                            normals=[[0, 0, 0], [0, 0, 0], [0.0, 0.3, 1.0]])
+    acq = _pgse(1000, 1e-2, axis=(0, 1, 1))
+    sig, _ = _run_both("spheres_rejection", sub, acq,
+                       abi.RunSpec(n_walkers=2048, seed=13, boundary_mode=abi.BOUNDARY_REJECTION))
+    assert sig.collisions > 100
+    sig, _ = _run_both("spheres_planes_cap", sub, acq, abi.RunSpec(n_walkers=2048, seed=14, bounce_cap=2))
+    assert sig.collisions > 100
+
+
+def test_invalid_descriptors_are_errors():
+    eng = _engine()
+    sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(100, 1e-2)
+    for run in (abi.RunSpec(n_walkers=0), abi.RunSpec(n_walkers=10, boundary_mode=7),
+                abi.RunSpec(n_walkers=10, init_mode=abi.INIT_INSIDE_PRIM, init_arg=3)):
+        with pytest.raises(eng.AxxError) as e:
+            eng.Plan(sub, acq, run)
+        assert e.value.code == -1
+    big = sb.single_cylinder(12.0, 6.5, D_in=2.0, D_ex=2.0)           # wider than half the cell
+    with pytest.raises(eng.AxxError) as e:
+        eng.Plan(big, acq, abi.RunSpec(n_walkers=10))
+    assert e.value.code == -5
+    overlap = abi.SubstrateSpec(a=12.0, b=12.0, c=12.0, D_lat=2.0, types=[0, 0],
+                                centers=[[3, 3, 0], [5, 3, 0]], radii=[1.5, 1.5], D=2.0)
+    with pytest.raises(eng.AxxError) as e:
+        eng.Plan(overlap, acq, abi.RunSpec(n_walkers=10))
+    assert e.value.code == -5
+    with eng.Plan(sub, acq, abi.RunSpec(n_walkers=10)) as p:
+        with pytest.raises(eng.AxxError) as e:
+            p.reduce()                       # before the walk finished
+        assert e.value.code == -8
+        with pytest.raises(eng.AxxError):
+            p.walk(5, 10)                    # must restart at 0 or continue
