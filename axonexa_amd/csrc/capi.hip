@@ -109,7 +109,8 @@ static void box_proj(const double n[3], const double lo[3], const double hi[3], 
 }
 
 static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
-    if (!(s->a > 0 && s->b > 0 && s->c > 0)) return fail(ctx, AXX_ERR_INVALID, "cell lengths must be > 0");
+    if (!(s->a > 0 && s->b > 0 && s->c > 0) || !std::isfinite(s->a + s->b + s->c))
+        return fail(ctx, AXX_ERR_INVALID, "cell lengths must be finite and > 0");
     if (s->n_prims < 0) return fail(ctx, AXX_ERR_INVALID, "n_prims < 0");
     if (s->n_prims > AXX_MAX_PRIMS) return fail(ctx, AXX_ERR_LIMIT, "n_prims > %d", AXX_MAX_PRIMS);
     if (s->n_prims > 0 && (!s->type || !s->center || !s->radius || !s->D || !s->T2))
@@ -120,14 +121,23 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     if (!q->profiles) return fail(ctx, AXX_ERR_INVALID, "profiles missing");
     if (q->n_waveforms < 1 || !q->wf_profile || !q->wf_amplitude || !q->wf_dir)
         return fail(ctx, AXX_ERR_INVALID, "waveforms missing");
-    for (int j = 0; j < q->n_waveforms; ++j)
+    for (int j = 0; j < q->n_waveforms; ++j) {
         if (q->wf_profile[j] < 0 || q->wf_profile[j] >= q->n_profiles)
             return fail(ctx, AXX_ERR_INVALID, "waveform %d: profile index out of range", j);
+        if (!std::isfinite(q->wf_amplitude[j] + q->wf_dir[3 * j] + q->wf_dir[3 * j + 1] + q->wf_dir[3 * j + 2]))
+            return fail(ctx, AXX_ERR_INVALID, "waveform %d: non-finite amplitude or direction", j);
+    }
+    if (!std::isfinite(q->dt) || !std::isfinite(q->gamma)) return fail(ctx, AXX_ERR_INVALID, "dt / gamma");
     for (long long i = 0; i < (long long)q->n_steps * q->n_profiles; ++i)
         if (!std::isfinite(q->profiles[i])) return fail(ctx, AXX_ERR_INVALID, "non-finite profile sample");
-    if (!(s->D_lat >= 0)) return fail(ctx, AXX_ERR_INVALID, "D_lat < 0");
+    if (!(s->D_lat >= 0) || !std::isfinite(s->D_lat) || std::isnan(s->T2_lat))
+        return fail(ctx, AXX_ERR_INVALID, "D_lat / T2_lat");
     double Lmax = std::sqrt(6.0 * s->D_lat * q->dt);
     for (int k = 0; k < s->n_prims; ++k) {
+        if (!std::isfinite(s->center[3 * k] + s->center[3 * k + 1] + s->center[3 * k + 2]))
+            return fail(ctx, AXX_ERR_INVALID, "primitive %d: non-finite centre", k);
+        if (std::isnan(s->T2[k]) || (s->kappa && !(s->kappa[k] >= 0 && std::isfinite(s->kappa[k]))))
+            return fail(ctx, AXX_ERR_INVALID, "primitive %d: T2 / permeability", k);
         if (s->type[k] == AXX_PRIM_PLANE) {
             if (!s->normal) return fail(ctx, AXX_ERR_INVALID, "plane normals missing");
             const double* v = s->normal + 3 * k;
@@ -474,6 +484,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     p->ctx = ctx;
     p->run = *r;
     p->N = r->n_walkers;
+    p->n_prims = s->n_prims;
     p->n_steps = q->n_steps;
     p->dt = q->dt;
     p->P = q->n_profiles;
@@ -749,6 +760,28 @@ extern "C" int axx_plan_get_state(axx_plan* p, uint32_t* host) {
 extern "C" int axx_plan_set_state(axx_plan* p, const uint32_t* host) {
     if (!p || !host) return AXX_ERR_INVALID;
     axx_ctx* ctx = p->ctx;
+    // the kernels index per-compartment tables with the compartment and
+    // assume positions within a cell length of the cell and one common step
+    {
+        const size_t N = (size_t)p->N;
+        const int* is = reinterpret_cast<const int*>(host);
+        const float* fs = reinterpret_cast<const float*>(host);
+        const int step = is[(size_t)F_STEP * N];
+        if (step < 1 || step > p->n_steps) return fail(ctx, AXX_ERR_INVALID, "state: step %d", step);
+        const float lo[3] = {-p->cell[0], -p->cell[1], -p->cell[2]};
+        const float hi[3] = {2 * p->cell[0], 2 * p->cell[1], 2 * p->cell[2]};
+        for (size_t w = 0; w < N; ++w) {
+            const int c = is[(size_t)F_COMP * N + w], c0 = is[(size_t)F_COMP0 * N + w];
+            if (c < 0 || c > p->n_prims || c0 < 0 || c0 > p->n_prims)
+                return fail(ctx, AXX_ERR_INVALID, "state: walker %zu compartment out of range", w);
+            if (is[(size_t)F_STEP * N + w] != step)
+                return fail(ctx, AXX_ERR_INVALID, "state: walkers at different steps");
+            for (int a = 0; a < 3; ++a) {
+                const float v = fs[(size_t)(F_X + a) * N + w];
+                if (!(v >= lo[a] && v <= hi[a])) return fail(ctx, AXX_ERR_INVALID, "state: walker %zu position", w);
+            }
+        }
+    }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const size_t nf = AXX_STATE_FIELDS(p->P);

@@ -29,7 +29,7 @@ struct axx_ctx {
 struct axx_plan {
     axx_ctx* ctx = nullptr;
     // host copies
-    int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7;
+    int n_steps = 0, P = 0, Pt = 0, M = 0, ax = 7, n_prims = 0;
     bool gen = false;       // spheres or planes present: general kernel path
     long long N = 0;
     axx_run run{};

@@ -246,3 +246,30 @@ def test_invalid_descriptors_are_errors():
         assert e.value.code == -8
         with pytest.raises(eng.AxxError):
             p.walk(5, 10)                    # must restart at 0 or continue
+
+
+def test_restored_state_is_checked():
+    eng = _engine()
+    sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(300, 1e-2)
+    run = abi.RunSpec(n_walkers=256, seed=3)
+    with eng.Plan(sub, acq, run) as p:
+        p.walk(0, 100)
+        st = p.get_state()
+        v = abi.state_view(st, 256, acq.n_profiles)
+        for field, value in (("comp", 7), ("step", 0)):
+            bad = st.copy()
+            abi.state_view(bad, 256, acq.n_profiles)[field][5] = value
+            with pytest.raises(eng.AxxError):
+                p.set_state(bad)
+        bad = st.copy()
+        abi.state_view(bad, 256, acq.n_profiles)["pos"][3, 0] = np.nan
+        with pytest.raises(eng.AxxError):
+            p.set_state(bad)
+        p.set_state(st)                     # a good state resumes exactly
+        p.walk(100, 300).reduce()
+        a = p.results()
+    with eng.Plan(sub, acq, run) as q:
+        q.walk().reduce()
+        b = q.results()
+    assert np.array_equal(a.sum_cos, b.sum_cos) and v["step"][0] == 100
