@@ -138,7 +138,7 @@ __device__ __forceinline__ bool hit_test(float ox, float oy, float oz, float ux,
 
 
 // Correctly rounded sqrt for arguments known to be normal and finite
-// (here (1-c)(1+c) in [2^-23, 1]): hardware estimate plus the one-ulp
+// (here 1 - c^2 in [2^-22, 1]): hardware estimate plus the one-ulp
 // residual correction; equal to IEEE sqrtf on that domain.
 __device__ __forceinline__ float sqrt_unit(float x) {
     float y = __builtin_amdgcn_sqrtf(x);

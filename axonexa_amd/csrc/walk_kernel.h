@@ -238,8 +238,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         const int nrel = C.nrel;
         const uint16_t* cl = C.cl;
         // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
-        const float ct = unif(wb) * 2.0f - 1.0f;
-        const float sth = sqrt_unit((1.0f - ct) * (1.0f + ct));
+        // cos(theta) = 2u - 1 and sin^2 = 1 - c^2, each in one rounding
+        const float ct = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
+        const float sth = sqrt_unit(fmaf(-ct, ct, 1.0f));
         float sphi, cphi;
         sincos_turn(wa, sphi, cphi);
         float ux = sth * cphi, uy = sth * sphi, uz = ct;
@@ -404,7 +405,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         dz = dz + (ez - z);
         x = wrap ? wrap1(ex, A.a) : ex;
         y = wrap ? wrap1(ey, A.b) : ey;
-        z = wrap ? wrap1(ez, A.c) : ez;
+        z = (wrap && GEN) ? wrap1(ez, A.c) : ez;   // cylinders only: z enters no geometry
         // phase moments (kernelLattice.cuh:81-85): wave-uniform profile samples
         const_fptr g = prof + (size_t)i * P;
 #pragma unroll

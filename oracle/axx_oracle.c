@@ -143,6 +143,7 @@ typedef struct {
     float Lmax;
     float H2c, H2s; /* squared half of the smallest cell side (xy; xyz) */
     float plane_eps;/* minimum distance from a plane after reflecting from it */
+    int any_3d;     /* a sphere or plane is present (z matters) */
     /* optional cell bins (binned variant of the scan) */
     int nb[3];
     float binw_inv[3];
@@ -306,6 +307,7 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
         P->thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
     }
     P->Lmax = (float)Lmax;
+    for (int k = 0; k < n; ++k) P->any_3d |= s->type[k] != AXX_PRIM_CYLINDER_XY;
     /* periodic images: every shift in {-1,0,1}^2 (x{-1,0,1} for spheres)
      * whose grown shape touches the cell, in (original, shift) order */
     /* positions wrap after every second step, so a step may start up to one
@@ -659,8 +661,9 @@ static void* job_run(void* arg) {
             uint32_t h = (uint32_t)(i - 1) & 1u;
             if (h == 0 || i == ibeg) draw(wid, (uint32_t)(i - 1) >> 1, STREAM_STEP << 24, inst, key, o);
             uint32_t wa = o[2 * h], wb = o[2 * h + 1];
-            float c = unif(wb) * 2.0f - 1.0f;
-            float sth = sqrtf((1.0f - c) * (1.0f + c));
+            /* cos(theta) = 2u - 1 in one rounding; sin^2 = 1 - c^2 in one rounding */
+            float c = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
+            float sth = sqrtf(fmaf(-c, c, 1.0f));
             float sphi, cphi;
             orc_sincos_turn(wa, &sphi, &cphi);
             float ux = sth * cphi, uy = sth * sphi, uz = c;
@@ -793,7 +796,8 @@ static void* job_run(void* arg) {
             if ((i & 1) == 0) {
                 x = wrap1(ex, P->a);
                 y = wrap1(ey, P->b);
-                z = wrap1(ez, P->c);
+                /* with cylinders only, z enters no geometry: it is not wrapped */
+                z = P->any_3d ? wrap1(ez, P->c) : ez;
             } else {
                 x = ex;
                 y = ey;

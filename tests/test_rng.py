@@ -75,7 +75,7 @@ def test_direction_distribution_is_isotropic():
     for wid in range(n // 2):
         o = po.philox([wid, 0, 0, 0], [1234, 0])
         for wa, wb in ((o[0], o[1]), (o[2], o[3])):
-            cts.append(po.unif(int(wb)) * 2.0 - 1.0)
+            cts.append(2.0 * po.unif(int(wb)) - 1.0)
             s, c = po.sincos_turn(int(wa))
             phis.append(math.atan2(s, c) % (2 * math.pi))
     cts, phis = np.array(cts), np.array(phis)
