@@ -36,6 +36,7 @@ PRIM_PLANE = 2
 BOUNDARY_SPECULAR = 0
 BOUNDARY_PERMEABLE = 1
 BOUNDARY_REJECTION = 2
+STEP_ISOTROPIC, STEP_AXES, STEP_GAUSSIAN = 0, 1, 2
 
 INIT_UNIFORM = 0
 INIT_COMPARTMENT = 1
@@ -81,6 +82,7 @@ class Run(C.Structure):
         ("n_walkers", C.c_int64), ("seed", C.c_uint64), ("instance", C.c_uint32),
         ("walker_offset", C.c_uint32), ("init_mode", C.c_int), ("init_arg", C.c_int),
         ("boundary_mode", C.c_int), ("relaxation", C.c_int), ("bounce_cap", C.c_int),
+        ("step_mode", C.c_int),
     ]
 
 
@@ -208,6 +210,7 @@ class RunSpec:
     boundary_mode: int = BOUNDARY_SPECULAR
     relaxation: int = 0
     bounce_cap: int = 0
+    step_mode: int = 0           # STEP_ISOTROPIC / STEP_AXES / STEP_GAUSSIAN
 
     def struct(self) -> Run:
         r = Run()
@@ -218,6 +221,7 @@ class RunSpec:
         r.init_mode, r.init_arg = int(self.init_mode), int(self.init_arg)
         r.boundary_mode, r.relaxation = int(self.boundary_mode), int(self.relaxation)
         r.bounce_cap = int(self.bounce_cap)
+        r.step_mode = int(self.step_mode)
         return r
 
 

@@ -64,7 +64,7 @@ def build_lib(verbose: bool = False) -> str:
     return lib
 
 
-DRIVERS = ["cos_onecyl", "gammadist"]
+DRIVERS = ["cos_onecyl", "gammadist", "wccompare"]
 
 
 def build_drivers() -> list:

@@ -26,8 +26,9 @@
 namespace axx {
 
 // Instance kernel variants: padded profile width 1, 4 or 8, every gradient
-// axis, cylinders-only or general geometry.  class = 2 * width_index + gen.
-constexpr int kBatchMaxProfiles = 8;
+// axis and step generator, reflecting/permeable or rejection boundaries,
+// cylinders-only or general geometry.  class = 4 * width_index + 2 * rej + gen.
+constexpr int kBatchMaxProfiles = AXX_BATCH_MAX_PROFILES;
 constexpr int kBatchBlockMax = 512;
 
 struct BatchWalkArgs {
@@ -38,10 +39,10 @@ struct BatchWalkArgs {
     unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
 };
 
-template <int PT, bool GEN>
+template <int PT, bool GEN, bool REJ>
 __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
     for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x)
-        walk_lane<PT, 7, GEN, false>(A, T, A.prof, (uint32_t)w);
+        walk_lane<PT, 7, GEN, REJ, true>(A, T, A.prof, (uint32_t)w);
 }
 
 __global__ __launch_bounds__(kBatchBlockMax) void batch_walk_kernel(BatchWalkArgs B) {
@@ -67,12 +68,18 @@ __global__ __launch_bounds__(kBatchBlockMax) void batch_walk_kernel(BatchWalkArg
             staged = i;
         }
         switch (c) {
-        case 0: batch_tile<1, false>(A, T, w0, w1); break;
-        case 1: batch_tile<1, true>(A, T, w0, w1); break;
-        case 2: batch_tile<4, false>(A, T, w0, w1); break;
-        case 3: batch_tile<4, true>(A, T, w0, w1); break;
-        case 4: batch_tile<8, false>(A, T, w0, w1); break;
-        default: batch_tile<8, true>(A, T, w0, w1); break;
+        case 0: batch_tile<1, false, false>(A, T, w0, w1); break;
+        case 1: batch_tile<1, true, false>(A, T, w0, w1); break;
+        case 2: batch_tile<1, false, true>(A, T, w0, w1); break;
+        case 3: batch_tile<1, true, true>(A, T, w0, w1); break;
+        case 4: batch_tile<4, false, false>(A, T, w0, w1); break;
+        case 5: batch_tile<4, true, false>(A, T, w0, w1); break;
+        case 6: batch_tile<4, false, true>(A, T, w0, w1); break;
+        case 7: batch_tile<4, true, true>(A, T, w0, w1); break;
+        case 8: batch_tile<8, false, false>(A, T, w0, w1); break;
+        case 9: batch_tile<8, true, false>(A, T, w0, w1); break;
+        case 10: batch_tile<8, false, true>(A, T, w0, w1); break;
+        default: batch_tile<8, true, true>(A, T, w0, w1); break;
         }
     }
 }
@@ -155,7 +162,7 @@ std::map<std::string, std::pair<int, int>>& tune_cache() {
 
 int class_of(const axx_plan* p) {
     const int wi = p->Pt <= 1 ? 0 : (p->Pt <= 4 ? 1 : 2);
-    return 2 * wi + (p->gen ? 1 : 0);
+    return 4 * wi + (p->run.boundary_mode == AXX_BOUNDARY_REJECTION ? 2 : 0) + (p->gen ? 1 : 0);
 }
 
 int min_pt_for(int P) { return P <= 1 ? 1 : (P <= 4 ? 4 : 8); }
@@ -331,10 +338,6 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
         if (!I.substrate || !I.acq || !I.run) {
             batch_free(b);
             return fail(ctx, AXX_ERR_INVALID, "instance %d: null descriptor", i);
-        }
-        if (I.run->boundary_mode == AXX_BOUNDARY_REJECTION) {
-            batch_free(b);
-            return fail(ctx, AXX_ERR_INVALID, "instance %d: rejection runs are not batched", i);
         }
         if (I.acq->n_profiles > kBatchMaxProfiles) {
             batch_free(b);

@@ -229,8 +229,13 @@ static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
 // (float)(centre + shift) computed in double; compartments use
 // L = (float)sqrt(6 D dt), rate = (float)(dt / T2), thresholds
 // (float)(4 kappa / v).
+// Longest step of a step generator in units of sqrt(6 D dt): Gaussian steps
+// have |n| <= sqrt(2 * 33.27) = 8.157 standard normals (24-bit uniforms),
+// i.e. at most 4.71 such units.
+static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
+
 static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_t lds_budget,
-                   double bin_width_req, HostPrep& H) {
+                   double bin_width_req, int step_mode, HostPrep& H) {
     const int n = s->n_prims;
     const double dt = q->dt;
     H.L.resize(n + 1);
@@ -266,7 +271,7 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
     // reach of a pair of steps (one lookup serves both; the first may end up
     // to one step outside the cell); a plane counts as hit within plane_eps
     // of it, so its reach is longer
-    const double reach = 2.0 * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
+    const double reach = 2.0 * step_reach_factor(step_mode) * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
                          (any_plane ? 4.0 * H.plane_eps : 0.0);
     for (int k = 0; k < n; ++k) {
         const int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
@@ -403,15 +408,15 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
 // ---------------------------------------------------------------------------
 // Kernel dispatch over (profile width, axes, general geometry, rejection) instantiations
 
-static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej) {
+static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej, bool alt) {
     switch (Pt) {
-    case 1: return pick_walk_p1(ax, gen, rej);
-    case 2: return pick_walk_p2(ax, gen, rej);
-    case 4: return pick_walk_p4(ax, gen, rej);
-    case 8: return pick_walk_p8(ax, gen, rej);
-    case 16: return pick_walk_p16(ax, gen, rej);
-    case 32: return pick_walk_p32(ax, gen, rej);
-    default: return pick_walk_p64(ax, gen, rej);
+    case 1: return pick_walk_p1(ax, gen, rej, alt);
+    case 2: return pick_walk_p2(ax, gen, rej, alt);
+    case 4: return pick_walk_p4(ax, gen, rej, alt);
+    case 8: return pick_walk_p8(ax, gen, rej, alt);
+    case 16: return pick_walk_p16(ax, gen, rej, alt);
+    case 32: return pick_walk_p32(ax, gen, rej, alt);
+    default: return pick_walk_p64(ax, gen, rej, alt);
     }
 }
 
@@ -479,7 +484,16 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if (tune && tune->block && (tune->block < 64 || tune->block > 1024 || tune->block % 64))
         return fail(ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
     if (tune && tune->bin_width < 0) return fail(ctx, AXX_ERR_INVALID, "bin_width < 0");
-    rc = prepare(ctx, s, q, budget, tune ? tune->bin_width : 0.0, H);
+    if (r->step_mode < 0 || r->step_mode > 2) return fail(ctx, AXX_ERR_INVALID, "step_mode");
+    if (r->step_mode == AXX_STEP_GAUSSIAN) {
+        double Lmax = std::sqrt(6.0 * s->D_lat * q->dt);
+        for (int k = 0; k < s->n_prims; ++k)
+            if (s->type[k] != AXX_PRIM_PLANE) Lmax = std::max(Lmax, std::sqrt(6.0 * s->D[k] * q->dt));
+        const double need = 2.0 * step_reach_factor(AXX_STEP_GAUSSIAN) * Lmax;
+        if (need >= s->a || need >= s->b || need >= s->c)
+            return fail(ctx, AXX_ERR_GEOMETRY, "Gaussian steps need a cell wider than %.3g", need);
+    }
+    rc = prepare(ctx, s, q, budget, tune ? tune->bin_width : 0.0, r->step_mode, H);
     if (rc) return rc;
     p->ctx = ctx;
     p->run = *r;
@@ -603,6 +617,7 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     A.k0 = (uint32_t)p->run.seed; A.k1 = (uint32_t)(p->run.seed >> 32);
     A.inst = p->run.instance; A.woff = p->run.walker_offset;
     A.mode = p->run.boundary_mode; A.init_mode = p->run.init_mode; A.init_arg = p->run.init_arg;
+    A.step_mode = p->run.step_mode;
     A.init_img = p->init_img;
     A.i0 = i0; A.i1 = i1;
     A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
@@ -613,7 +628,8 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
 static int launch_walk(axx_plan* p, int i0, int i1) {
     axx_ctx* ctx = p->ctx;
     WalkArgs A = plan_walk_args(p, i0, i1);
-    walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
+    walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION,
+                           p->run.step_mode != AXX_STEP_ISOTROPIC);
     const int block = std::min(p->block, walk_max_block(p->Pt));
     const long long grid = (p->N + block - 1) / block;
     if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");

@@ -43,6 +43,7 @@ __device__ __forceinline__ u32x4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
 constexpr uint32_t kStreamStep = 0u;
 constexpr uint32_t kStreamColl = 1u << 24;
 constexpr uint32_t kStreamInit = 2u << 24;
+constexpr uint32_t kStreamGauss = 3u << 24;   // extra words of Gaussian steps
 
 // Uniform in (0,1): odd multiples of 2^-24 (exactly representable).
 __device__ __forceinline__ float unif(uint32_t w) {
@@ -72,6 +73,31 @@ __device__ __forceinline__ void sincos_turn(uint32_t w, float& s, float& c) {
     float b0 = (qq & 1u) ? sp : cp;   // cos magnitude source
     s = (qq >= 2u) ? -a0 : a0;
     c = (qq == 1u || qq == 2u) ? -b0 : b0;
+}
+
+// -2 ln u(w) for the Box-Muller transform of Gaussian steps: u = k 2^-24
+// with k odd, ln u = (e - 24) ln 2 + ln m for k = 2^e m, m in [sqrt(1/2),
+// sqrt(2)), ln m = 2 atanh(s) with s = (m - 1) / (m + 1), |s| < 0.172, by
+// its odd series to s^9 (truncation < 4e-10).  The result lies in
+// [1.19e-7, 33.3].
+__device__ __forceinline__ float neg2log_unif(uint32_t w) {
+    const float x = (float)((w >> 8) | 1u);
+    const uint32_t bits = __float_as_uint(x);
+    int e = (int)(bits >> 23) - 127;
+    float m = __uint_as_float((bits & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = fmaf(s2, 0.11111111f, 0.14285714f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 0.33333334f);
+    p = fmaf(s2, p, 1.0f);
+    const float lnm = (2.0f * s) * p;
+    const float lnu = fmaf((float)(e - 24), 0.69314718f, lnm);
+    return -2.0f * lnu;
 }
 
 // Periodic wrap of one coordinate (Lattice::lmod, strict comparisons).

@@ -42,6 +42,7 @@ struct WalkArgs {
     long long N;
     uint32_t k0, k1, inst, woff;
     int mode, init_mode, init_arg, init_img;
+    int step_mode;                          // 0 isotropic, 1 axis signs, 2 Gaussian
     int i0, i1, cap;
     int* __restrict__ err;
 };
@@ -160,13 +161,13 @@ __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
 typedef const __attribute__((address_space(4))) float* const_fptr;
 
 typedef void (*walk_fn)(WalkArgs, const float*);
-walk_fn pick_walk_p1(int ax, bool gen, bool rej);
-walk_fn pick_walk_p2(int ax, bool gen, bool rej);
-walk_fn pick_walk_p4(int ax, bool gen, bool rej);
-walk_fn pick_walk_p8(int ax, bool gen, bool rej);
-walk_fn pick_walk_p16(int ax, bool gen, bool rej);
-walk_fn pick_walk_p32(int ax, bool gen, bool rej);
-walk_fn pick_walk_p64(int ax, bool gen, bool rej);
+walk_fn pick_walk_p1(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p2(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p4(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p8(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p16(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p32(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p64(int ax, bool gen, bool rej, bool alt);
 
 // Wide profile sets keep up to 192 moments per lane: those instantiations are
 // limited to 512 threads per block so that each lane may use 256 VGPRs.
@@ -174,8 +175,10 @@ constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 
 // One lane walks walker wo of A over steps [max(A.i0, 1), A.i1) with the
 // substrate tables T in LDS.  GEN: substrates with spheres or planes (3D
-// bins, per-image meta words); otherwise cylinders only.
-template <int P, int AX, bool GEN, bool REJ>
+// bins, per-image meta words); otherwise cylinders only.  ALT: the axis-sign
+// and Gaussian step generators are compiled in (selected by A.step_mode);
+// otherwise isotropic steps only.
+template <int P, int AX, bool GEN, bool REJ, bool ALT>
 __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                                           const float* __restrict__ prof_g, const uint32_t wo) {
     const long long N = A.N;
@@ -234,17 +237,45 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     // One step i: direction from the two Philox words, boundary handling,
     // displacement, phase moments; the position is wrapped into the cell
     // after the second step of each pair (i even) (kernelLattice.cuh:46-99).
-    auto step = [&](const int i, const uint32_t wa, const uint32_t wb, const Cands C, const bool wrap) {
+    auto step = [&](const int i, const uint32_t wa, const uint32_t wb, const uint32_t wc, const uint32_t wd,
+                    const Cands C, const bool wrap) {
         const int nrel = C.nrel;
         const uint16_t* cl = C.cl;
-        // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
-        // cos(theta) = 2u - 1 and sin^2 = 1 - c^2, each in one rounding
-        const float ct = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
-        const float sth = sqrt_unit(fmaf(-ct, ct, 1.0f));
-        float sphi, cphi;
-        sincos_turn(wa, sphi, cphi);
-        float ux = sth * cphi, uy = sth * sphi, uz = ct;
-        float ell = Lc;
+        float ux, uy, uz, gs = 1.0f;   // direction; step length = gs * (compartment step)
+        if (!ALT || A.step_mode == 0) {
+            // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
+            // cos(theta) = 2u - 1 and sin^2 = 1 - c^2, each in one rounding
+            const float ct = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
+            const float sth = sqrt_unit(fmaf(-ct, ct, 1.0f));
+            float sphi, cphi;
+            sincos_turn(wa, sphi, cphi);
+            ux = sth * cphi;
+            uy = sth * sphi;
+            uz = ct;
+        } else if (A.step_mode == 1) {
+            // a random sign per axis, +-sqrt(2 D dt) each (kernelPhase.cuh:31-35
+            // with the per-axis length that keeps D)
+            const float k3 = 0.57735026f;
+            ux = (wa & 1u) ? -k3 : k3;
+            uy = (wa & 2u) ? -k3 : k3;
+            uz = (wa & 4u) ? -k3 : k3;
+        } else {
+            // Gaussian step of variance 2 D dt per axis (kernelWC.cuh:56-67):
+            // Box-Muller on (wa, wb) and (wc, wd); length |n| sqrt(2 D dt)
+            const float r1 = sqrt_unit(neg2log_unif(wa));
+            const float r2 = sqrt_unit(neg2log_unif(wc));
+            float s1, c1, s2, c2;
+            sincos_turn(wb, s1, c1);
+            sincos_turn(wd, s2, c2);
+            const float n1 = r1 * c1, n2 = r1 * s1, n3 = r2 * c2;
+            const float nn = sqrtf(fmaf(n3, n3, fmaf(n2, n2, n1 * n1)));
+            const float inv = 1.0f / nn;
+            ux = n1 * inv;
+            uy = n2 * inv;
+            uz = n3 * inv;
+            gs = nn * 0.57735026f;
+        }
+        float ell = gs * Lc;
         float sx = x, sy = y, sz = z;
 
         int lastpl = -1;     // image of the plane of the step's last reflection
@@ -341,14 +372,14 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 if (transmit) {
                     // continue on the far side with its speed (boundaryCheck.cuh:211-215)
                     const int nc = ins ? 0 : pcomp;
-                    const float f = (ell - t) / Lc;
+                    const float f = (ell - t) / (gs * Lc);
                     const float ph = 1.0f - f;
                     E = fmaf((float)(i - li) + (ph - lf), A.rate[comp], E);
                     li = i;
                     lf = ph;
                     comp = nc;
                     Lc = A.compL[nc];
-                    ell = f * Lc;
+                    ell = f * (gs * Lc);
                     ntran++;
                 } else {
                     // specular reflection about the surface normal (p - c) / R, or
@@ -419,21 +450,32 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     // Steps are paired: one Philox call, counter (walker, (i-1)/2, 0, instance),
     // gives words (x, y) to step i with i-1 even and (z, w) to the next one.
+    // Gaussian steps take two more words each from a second call, counter
+    // (walker, (i-1)/2, 3 << 24, instance).
+    auto gauss = [&](const int ii) {
+        u32x4 g{0u, 0u, 0u, 0u};
+        if (ALT && A.step_mode == 2)
+            g = philox10(wid, (uint32_t)(ii - 1) >> 1, kStreamGauss, A.inst, A.k0, A.k1);
+        return g;
+    };
     int i = A.i0 < 1 ? 1 : A.i0;
     if (((i - 1) & 1) && i < A.i1) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        step(i, o.z, o.w, lookup(), true);
+        const u32x4 g = gauss(i);
+        step(i, o.z, o.w, g.z, g.w, lookup(), true);
         ++i;
     }
     for (; i + 1 < A.i1; i += 2) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+        const u32x4 g = gauss(i);
         const Cands c = lookup();
-        step(i, o.x, o.y, c, false);
-        step(i + 1, o.z, o.w, c, true);
+        step(i, o.x, o.y, g.x, g.y, c, false);
+        step(i + 1, o.z, o.w, g.z, g.w, c, true);
     }
     if (i < A.i1) {
         const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        step(i, o.x, o.y, lookup(), false);
+        const u32x4 g = gauss(i);
+        step(i, o.x, o.y, g.x, g.y, lookup(), false);
     }
 
     // Re-derive the state addresses after the loop (the empty asm hides
@@ -460,13 +502,13 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     }
 }
 
-template <int P, int AX, bool GEN, bool REJ>
+template <int P, int AX, bool GEN, bool REJ, bool ALT>
 __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsTables T = stage_tables<GEN>(A, lds);
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= A.N) return;
-    walk_lane<P, AX, GEN, REJ>(A, T, prof_g, (uint32_t)w);
+    walk_lane<P, AX, GEN, REJ, ALT>(A, T, prof_g, (uint32_t)w);
 }
 
 }  // namespace axx

@@ -116,7 +116,15 @@ typedef struct axx_run {
     int boundary_mode;       /* AXX_BOUNDARY_*                                   */
     int relaxation;          /* 1: weight by exp(-sum dt/T2) (USE_RELAXATION)    */
     int bounce_cap;          /* max boundary events per step (0 -> 256)          */
+    int step_mode;           /* AXX_STEP_*                                       */
 } axx_run;
+
+/* ---- step generators (kernelLattice.cuh:52-57; kernelPhase.cuh:31-35;
+ *      kernelWC.cuh:56-67) ---- */
+#define AXX_STEP_ISOTROPIC 0  /* length sqrt(6 D dt), direction uniform on the sphere   */
+#define AXX_STEP_AXES      1  /* +-sqrt(2 D dt) on each axis, random signs              */
+#define AXX_STEP_GAUSSIAN  2  /* N(0, 2 D dt) on each axis (Box-Muller); the cell must
+                                 exceed 9.5 sqrt(6 D dt), the longest pair of steps     */
 
 /*
  * Results: the reference returns raw Mx_j = sum_w cos(phi_j,w)*e^{-T2f_w} and
@@ -235,10 +243,12 @@ void axx_plan_destroy(axx_plan* plan);
  * one hipGraph.  No reference counterpart beyond the loop over acquisitions
  * of phaseAcquisitionStream<F>::runAcquisitionStreamLattice
  * (phaseAcquisitionStream.cuh:425-613); every instance gives bitwise the
- * walkers and sums of its own axx_run_phase.  Limits: at most 8 profiles per
- * instance, no rejection policy, n_walkers < 2^31.  Instances keep their own
- * seed / instance id / walker offset, so replicas stay independent.
+ * walkers and sums of its own axx_run_phase.  Limits: at most
+ * AXX_BATCH_MAX_PROFILES profiles per instance, n_walkers < 2^31; every
+ * boundary policy and step generator.  Instances keep their own seed / instance id / walker
+ * offset, so replicas stay independent.
  */
+#define AXX_BATCH_MAX_PROFILES 8
 typedef struct axx_instance {
     const axx_substrate* substrate;
     const axx_acq* acq;

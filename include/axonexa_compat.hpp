@@ -20,6 +20,7 @@
 #pragma once
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -27,6 +28,7 @@
 #include <cstring>
 #include <iostream>
 #include <limits>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -570,18 +572,46 @@ class phaseAcquisitionStream {
     template <class B>
     void runAcquisition(int num, B& basis, real dt, int blocks, int threads, int numOfSM) {
         (void)blocks; (void)threads; (void)numOfSM;
-        axx_compat::Prim p = basis.p;
-        const real L = 6.0 * p.R;
-        p.cx = L / 2; p.cy = L / 2; p.cz = L / 2;
-        Lattice lat(L, L, L, p.T2, p.T1, p.D, 1);
+        Lattice lat;
+        const axx_compat::Prim p = pore(basis, lat);
         run(num, std::vector<axx_compat::Prim>{p}, lat, dt, AXX_INIT_INSIDE_PRIM, 0);
     }
-    // All acquisitions, one device (phaseAcquisitionStream.cuh:425-613)
+    // The reference's serial CPU path for a single pore
+    // (phaseAcquisitionStream.cuh:90-124, CPUkernels.cuh:80-172): here the
+    // same walk runs on the engine (its results equal the CPU oracle's bit
+    // for bit).
+    template <class B>
+    void runAcquisitionCPU(int num, B& basis, real dt) { runAcquisition(num, basis, dt, 0, 0, 0); }
+    // Gaussian steps with rejection at the pore wall (kernelWC.cuh:25-89,
+    // phaseAcquisitionStream.cuh:192-252).
+    template <class B>
+    void runAcquisitionWC(int num, B& basis, real dt, int blocks, int threads) {
+        (void)blocks; (void)threads;
+        Lattice lat;
+        const axx_compat::Prim p = pore(basis, lat);
+        run(num, std::vector<axx_compat::Prim>{p}, lat, dt, AXX_INIT_INSIDE_PRIM, 0, AXX_STEP_GAUSSIAN,
+            AXX_BOUNDARY_REJECTION);
+    }
+    // Every acquisition, single pore (phaseAcquisitionStream.cuh:254-423):
+    // one batched persistent launch for all of them (each acquisition gets
+    // exactly what runAcquisition(i, ...) gives it).
+    template <class B>
+    void runAcquisitionStream(B& basis, real dt, int blocks, int threads, int numOfDevices, std::vector<int>& plan,
+                              std::vector<int>& numOfSMPerDevice) {
+        (void)blocks; (void)threads; (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
+        Lattice lat;
+        const std::vector<axx_compat::Prim> prims{pore(basis, lat)};
+        run_all(prims, lat, dt, AXX_INIT_INSIDE_PRIM, 0);
+    }
+    // All acquisitions, one device (phaseAcquisitionStream.cuh:425-613): one
+    // batched persistent launch, results equal to runAcquisitionLattice(i, ...)
     template <class B>
     void runAcquisitionStreamLattice(B* basis, Lattice& lat, real dt, int blocks, int threads, int numOfDevices,
                                      std::vector<int>& plan, std::vector<int>& numOfSMPerDevice) {
-        (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
-        for (int i = 0; i < (int)acqSet.size(); ++i) runAcquisitionLattice(i, basis, lat, dt, blocks, threads, 0);
+        (void)blocks; (void)threads; (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
+        std::vector<axx_compat::Prim> prims;
+        for (int i = 0; i < lat.getBasisSize(); ++i) prims.push_back(basis[i].p);
+        run_all(prims, lat, dt, init_mode(), init_arg());
     }
     void changeSeeds(long s) {
         for (auto& a : acqSet) a.getSeed() = (int)s;
@@ -608,50 +638,137 @@ class phaseAcquisitionStream {
         return 0;
 #endif
     }
-    void run(int num, const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode, int iarg) {
+    static int step_mode() {
+#if defined LATTICE_PICKING
+        return AXX_STEP_AXES;    // +-sqrt(2 D dt) per axis (the reference's +-L per axis is defect D4)
+#else
+        return AXX_STEP_ISOTROPIC;
+#endif
+    }
+    // A single pore centred in a periodic cell six radii wide (so the pore's
+    // images never matter); the cell takes the pore's D and T2.
+    template <class B>
+    static axx_compat::Prim pore(B& basis, Lattice& lat) {
+        axx_compat::Prim p = basis.p;
+        const real L = 6.0 * p.R;
+        p.cx = L / 2; p.cy = L / 2; p.cz = L / 2;
+        lat = Lattice(L, L, L, p.T2, p.T1, p.D, 1);
+        return p;
+    }
+    static int boundary_mode() {
+#if defined USE_PERMEABLE
+        return AXX_BOUNDARY_PERMEABLE;
+#elif defined SPECULAR_REFLECTION
+        return AXX_BOUNDARY_SPECULAR;
+#else
+        return AXX_BOUNDARY_REJECTION;
+#endif
+    }
+    // Acquisition num described for the C ABI; the descriptors point into
+    // the vectors, so a Desc is neither copied nor moved once filled.
+    struct Desc {
+        axx_compat::Sampled<F> S;
+        std::vector<int> type;
+        std::vector<double> center, radius, D, T2, kappa, normal, mx, my;
+        axx_substrate s{};
+        axx_acq q{};
+        axx_run r{};
+        axx_result out{};
+        Desc() = default;
+        Desc(const Desc&) = delete;
+        Desc& operator=(const Desc&) = delete;
+    };
+    void describe(Desc& d, int num, const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode,
+                  int iarg, int smode, int boundary) {
         phaseAcquisition<F>& acq = acqSet[num];
         const int n = acq.getNumOfSteps();
-        auto S = axx_compat::sample(acq.getGradientFunctors(), n, (double)dt);
-        std::vector<int> type;
-        std::vector<double> center, radius, D, T2, kappa, normal;
+        d.S = axx_compat::sample(acq.getGradientFunctors(), n, (double)dt);
         for (auto& p : prims) {
-            type.push_back(p.type);
-            center.push_back(p.cx); center.push_back(p.cy); center.push_back(p.cz);
-            radius.push_back(p.R); D.push_back(p.D); T2.push_back(p.T2); kappa.push_back(p.kappa);
-            normal.push_back(p.nx); normal.push_back(p.ny); normal.push_back(p.nz);
+            d.type.push_back(p.type);
+            d.center.push_back(p.cx); d.center.push_back(p.cy); d.center.push_back(p.cz);
+            d.radius.push_back(p.R); d.D.push_back(p.D); d.T2.push_back(p.T2); d.kappa.push_back(p.kappa);
+            d.normal.push_back(p.nx); d.normal.push_back(p.ny); d.normal.push_back(p.nz);
         }
-        axx_substrate s{};
+        axx_substrate& s = d.s;
         s.a = lat.a; s.b = lat.b; s.c = lat.c; s.D_lat = lat.D; s.T2_lat = lat.T2;
         s.n_prims = (int)prims.size();
-        s.type = type.data(); s.center = center.data(); s.radius = radius.data();
-        s.D = D.data(); s.T2 = T2.data(); s.kappa = kappa.data(); s.normal = normal.data();
-        axx_acq q{};
+        s.type = d.type.data(); s.center = d.center.data(); s.radius = d.radius.data();
+        s.D = d.D.data(); s.T2 = d.T2.data(); s.kappa = d.kappa.data(); s.normal = d.normal.data();
+        axx_acq& q = d.q;
         q.n_steps = n; q.dt = dt; q.gamma = GAMMA;
-        q.n_profiles = S.P; q.profiles = S.prof.data();
-        q.n_waveforms = (int)S.wf_p.size(); q.wf_profile = S.wf_p.data();
-        q.wf_amplitude = S.amp.data(); q.wf_dir = S.dir.data();
-        axx_run r{};
+        q.n_profiles = d.S.P; q.profiles = d.S.prof.data();
+        q.n_waveforms = (int)d.S.wf_p.size(); q.wf_profile = d.S.wf_p.data();
+        q.wf_amplitude = d.S.amp.data(); q.wf_dir = d.S.dir.data();
+        axx_run& r = d.r;
         r.n_walkers = number_of_particles;
         r.seed = (uint64_t)(uint32_t)acq.getSeed();
         r.instance = (uint32_t)num;
         r.init_mode = imode; r.init_arg = iarg;
-#if defined USE_PERMEABLE
-        r.boundary_mode = AXX_BOUNDARY_PERMEABLE;
-#elif defined SPECULAR_REFLECTION
-        r.boundary_mode = AXX_BOUNDARY_SPECULAR;
-#else
-        r.boundary_mode = AXX_BOUNDARY_REJECTION;
-#endif
+        r.boundary_mode = boundary >= 0 ? boundary : boundary_mode();
 #if defined USE_RELAXATION
         r.relaxation = 1;
 #endif
-        const int M = q.n_waveforms;
-        std::vector<double> mx(M), my(M);
-        axx_result out{};
-        out.sum_cos = mx.data(); out.sum_sin = my.data();
-        axx_compat::check(axx_run_phase(axx_compat::context(), &s, &q, &r, &out), "axx_run_phase");
-        for (int j = 0; j < M; ++j) { acq.getMx()[j] = (real)mx[j]; acq.getMy()[j] = (real)my[j]; }
+        r.step_mode = smode;
+        d.mx.assign(q.n_waveforms, 0.0);
+        d.my.assign(q.n_waveforms, 0.0);
+        d.out.sum_cos = d.mx.data(); d.out.sum_sin = d.my.data();
     }
+    void store(const Desc& d, int num) {
+        phaseAcquisition<F>& acq = acqSet[num];
+        for (int j = 0; j < d.q.n_waveforms; ++j) { acq.getMx()[j] = (real)d.mx[j]; acq.getMy()[j] = (real)d.my[j]; }
+    }
+    void run(int num, const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode, int iarg,
+             int smode = step_mode(), int boundary = -1) {
+        Desc d;
+        describe(d, num, prims, lat, dt, imode, iarg, smode, boundary);
+        axx_compat::check(axx_run_phase(axx_compat::context(), &d.s, &d.q, &d.r, &d.out), "axx_run_phase");
+        store(d, num);
+    }
+    // Every acquisition in one batched launch (axx_run_batch) when the batch
+    // takes them (no rejection policy, at most AXX_BATCH_MAX_PROFILES
+    // distinct waveform shapes each); otherwise one run per acquisition.
+    void run_all(const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode, int iarg) {
+        const int n = (int)acqSet.size();
+        if (n == 0) return;
+        std::vector<std::unique_ptr<Desc>> ds;
+        bool batched = true;
+        for (int i = 0; i < n; ++i) {
+            ds.emplace_back(new Desc);
+            describe(*ds.back(), i, prims, lat, dt, imode, iarg, step_mode(), -1);
+            batched = batched && ds.back()->r.boundary_mode != AXX_BOUNDARY_REJECTION &&
+                      ds.back()->q.n_profiles <= AXX_BATCH_MAX_PROFILES;
+        }
+        if (batched) {
+            std::vector<axx_instance> inst(n);
+            std::vector<axx_result> out(n);
+            for (int i = 0; i < n; ++i) {
+                inst[i].substrate = &ds[i]->s; inst[i].acq = &ds[i]->q; inst[i].run = &ds[i]->r;
+                out[i] = ds[i]->out;
+            }
+            axx_compat::check(axx_run_batch(axx_compat::context(), n, inst.data(), out.data()), "axx_run_batch");
+        } else {
+            for (int i = 0; i < n; ++i) {
+                Desc& d = *ds[i];
+                axx_compat::check(axx_run_phase(axx_compat::context(), &d.s, &d.q, &d.r, &d.out), "axx_run_phase");
+            }
+        }
+        for (int i = 0; i < n; ++i) store(*ds[i], i);
+    }
+};
+
+// ------------------------------------------------------------------ timer
+// CPUtimer (src/util/timer.cuh): start / stop / getTime in seconds.  Wall
+// clock here: the reference's clock() counts process CPU time, which is not
+// the duration of a run the host thread spends waiting on the GPU.
+class CPUtimer {
+  public:
+    void start() { t0 = std::chrono::steady_clock::now(); }
+    void stop() { t1 = std::chrono::steady_clock::now(); }
+    double getTime() const { return std::chrono::duration<double>(t1 - t0).count(); }
+    void display() const { std::cout << " Elapsed time = " << getTime() << " s " << std::endl; }
+
+  private:
+    std::chrono::steady_clock::time_point t0{}, t1{};
 };
 
 // ------------------------------------------------- random packing (host)

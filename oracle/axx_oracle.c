@@ -70,6 +70,7 @@ float orc_unif(uint32_t w) { return unif(w); }
 #define STREAM_STEP 0u
 #define STREAM_COLL 1u
 #define STREAM_INIT 2u
+#define STREAM_GAUSS 3u
 
 static inline void draw(uint32_t wid, uint32_t idx, uint32_t stream_sub, uint32_t inst,
                         const uint32_t key[2], uint32_t out[4]) {
@@ -115,6 +116,37 @@ void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
     case 2: *c_out = -cp; *s_out = -sp; break;
     default: *c_out = sp; *s_out = -cp; break;
     }
+}
+
+/* -2 ln u(w) for Box-Muller (Gaussian steps, kernelWC.cuh:56-61): u = k 2^-24
+ * with k odd, ln u = (e - 24) ln 2 + ln m for k = 2^e m, m in [sqrt(1/2),
+ * sqrt(2)), ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| < 0.172, odd series
+ * to s^9 (truncation < 4e-10).  Result in [1.19e-7, 33.3]. */
+static inline float neg2log_unif(uint32_t w) {
+    float x = (float)((w >> 8) | 1u);
+    uint32_t bits;
+    memcpy(&bits, &x, 4);
+    int e = (int)(bits >> 23) - 127;
+    uint32_t mb = (bits & 0x007FFFFFu) | 0x3F800000u;
+    float m;
+    memcpy(&m, &mb, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = fmaf(s2, 0.11111111f, 0.14285714f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 0.33333334f);
+    p = fmaf(s2, p, 1.0f);
+    float lnm = (2.0f * s) * p;
+    float lnu = fmaf((float)(e - 24), 0.69314718f, lnm);
+    return -2.0f * lnu;
+}
+
+void orc_neg2log_unif(const uint32_t* w, float* out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) out[k] = neg2log_unif(w[k]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -265,7 +297,13 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
     return AXX_OK;
 }
 
-static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width) {
+/* Longest step in units of sqrt(6 D dt): Gaussian steps are at most
+ * |n| = sqrt(2 * 33.27) = 8.157 standard normals long (24-bit uniforms),
+ * 4.71 such units. */
+static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
+
+static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width,
+                       double reach_factor) {
     memset(P, 0, sizeof(*P));
     int rc = orc_validate(s, q);
     if (rc) return rc;
@@ -313,7 +351,8 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
     /* positions wrap after every second step, so a step may start up to one
      * step length outside the cell: images within two step lengths; a plane
      * counts as hit within plane_eps of it, so its reach is longer */
-    double reach = 2.0 * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) + (any_plane ? 4.0 * P->plane_eps : 0.0);
+    double reach = 2.0 * reach_factor * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) +
+                   (any_plane ? 4.0 * P->plane_eps : 0.0);
     double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
     int cap = n * 27 + 1;
     P->img = malloc(sizeof(oprim) * cap);
@@ -655,19 +694,45 @@ static void* job_run(void* arg) {
                 for (int a = 0; a < 3; ++a) m[p][a] = FS(AXX_F_MOM + 3 * p + a)[w];
         }
         int ibeg = J->i0 < 1 ? 1 : J->i0;
-        uint32_t o[4] = {0, 0, 0, 0};
+        uint32_t o[4] = {0, 0, 0, 0}, og[4] = {0, 0, 0, 0};
         for (int i = ibeg; i < J->i1; ++i) {
-            /* draws: one Philox call per pair of steps (i-1)>>1 */
+            /* draws: one Philox call per pair of steps (i-1)>>1 (Gaussian
+             * steps: a second one on stream 3) */
             uint32_t h = (uint32_t)(i - 1) & 1u;
-            if (h == 0 || i == ibeg) draw(wid, (uint32_t)(i - 1) >> 1, STREAM_STEP << 24, inst, key, o);
+            if (h == 0 || i == ibeg) {
+                draw(wid, (uint32_t)(i - 1) >> 1, STREAM_STEP << 24, inst, key, o);
+                if (R->step_mode == AXX_STEP_GAUSSIAN)
+                    draw(wid, (uint32_t)(i - 1) >> 1, STREAM_GAUSS << 24, inst, key, og);
+            }
             uint32_t wa = o[2 * h], wb = o[2 * h + 1];
-            /* cos(theta) = 2u - 1 in one rounding; sin^2 = 1 - c^2 in one rounding */
-            float c = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
-            float sth = sqrtf(fmaf(-c, c, 1.0f));
-            float sphi, cphi;
-            orc_sincos_turn(wa, &sphi, &cphi);
-            float ux = sth * cphi, uy = sth * sphi, uz = c;
-            float ell = P->L[comp];
+            float ux, uy, uz, gs = 1.0f;
+            if (R->step_mode == AXX_STEP_ISOTROPIC) {
+                /* cos(theta) = 2u - 1 in one rounding; sin^2 = 1 - c^2 in one rounding */
+                float c = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
+                float sth = sqrtf(fmaf(-c, c, 1.0f));
+                float sphi, cphi;
+                orc_sincos_turn(wa, &sphi, &cphi);
+                ux = sth * cphi; uy = sth * sphi; uz = c;
+            } else if (R->step_mode == AXX_STEP_AXES) {
+                /* +-sqrt(2 D dt) per axis (kernelPhase.cuh:31-35, length kept) */
+                const float k3 = 0.57735026f;
+                ux = (wa & 1u) ? -k3 : k3;
+                uy = (wa & 2u) ? -k3 : k3;
+                uz = (wa & 4u) ? -k3 : k3;
+            } else {
+                /* N(0, 2 D dt) per axis by Box-Muller (kernelWC.cuh:56-67) */
+                uint32_t wc = og[2 * h], wd = og[2 * h + 1];
+                float r1 = sqrtf(neg2log_unif(wa)), r2 = sqrtf(neg2log_unif(wc));
+                float s1, c1, s2, c2;
+                orc_sincos_turn(wb, &s1, &c1);
+                orc_sincos_turn(wd, &s2, &c2);
+                float n1 = r1 * c1, n2 = r1 * s1, n3 = r2 * c2;
+                float nn = sqrtf(fmaf(n3, n3, fmaf(n2, n2, n1 * n1)));
+                float inv = 1.0f / nn;
+                ux = n1 * inv; uy = n2 * inv; uz = n3 * inv;
+                gs = nn * 0.57735026f;
+            }
+            float ell = gs * P->L[comp];
             int lastplane = -1;
             float sx = x, sy = y, sz = z;
             const int* list;
@@ -738,13 +803,13 @@ static void* job_run(void* arg) {
                     if (transmit) {
                         int nc = inside ? 0 : pk->comp;
                         float rem = ell - t;
-                        float f = rem / P->L[comp];
+                        float f = rem / (gs * P->L[comp]);
                         float ph = 1.0f - f;
                         E = fmaf((float)(i - li) + (ph - lf), P->rate[comp], E);
                         li = i;
                         lf = ph;
                         comp = nc;
-                        ell = f * P->L[nc];
+                        ell = f * (gs * P->L[nc]);
                         ntran++;
                     } else {
                         float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
@@ -842,8 +907,13 @@ int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int s
     if (run->n_walkers < 1) return AXX_ERR_INVALID;
     if (step_begin < 0 || step_end > q->n_steps || step_begin > step_end) return AXX_ERR_INVALID;
     if (run->boundary_mode < 0 || run->boundary_mode > 2) return AXX_ERR_INVALID;
+    if (run->step_mode < 0 || run->step_mode > 2) return AXX_ERR_INVALID;
     oprep P;
-    int rc = oprep_build(&P, s, q, bin_width);
+    int rc = oprep_build(&P, s, q, bin_width, step_reach_factor(run->step_mode));
+    if (!rc && run->step_mode == AXX_STEP_GAUSSIAN) {
+        double need = 2.0 * step_reach_factor(AXX_STEP_GAUSSIAN) * (double)P.Lmax;
+        if (need >= s->a || need >= s->b || need >= s->c) rc = AXX_ERR_GEOMETRY;
+    }
     if (rc) { oprep_free(&P); return rc; }
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
@@ -878,7 +948,7 @@ int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
                const uint32_t* state, axx_result* out) {
     if (!s || !q || !run || !state || !out) return AXX_ERR_INVALID;
     oprep P;
-    int rc = oprep_build(&P, s, q, 0.0);
+    int rc = oprep_build(&P, s, q, 0.0, 1.0);
     if (rc) { oprep_free(&P); return rc; }
     int64_t N = run->n_walkers;
     int M = q->n_waveforms, Pn = q->n_profiles, n = q->n_steps;
@@ -940,7 +1010,7 @@ int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
 /* Expanded primitive table (for tests of the image construction). */
 int orc_image_count(const axx_substrate* s, const axx_acq* q) {
     oprep P;
-    int rc = oprep_build(&P, s, q, 0.0);
+    int rc = oprep_build(&P, s, q, 0.0, 1.0);
     int n = rc ? rc : P.n_img;
     oprep_free(&P);
     return n;

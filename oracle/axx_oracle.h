@@ -12,6 +12,8 @@ extern "C" {
 #endif
 void  orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float orc_unif(uint32_t w);
+/* -2 ln u(w[k]) of the Gaussian step generator, k < n */
+void orc_neg2log_unif(const uint32_t* w, float* out, int64_t n);
 void  orc_sincos_turn(uint32_t w, float* s, float* c);
 int   orc_validate(const axx_substrate* s, const axx_acq* q);
 int   orc_axes_mask(const axx_acq* q);

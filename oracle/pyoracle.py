@@ -34,6 +34,7 @@ def lib():
         L.orc_unif.argtypes = [C.c_uint32]
         L.orc_unif.restype = C.c_float
         L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_neg2log_unif.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.orc_validate.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
         L.orc_walk.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                                C.c_int, C.c_int, u32p, C.c_int, C.c_double]
@@ -73,6 +74,14 @@ def sincos_turn(w: int):
     s, c = C.c_float(), C.c_float()
     lib().orc_sincos_turn(int(w) & 0xFFFFFFFF, C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def neg2log_unif(w) -> np.ndarray:
+    """-2 ln u(w) of the Gaussian step generator, elementwise."""
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    out = np.empty(w.shape, np.float32)
+    lib().orc_neg2log_unif(w.ctypes.data, out.ctypes.data, w.size)
+    return out
 
 
 def validate(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:

@@ -89,14 +89,47 @@ def test_batch_autotune_keeps_results():
                 assert np.array_equal(b.results(i).sum_cos, before[i]), (blk, wpt, i)
 
 
+def test_batch_mixes_policies_and_step_generators():
+    """Rejection, reflecting and permeable instances with isotropic, axis-sign
+    and Gaussian steps in one launch (the streaming acquisitions of a WC
+    comparison), each bitwise its oracle walk."""
+    from axonexa_amd import substrates as sb
+    dt = 2e-3
+    hexl = sb.hex_lattice(2.5, 6.0, 3, 2, D_in=1.7, D_ex=2.0, T2_in=70.0, T2_ex=50.0,
+                          kappa=sb.kappa_for_probability(0.05, 1.7, dt))
+    sph = sb.single_sphere(16.0, 5.0, D_in=2.0, D_ex=2.0)
+    acq = wf.build_acq([wf.CosGFunc(g * 1e-6, 0.4, 0.1, 2.0, ax)
+                        for g in (0.1, 0.4) for ax in ((1, 0, 0), (0, 0, 1))], 300, dt, W.GAMMA)
+
+    class Case:
+        pass
+
+    ws = []
+    for k, (sub, bmode, smode, imode) in enumerate([
+            (hexl, abi.BOUNDARY_PERMEABLE, abi.STEP_ISOTROPIC, abi.INIT_UNIFORM),
+            (sph, abi.BOUNDARY_REJECTION, abi.STEP_GAUSSIAN, abi.INIT_INSIDE_PRIM),
+            (hexl, abi.BOUNDARY_SPECULAR, abi.STEP_AXES, abi.INIT_UNIFORM),
+            (sph, abi.BOUNDARY_REJECTION, abi.STEP_ISOTROPIC, abi.INIT_INSIDE_PRIM),
+            (hexl, abi.BOUNDARY_REJECTION, abi.STEP_AXES, abi.INIT_UNIFORM),
+            (sph, abi.BOUNDARY_SPECULAR, abi.STEP_GAUSSIAN, abi.INIT_INSIDE_PRIM)]):
+        c = Case()
+        c.sub, c.acq = sub, acq
+        c.run = abi.RunSpec(n_walkers=700 + 211 * k, seed=40 + k, instance=k, boundary_mode=bmode,
+                            relaxation=1, step_mode=smode, init_mode=imode)
+        ws.append(c)
+    eng = _engine()
+    with eng.Batch([(w.sub, w.acq, w.run) for w in ws]) as b:
+        print(f"[batch] mixed policies, config {b.config()}", flush=True)
+        b.run()
+        for i, w in enumerate(ws):
+            g = _check_instance(b, i, w, f"mixed {i}")
+            assert g.collisions > 0, i
+
+
 def test_batch_rejects_what_it_cannot_run():
     ws = W.c4_batch(n_inst=2, scale=0.01, n_steps=100, seed=3)
     eng = _engine()
     w = ws[1]
-    rej = abi.RunSpec(n_walkers=100, seed=1, boundary_mode=abi.BOUNDARY_REJECTION)
-    with pytest.raises(eng.AxxError) as e:
-        eng.Batch([(w.sub, w.acq, rej)])
-    assert e.value.code == -1
     # nine distinct profiles: above the batch limit of eight
     fs = [wf.CosGFunc(1e-7, 0.08, 0.02, 1.0 + k, (1, 0, 0)) for k in range(9)]
     wide = wf.build_acq(fs, 100, 2e-3, W.GAMMA)

@@ -56,3 +56,15 @@ def test_gammadist_runs_replicas():
     assert rows.shape == (400, 4)
     assert np.all(np.isfinite(rows))
     assert np.all(np.abs(rows[:, 2]) <= 2048 + 1e-6)
+
+
+def test_wccompare_stream_equals_single_runs():
+    print("[gpu] driver wccompare 2048 0.05 4", flush=True)
+    out = subprocess.run([os.path.join(BIN, "wccompare"), "2048", "0.05", "4", "7"], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    rows = [l.split() for l in lines if l and l[0].isdigit()]
+    assert [int(r[0]) for r in rows] == [2, 4]
+    assert all(float(x) > 0 for r in rows for x in r[1:])
+    assert lines[-1] == "max |stream - single| = 0", lines[-1]

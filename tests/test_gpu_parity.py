@@ -273,3 +273,22 @@ def test_restored_state_is_checked():
         q.walk().reduce()
         b = q.results()
     assert np.array_equal(a.sum_cos, b.sum_cos) and v["step"][0] == 100
+
+
+@pytest.mark.parametrize("mode", [abi.STEP_AXES, abi.STEP_GAUSSIAN])
+def test_step_generators(mode):
+    dt = 2e-3
+    sub = sb.hex_lattice(2.5, 6.0, 3, 2, D_in=1.7, D_ex=2.0, T2_in=70.0, T2_ex=50.0,
+                         kappa=sb.kappa_for_probability(0.05, 1.7, dt))
+    acq = wf.build_acq([wf.CosGFunc(g * 1e-6, 3.0, 0.5, 2.5, ax)
+                        for g in (0.1, 0.4) for ax in ((1, 0, 0), (0, 0, 1))], 2500, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=2048, seed=21, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1,
+                      step_mode=mode)
+    sig, _ = _run_both(f"steps_{mode}_permeable", sub, acq, run, chunks=[1, 777])
+    assert sig.transmissions > 50
+    # the reference's WC setting: Gaussian (or axis) steps, rejection at a pore wall
+    sph = sb.single_sphere(16.0, 5.0, D_in=2.0, D_ex=2.0)
+    run = abi.RunSpec(n_walkers=2048, seed=22, step_mode=mode, boundary_mode=abi.BOUNDARY_REJECTION,
+                      init_mode=abi.INIT_INSIDE_PRIM)
+    sig, _ = _run_both(f"steps_{mode}_rejection", sph, _pgse(1500, 1e-2), run)
+    assert sig.n_intra == run.n_walkers and sig.collisions > 100

@@ -246,3 +246,70 @@ def test_walker_id_shards_reassemble_the_full_run():
         sums += po.reduce(sub, acq, r, s).sum_cos
     assert np.array_equal(np.concatenate(parts, axis=1).reshape(-1), st)
     assert np.allclose(sums, sig.sum_cos, rtol=1e-12, atol=1e-9)
+
+
+def test_gaussian_steps_give_exp_minus_bD_exactly():
+    # N(0, 2 D dt) per axis: the phase is Gaussian, E = exp(-b D) with the
+    # b of the sampled table (waveforms.b_discrete), within 3 sigma
+    gs = np.linspace(0.0, 0.25, 6) * 1e-6
+    acq = wf.build_acq([wf.RectGrad(g, 5.0, 10.0) for g in gs], 20000, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=20000, seed=8, step_mode=abi.STEP_GAUSSIAN)
+    sig, st = po.run_phase(sb.empty_box(20.0, 2.0), acq, run, bin_width=1.0)
+    for j in range(1, len(gs)):
+        b = wf.b_discrete(acq.profiles[:, acq.wf_profile[j]], acq.wf_amplitude[j], acq.dt, GAMMA)
+        assert abs(sig.E_re[j] - math.exp(-b * 2.0)) < 3 * sig.sigma_re[j], (j, sig.E_re[j])
+
+
+def test_gaussian_step_moments():
+    # one step: per-axis displacement N(0, 2 D dt): variance, kurtosis 3,
+    # isotropy, and lengths beyond the isotropic step length
+    acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 2, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=200000, seed=4, step_mode=abi.STEP_GAUSSIAN)
+    st = po.walk(sb.empty_box(50.0, 2.0), acq, run)
+    d = abi.state_view(st, run.n_walkers, 1)["disp"].astype(np.float64)
+    var = 2 * 2.0 * 1e-3
+    n = run.n_walkers
+    assert np.all(np.abs(d.mean(axis=0)) < 5 * math.sqrt(var / n))
+    assert np.all(np.abs(d.var(axis=0) / var - 1) < 5 * math.sqrt(2.0 / n))
+    kurt = (d ** 4).mean(axis=0) / d.var(axis=0) ** 2
+    assert np.all(np.abs(kurt - 3.0) < 5 * math.sqrt(96.0 / n))
+    L = math.sqrt(6 * 2.0 * 1e-3)
+    assert np.max(np.linalg.norm(d, axis=1)) > 2 * L
+
+
+def test_axis_sign_steps_give_the_cosine_product():
+    # +-s per axis: E = prod_k cos(gamma dt Q_k s) for a gradient along x
+    g = 0.2e-6
+    acq = wf.build_acq([wf.RectGrad(0.0, 5.0, 10.0), wf.RectGrad(g, 5.0, 10.0)], 20000, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=20000, seed=6, step_mode=abi.STEP_AXES)
+    sig, st = po.run_phase(sb.empty_box(20.0, 2.0), acq, run, bin_width=1.0)
+    prof = acq.profiles[:, acq.wf_profile[1]].astype(np.float64) * acq.wf_amplitude[1]
+    Q = np.cumsum(prof[::-1])[::-1][1:]          # Q_k = sum_{i >= k} G_i, k = 1..n-1
+    s = math.sqrt(2 * 2.0 * 1e-3)
+    exact = float(np.prod(np.cos(GAMMA * acq.dt * Q * s)))
+    assert abs(sig.E_re[1] - exact) < 3 * sig.sigma_re[1], (sig.E_re[1], exact)
+    # one step moves every axis by exactly +-s, all 8 sign patterns equally
+    one = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 2, 1e-3, GAMMA)
+    r1 = abi.RunSpec(n_walkers=8000, seed=6, step_mode=abi.STEP_AXES)
+    d = abi.state_view(po.walk(sb.empty_box(20.0, 2.0), one, r1), 8000, 1)["disp"].astype(np.float64)
+    assert np.allclose(np.abs(d), s, rtol=0.0, atol=4e-6)      # a few ulps of positions ~10 um
+    counts = np.bincount(((d > 0) * [1, 2, 4]).sum(axis=1), minlength=8)
+    assert counts.min() > 8000 / 8 - 5 * math.sqrt(8000 / 8)
+
+
+def test_gaussian_rejection_keeps_walkers_in_a_sphere():
+    # the reference's WC kernel: Gaussian steps, rejection at the pore wall
+    sub = sb.single_sphere(16.0, 5.0, D_in=2.0, D_ex=2.0)
+    acq = wf.build_acq([wf.RectGrad(0.1e-6, 5.0, 10.0)], 5000, 1e-3, GAMMA)
+    run = abi.RunSpec(n_walkers=4000, seed=2, step_mode=abi.STEP_GAUSSIAN,
+                      boundary_mode=abi.BOUNDARY_REJECTION, init_mode=abi.INIT_INSIDE_PRIM)
+    st = po.walk(sub, acq, run)
+    v = abi.state_view(st, run.n_walkers, acq.n_profiles)
+    r = np.linalg.norm(v["pos"].astype(np.float64) - 8.0, axis=1)
+    assert np.all(r < 5.0) and np.all(v["comp"] == 1) and int(v["ncoll"].sum()) > 1000
+
+
+def test_gaussian_steps_need_a_wide_cell():
+    acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 10, 1.0, GAMMA)      # L = 3.5 um
+    with pytest.raises(po.OracleError):
+        po.walk(sb.empty_box(30.0, 2.0), acq, abi.RunSpec(n_walkers=4, step_mode=abi.STEP_GAUSSIAN))

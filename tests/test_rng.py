@@ -86,3 +86,18 @@ def test_direction_distribution_is_isotropic():
     ux = st * np.cos(phis)
     assert abs(ux.mean()) < 4 / math.sqrt(n)
     assert abs((ux ** 2).mean() - 1 / 3) < 4 * math.sqrt(4 / 45 / n)
+
+
+def test_neg2log_unif_whole_grid_within_3_ulp():
+    """-2 ln u of the Gaussian steps against libm (fp64) at every one of the
+    2^23 values u = k 2^-24, k odd: at most 3 ulp of the fp32 result (the
+    atanh series to s^9 truncates below 4e-10, far under one ulp)."""
+    k = np.arange(1, 1 << 24, 2, dtype=np.uint64)
+    y = po.neg2log_unif((k << 8).astype(np.uint32)).astype(np.float64)
+    ref = -2.0 * np.log(k.astype(np.float64) * 2.0 ** -24)
+    ulp = np.abs(y - ref) / np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert ulp.max() < 3.0, ulp.max()
+    assert y.min() > 0.0 and abs(y.max() - ref.max()) < 1e-5
+    # the low byte of the word is ignored
+    assert np.array_equal(po.neg2log_unif(np.array([0x12345600, 0x123456FF], np.uint32))[0:1],
+                          po.neg2log_unif(np.array([0x123456FF], np.uint32)))
