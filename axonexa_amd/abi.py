@@ -294,3 +294,106 @@ def state_view(state: np.ndarray, n_walkers: int, n_profiles: int):
         "step": i[F_STEP],
         "mom": f[F_MOM:].reshape(n_profiles, 3, n_walkers).transpose(2, 0, 1),
     }
+
+
+# ---------------------------------------------------------------- Bloch path
+MAG_ROTATE, MAG_RK4, MAG_DOPRI = 0, 1, 2
+MAG_GROUP = 8
+
+
+class Field(C.Structure):
+    _fields_ = [
+        ("n_steps", C.c_int), ("dt", C.c_double), ("gamma", C.c_double),
+        ("n_fields", C.c_int), ("integrator", C.c_int),
+        ("substeps", _ip), ("samples", _dp),
+        ("origin", C.c_double * 3), ("m0", C.c_double * 3), ("T1", _dp),
+    ]
+
+
+class MagResult(C.Structure):
+    _fields_ = [
+        ("mx", _dp), ("my", _dp), ("mz", _dp), ("n_walkers", C.c_int64),
+        ("collisions", C.c_uint64), ("transmissions", C.c_uint64), ("cap_hits", C.c_uint64),
+    ]
+
+
+def mag_samples_per_step(integrator: int, substeps: np.ndarray) -> np.ndarray:
+    s = np.asarray(substeps, dtype=np.int64)
+    if integrator == MAG_ROTATE:
+        return s
+    if integrator == MAG_RK4:
+        return 2 * s + 1
+    return 6 * s
+
+
+@dataclass
+class FieldSpec:
+    """Sampled affine fields B(r, t) = b0(t) + J(t) (r - origin) of the
+    magnetization path: samples[k, j] = (b0[3], J[3][3] row major) of field j
+    at the k-th sample time (axx_field of include/axonexa.h)."""
+    n_steps: int
+    dt: float
+    gamma: float
+    samples: np.ndarray               # [n_samples, n_fields, 12] float64
+    integrator: int = MAG_ROTATE
+    substeps: Optional[np.ndarray] = None   # [n_steps] int32 or None (= 1)
+    origin: Sequence[float] = (0.0, 0.0, 0.0)
+    m0: Sequence[float] = (1.0, 0.0, 0.0)
+    T1: Optional[np.ndarray] = None   # [n_prims + 1] or None
+
+    def __post_init__(self):
+        self.samples = np.ascontiguousarray(self.samples, dtype=np.float64)
+        assert self.samples.ndim == 3 and self.samples.shape[2] == 12
+        if self.substeps is not None:
+            self.substeps = np.ascontiguousarray(self.substeps, dtype=np.int32).reshape(self.n_steps)
+        if self.T1 is not None:
+            self.T1 = np.ascontiguousarray(self.T1, dtype=np.float64).reshape(-1)
+        sub = self.substeps if self.substeps is not None else np.ones(self.n_steps, np.int64)
+        need = int(mag_samples_per_step(self.integrator, sub).sum())
+        assert self.samples.shape[0] == need, (self.samples.shape, need)
+
+    @property
+    def n_fields(self) -> int:
+        return int(self.samples.shape[1])
+
+    def struct(self) -> Field:
+        f = Field()
+        f.n_steps, f.dt, f.gamma = int(self.n_steps), float(self.dt), float(self.gamma)
+        f.n_fields, f.integrator = self.n_fields, int(self.integrator)
+        f.substeps = _ptr(self.substeps, C.c_int)
+        f.samples = _ptr(self.samples, C.c_double)
+        for a in range(3):
+            f.origin[a] = float(self.origin[a])
+            f.m0[a] = float(self.m0[a])
+        f.T1 = _ptr(self.T1, C.c_double)
+        f._keep = self
+        return f
+
+
+@dataclass
+class MagSignals:
+    """Sums over walkers of the magnetization per field."""
+    mx: np.ndarray
+    my: np.ndarray
+    mz: np.ndarray
+    n_walkers: int
+    collisions: int = 0
+    transmissions: int = 0
+    cap_hits: int = 0
+
+    @property
+    def transverse(self) -> np.ndarray:
+        """|Mxy| / N (the signal of magAcquisition::calcADC)."""
+        return np.hypot(self.mx, self.my) / self.n_walkers
+
+
+def mag_state_fields(n_fields_in_group: int) -> int:
+    return F_MOM + 6 * n_fields_in_group
+
+
+def mag_from_state(state: np.ndarray, n_walkers: int, F: int) -> np.ndarray:
+    """[n_walkers, F, 3] float64 magnetization from a group's SoA state."""
+    st = state.reshape(mag_state_fields(F), n_walkers)
+    words = st[F_MOM:].reshape(F, 3, 2, n_walkers)
+    both = (words[:, :, 0].astype(np.uint64) | (words[:, :, 1].astype(np.uint64) << np.uint64(32)))
+    return both.view(np.float64).transpose(2, 0, 1)

@@ -41,8 +41,11 @@ struct BatchWalkArgs {
 
 template <int PT, bool GEN, bool REJ>
 __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
-    for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x)
-        walk_lane<PT, 7, GEN, REJ, true>(A, T, A.prof, (uint32_t)w);
+    for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x) {
+        PhaseAcc<PT, 7> acc;
+        acc.prof = (const_fptr)A.prof;
+        walk_lane<PhaseAcc<PT, 7>, GEN, REJ, true>(A, T, acc, (uint32_t)w);
+    }
 }
 
 __global__ __launch_bounds__(kBatchBlockMax) void batch_walk_kernel(BatchWalkArgs B) {

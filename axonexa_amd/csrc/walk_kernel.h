@@ -173,38 +173,70 @@ walk_fn pick_walk_p64(int ax, bool gen, bool rej, bool alt);
 // limited to 512 threads per block so that each lane may use 256 VGPRs.
 constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 
+// Phase moments of P gradient profiles over the axes in AX
+// (kernelLattice.cuh:81-85): m_p += g_p(i) * (r_i - r_0) per step, with the
+// wave-uniform profile samples of step i.
+template <int P, int AX>
+struct PhaseAcc {
+    static constexpr bool kOneSite = false;
+    const_fptr prof;
+    float m[P][3];
+
+    __device__ __forceinline__ void load(const float* fs, long long N, uint32_t wo) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            m[p][0] = (AX & 1) ? (fs + (F_MOM + 3 * p + 0) * N)[wo] : 0.0f;
+            m[p][1] = (AX & 2) ? (fs + (F_MOM + 3 * p + 1) * N)[wo] : 0.0f;
+            m[p][2] = (AX & 4) ? (fs + (F_MOM + 3 * p + 2) * N)[wo] : 0.0f;
+        }
+    }
+    // the position at t = 0 carries no moment
+    __device__ __forceinline__ void at_start(float, float, float, int) {}
+    __device__ __forceinline__ void step(int i, float dx, float dy, float dz, int) {
+        const_fptr g = prof + (size_t)i * P;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const float gp = g[p];
+            if (AX & 1) m[p][0] = fmaf(gp, dx, m[p][0]);
+            if (AX & 2) m[p][1] = fmaf(gp, dy, m[p][1]);
+            if (AX & 4) m[p][2] = fmaf(gp, dz, m[p][2]);
+        }
+    }
+    __device__ __forceinline__ void store(float* fs, long long N, uint32_t wo) const {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            if (AX & 1) (fs + (F_MOM + 3 * p + 0) * N)[wo] = m[p][0];
+            if (AX & 2) (fs + (F_MOM + 3 * p + 1) * N)[wo] = m[p][1];
+            if (AX & 4) (fs + (F_MOM + 3 * p + 2) * N)[wo] = m[p][2];
+        }
+    }
+};
+
 // One lane walks walker wo of A over steps [max(A.i0, 1), A.i1) with the
-// substrate tables T in LDS.  GEN: substrates with spheres or planes (3D
-// bins, per-image meta words); otherwise cylinders only.  ALT: the axis-sign
-// and Gaussian step generators are compiled in (selected by A.step_mode);
+// substrate tables T in LDS; acc sees the walker at t = 0 (when A.i0 <= 0)
+// and after every step.  GEN: substrates with spheres or planes (3D bins,
+// per-image meta words); otherwise cylinders only.  ALT: the axis-sign and
+// Gaussian step generators are compiled in (selected by A.step_mode);
 // otherwise isotropic steps only.
-template <int P, int AX, bool GEN, bool REJ, bool ALT>
-__device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
-                                          const float* __restrict__ prof_g, const uint32_t wo) {
+template <class Acc, bool GEN, bool REJ, bool ALT>
+__device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T, Acc& acc, const uint32_t wo) {
     const long long N = A.N;
     uint32_t* st = A.state;
     float* fs = reinterpret_cast<float*>(st);
     int* is = reinterpret_cast<int*>(st);
     const uint32_t wid = A.woff + wo;
-    const_fptr prof = (const_fptr)prof_g;
 
     // field f of this walker: base + f*N (wave-uniform) + wo (32-bit lane offset)
     float x, y, z, dx, dy, dz, E, lf;
     int comp, li;
     uint32_t ncoll, ntran, ncap;
-    float m[P][3];
     x = (fs + F_X * N)[wo]; y = (fs + F_Y * N)[wo]; z = (fs + F_Z * N)[wo];
     dx = (fs + F_DX * N)[wo]; dy = (fs + F_DY * N)[wo]; dz = (fs + F_DZ * N)[wo];
     E = (fs + F_E * N)[wo]; lf = (fs + F_LF * N)[wo];
     comp = (is + F_COMP * N)[wo]; li = (is + F_LI * N)[wo];
     ncoll = (st + F_NCOLL * N)[wo]; ntran = (st + F_NTRAN * N)[wo];
     ncap = (st + F_NCAP * N)[wo];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        m[p][0] = (AX & 1) ? (fs + (F_MOM + 3 * p + 0) * N)[wo] : 0.0f;
-        m[p][1] = (AX & 2) ? (fs + (F_MOM + 3 * p + 1) * N)[wo] : 0.0f;
-        m[p][2] = (AX & 4) ? (fs + (F_MOM + 3 * p + 2) * N)[wo] : 0.0f;
-    }
+    acc.load(fs, N, wo);
 
     const int cap = A.cap;
     const bool permeable = A.mode == 1;
@@ -437,15 +469,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         x = wrap ? wrap1(ex, A.a) : ex;
         y = wrap ? wrap1(ey, A.b) : ey;
         z = (wrap && GEN) ? wrap1(ez, A.c) : ez;   // cylinders only: z enters no geometry
-        // phase moments (kernelLattice.cuh:81-85): wave-uniform profile samples
-        const_fptr g = prof + (size_t)i * P;
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const float gp = g[p];
-            if (AX & 1) m[p][0] = fmaf(gp, dx, m[p][0]);
-            if (AX & 2) m[p][1] = fmaf(gp, dy, m[p][1]);
-            if (AX & 4) m[p][2] = fmaf(gp, dz, m[p][2]);
-        }
+        acc.step(i, dx, dy, dz, comp);
     };
 
     // Steps are paired: one Philox call, counter (walker, (i-1)/2, 0, instance),
@@ -458,24 +482,42 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
             g = philox10(wid, (uint32_t)(ii - 1) >> 1, kStreamGauss, A.inst, A.k0, A.k1);
         return g;
     };
+    if (A.i0 < 1 && A.i1 > 0) acc.at_start(dx, dy, dz, comp);
     int i = A.i0 < 1 ? 1 : A.i0;
-    if (((i - 1) & 1) && i < A.i1) {
-        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        const u32x4 g = gauss(i);
-        step(i, o.z, o.w, g.z, g.w, lookup(), true);
-        ++i;
-    }
-    for (; i + 1 < A.i1; i += 2) {
-        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        const u32x4 g = gauss(i);
-        const Cands c = lookup();
-        step(i, o.x, o.y, g.x, g.y, c, false);
-        step(i + 1, o.z, o.w, g.z, g.w, c, true);
-    }
-    if (i < A.i1) {
-        const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
-        const u32x4 g = gauss(i);
-        step(i, o.x, o.y, g.x, g.y, lookup(), false);
+    if (Acc::kOneSite) {
+        // one step per iteration (draws and lookup on the first step of each
+        // pair), so that a heavy per-step update is inlined once
+        u32x4 o{0u, 0u, 0u, 0u}, g{0u, 0u, 0u, 0u};
+        Cands c{0, T.cand};
+        const int ifirst = i;
+        for (; i < A.i1; ++i) {
+            const bool second = ((i - 1) & 1) != 0;
+            if (!second || i == ifirst) {
+                o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+                g = gauss(i);
+                c = lookup();
+            }
+            step(i, second ? o.z : o.x, second ? o.w : o.y, second ? g.z : g.x, second ? g.w : g.y, c, second);
+        }
+    } else {
+        if (((i - 1) & 1) && i < A.i1) {
+            const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+            const u32x4 g = gauss(i);
+            step(i, o.z, o.w, g.z, g.w, lookup(), true);
+            ++i;
+        }
+        for (; i + 1 < A.i1; i += 2) {
+            const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+            const u32x4 g = gauss(i);
+            const Cands c = lookup();
+            step(i, o.x, o.y, g.x, g.y, c, false);
+            step(i + 1, o.z, o.w, g.z, g.w, c, true);
+        }
+        if (i < A.i1) {
+            const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
+            const u32x4 g = gauss(i);
+            step(i, o.x, o.y, g.x, g.y, lookup(), false);
+        }
     }
 
     // Re-derive the state addresses after the loop (the empty asm hides
@@ -494,12 +536,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     (sto + F_NCOLL * N)[wo2] = ncoll; (sto + F_NTRAN * N)[wo2] = ntran;
     (sto + F_NCAP * N)[wo2] = ncap;
     (iso + F_STEP * N)[wo2] = A.i1;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if (AX & 1) (fso + (F_MOM + 3 * p + 0) * N)[wo2] = m[p][0];
-        if (AX & 2) (fso + (F_MOM + 3 * p + 1) * N)[wo2] = m[p][1];
-        if (AX & 4) (fso + (F_MOM + 3 * p + 2) * N)[wo2] = m[p][2];
-    }
+    acc.store(fso, N, wo2);
 }
 
 template <int P, int AX, bool GEN, bool REJ, bool ALT>
@@ -508,7 +545,9 @@ __global__ __launch_bounds__(walk_max_block(P)) void walk_kernel(WalkArgs A, con
     const LdsTables T = stage_tables<GEN>(A, lds);
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= A.N) return;
-    walk_lane<P, AX, GEN, REJ, ALT>(A, T, prof_g, (uint32_t)w);
+    PhaseAcc<P, AX> acc;
+    acc.prof = (const_fptr)prof_g;
+    walk_lane<PhaseAcc<P, AX>, GEN, REJ, ALT>(A, T, acc, (uint32_t)w);
 }
 
 }  // namespace axx

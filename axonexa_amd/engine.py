@@ -92,6 +92,15 @@ def lib():
     L.axx_mlp_set_data.argtypes = [vp, fp, fp, C.c_int]
     L.axx_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, fp]
     L.axx_mlp_destroy.argtypes = [vp]
+    Fp, MR = C.POINTER(abi.Field), C.POINTER(abi.MagResult)
+    L.axx_mag_sample_count.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.axx_mag_create.argtypes = [vp, S, Fp, R, C.POINTER(vp)]
+    L.axx_mag_walk.argtypes = [vp, C.c_int, C.c_int]
+    L.axx_mag_time_walk.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.axx_mag_results.argtypes = [vp, MR]
+    L.axx_mag_get_state.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32)]
+    L.axx_mag_destroy.argtypes = [vp]
+    L.axx_run_mag.argtypes = [vp, S, Fp, R, MR]
     _LIB = L
     return L
 
@@ -229,6 +238,91 @@ def run_phase(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
     ctx.check(lib().axx_run_phase(ctx.handle, C.byref(s), C.byref(q), C.byref(r), C.byref(res)),
               "axx_run_phase")
     return abi.signals_from(res, bufs)
+
+
+def _mag_result(n_fields: int):
+    bufs = {k: np.zeros(n_fields) for k in ("mx", "my", "mz")}
+    r = abi.MagResult()
+    for k, v in bufs.items():
+        setattr(r, k, v.ctypes.data_as(C.POINTER(C.c_double)))
+    return r, bufs
+
+
+def _mag_signals(r, bufs) -> abi.MagSignals:
+    return abi.MagSignals(n_walkers=int(r.n_walkers), collisions=int(r.collisions),
+                          transmissions=int(r.transmissions), cap_hits=int(r.cap_hits),
+                          **{k: v.copy() for k, v in bufs.items()})
+
+
+class Mag:
+    """Magnetization (Bloch) run: create -> walk (chunks) -> results
+    (axx_mag_*)."""
+
+    def __init__(self, sub: abi.SubstrateSpec, field: abi.FieldSpec, run: abi.RunSpec,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.sub, self.field, self.run = sub, field, run
+        self._s, self._f, self._r = sub.struct(), field.struct(), run.struct()
+        self._h = C.c_void_p()
+        self.ctx.check(lib().axx_mag_create(self.ctx.handle, C.byref(self._s), C.byref(self._f),
+                                            C.byref(self._r), C.byref(self._h)), "axx_mag_create")
+
+    @property
+    def n_groups(self) -> int:
+        return (self.field.n_fields + abi.MAG_GROUP - 1) // abi.MAG_GROUP
+
+    def group_fields(self, g: int) -> int:
+        return min(abi.MAG_GROUP, self.field.n_fields - g * abi.MAG_GROUP)
+
+    def walk(self, step_begin: int = 0, step_end: Optional[int] = None):
+        if step_end is None:
+            step_end = self.field.n_steps
+        self.ctx.check(lib().axx_mag_walk(self._h, int(step_begin), int(step_end)), "axx_mag_walk")
+        return self
+
+    def time_walk(self, step_begin: int, step_end: int) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().axx_mag_time_walk(self._h, int(step_begin), int(step_end), C.byref(ms)),
+                       "axx_mag_time_walk")
+        return ms.value
+
+    def results(self) -> abi.MagSignals:
+        res, bufs = _mag_result(self.field.n_fields)
+        self.ctx.check(lib().axx_mag_results(self._h, C.byref(res)), "axx_mag_results")
+        return _mag_signals(res, bufs)
+
+    def get_state(self, group: int = 0) -> np.ndarray:
+        st = np.zeros(abi.mag_state_fields(self.group_fields(group)) * self.run.n_walkers, np.uint32)
+        self.ctx.check(lib().axx_mag_get_state(self._h, int(group), st.ctypes.data_as(C.POINTER(C.c_uint32))),
+                       "axx_mag_get_state")
+        return st
+
+    def close(self):
+        if self._h:
+            lib().axx_mag_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def run_mag(sub: abi.SubstrateSpec, field: abi.FieldSpec, run: abi.RunSpec,
+            ctx: Optional[Context] = None) -> abi.MagSignals:
+    """One-shot magnetization run (axx_run_mag)."""
+    ctx = ctx or default_context()
+    s, f, r = sub.struct(), field.struct(), run.struct()
+    res, bufs = _mag_result(field.n_fields)
+    ctx.check(lib().axx_run_mag(ctx.handle, C.byref(s), C.byref(f), C.byref(r), C.byref(res)), "axx_run_mag")
+    return _mag_signals(res, bufs)
 
 
 class Batch:

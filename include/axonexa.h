@@ -302,6 +302,77 @@ int  axx_mlp_set_data(axx_mlp* m, const float* x, const float* t, int n);
 int  axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_t seed, float* loss_trace);
 void axx_mlp_destroy(axx_mlp* m);
 
+/*
+ * Magnetization (Bloch) path: magAcquisitionStream<F>::runAcquisition
+ * (magAcquisitionStream.cuh:37-103) running updateWalkersMag
+ * (kernelMag.cuh:1-141).  The walkers move exactly as in axx_run_phase with
+ * the same substrate and run (same random numbers, so the same trajectories);
+ * each carries one magnetization vector per field, integrated in double
+ * precision at t_i = i dt, i = 0 .. n_steps-1, with the field at its
+ * position after step i (its start for i = 0):
+ *   AXX_MAG_ROTATE  exact rotation about B by -gamma |B| h per substep
+ *                   (updateMag_noT2_rotate / _multistep, blochdiff.cuh:150-210)
+ *   AXX_MAG_RK4     classical Runge-Kutta on dM/dt = gamma M x B, with
+ *                   -(Mx/T2, My/T2, (Mz-1)/T1) when run.relaxation
+ *                   (updateMagRK4, blochdiff.cuh:24-67)
+ *   AXX_MAG_DOPRI   Dormand-Prince fifth-order steps, same equation
+ *                   (updateMagDORP, blochdiff.cuh:68-148; fixed substeps)
+ * Step i is split into substeps[i] substeps of h = dt / substeps[i].  Fields
+ * are affine in the position, B(r, t) = b0(t) + J(t) (r - origin), r the
+ * unwrapped position; samples[] holds, step after step and for each sample
+ * time of the step, one record of 12 doubles (b0[3], J[3][3] row major) per
+ * field.  Sample times of step i, t_i = i dt:
+ *   ROTATE  t_i + k h (k < s_i);  RK4  t_i + k h/2 (k <= 2 s_i);
+ *   DOPRI   t_i + (k + c) h (k < s_i; c = 0, 1/5, 3/10, 4/5, 8/9, 1),
+ * i.e. s_i, 2 s_i + 1 and 6 s_i samples (axx_mag_sample_count).
+ * Fields are walked in groups of AXX_MAG_GROUP; every group walks the same
+ * walkers.  T1: per compartment (lattice, then primitives), NULL = none;
+ * T2 comes from the substrate.  Results are the sums over walkers of Mx, My,
+ * Mz per field (magAcquisition::getMx/getMy/getMz).
+ */
+#define AXX_MAG_ROTATE 0
+#define AXX_MAG_RK4    1
+#define AXX_MAG_DOPRI  2
+#define AXX_MAG_GROUP  8
+typedef struct axx_field {
+    int           n_steps;
+    double        dt;
+    double        gamma;
+    int           n_fields;
+    int           integrator;   /* AXX_MAG_*                                  */
+    const int*    substeps;     /* [n_steps], each in [1, 4096]; NULL = all 1 */
+    const double* samples;      /* see above                                  */
+    double        origin[3];
+    double        m0[3];        /* magnetization at t = 0                     */
+    const double* T1;           /* [n_prims + 1] or NULL                      */
+} axx_field;
+
+typedef struct axx_mag_result {
+    double*  mx;                /* [n_fields] sums over walkers (or NULL)     */
+    double*  my;
+    double*  mz;
+    int64_t  n_walkers;
+    uint64_t collisions, transmissions, cap_hits;
+} axx_mag_result;
+
+typedef struct axx_mag axx_mag;
+/* Number of field samples the table must hold (per field). */
+int  axx_mag_sample_count(int integrator, int n_steps, const int* substeps, int64_t* n_samples);
+int  axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
+                    axx_mag** out);
+/* Steps [step_begin, step_end) of every group; 0 places the walkers and sets
+ * M = m0 first (same chunking rules as axx_plan_walk). */
+int  axx_mag_walk(axx_mag* m, int step_begin, int step_end);
+int  axx_mag_time_walk(axx_mag* m, int step_begin, int step_end, float* ms);
+int  axx_mag_results(axx_mag* m, axx_mag_result* out);   /* synchronises */
+/* Walker state of one group: AXX_F_MOM + 6 * F words per walker (F fields
+ * in the group); M of field j, component c is the double whose low word is
+ * AXX_F_MOM + 6 j + 2 c and high word the next one. */
+int  axx_mag_get_state(axx_mag* m, int group, uint32_t* host_state);
+void axx_mag_destroy(axx_mag* m);
+int  axx_run_mag(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
+                 axx_mag_result* out);
+
 /* Library version string. */
 const char* axx_version(void);
 

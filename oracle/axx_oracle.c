@@ -149,6 +149,198 @@ void orc_neg2log_unif(const uint32_t* w, float* out, int64_t n) {
     for (int64_t k = 0; k < n; ++k) out[k] = neg2log_unif(w[k]);
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Magnetization (Bloch) path in double precision (blochdiff.cuh:7-210,
+ * kernelMag.cuh:1-141); same operations in the same order as the kernel
+ * (axonexa_amd/csrc/mag_math.h), so each walker's M matches bit for bit. */
+typedef struct { double x, y, z; } d3;
+
+/* sqrt and a residual test of its two neighbours (libm's sqrt is already
+ * correctly rounded; the test then keeps it) */
+static inline double sqrt_cr(double v) {
+    double y = sqrt(v);
+    int64_t b;
+    memcpy(&b, &y, 8);
+    int64_t bd = b - 1, bu = b + 1;
+    double ydn, yup;
+    memcpy(&ydn, &bd, 8);
+    memcpy(&yup, &bu, 8);
+    double rdn = fma(-ydn, y, v), rup = fma(-yup, y, v);
+    y = rdn <= 0.0 ? ydn : y;
+    y = rup > 0.0 ? yup : y;
+    return y;
+}
+
+/* sin, cos: Cody-Waite reduction by pi/2 (three fused parts), Taylor
+ * polynomials of sin to r^17 and cos to r^16 on |r| <= pi/4 */
+static inline void sincos64(double x, double* s, double* c) {
+    const double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = fma(-k, 0x1.921fb54442d18p+0, x);
+    r = fma(-k, 0x1.1a62633145c07p-54, r);
+    r = fma(-k, -0x1.f1976b7ed8fbcp-110, r);
+    const double z = r * r;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = fma(z, ps, -0x1.ae7f3e733b81fp-41);
+    ps = fma(z, ps, 0x1.6124613a86d09p-33);
+    ps = fma(z, ps, -0x1.ae64567f544e4p-26);
+    ps = fma(z, ps, 0x1.71de3a556c734p-19);
+    ps = fma(z, ps, -0x1.a01a01a01a01ap-13);
+    ps = fma(z, ps, 0x1.1111111111111p-7);
+    ps = fma(z, ps, -0x1.5555555555555p-3);
+    const double sr = fma(r * z, ps, r);
+    double pc = 0x1.ae7f3e733b81fp-45;
+    pc = fma(z, pc, -0x1.93974a8c07c9dp-37);
+    pc = fma(z, pc, 0x1.1eed8eff8d898p-29);
+    pc = fma(z, pc, -0x1.27e4fb7789f5cp-22);
+    pc = fma(z, pc, 0x1.a01a01a01a01ap-16);
+    pc = fma(z, pc, -0x1.6c16c16c16c17p-10);
+    pc = fma(z, pc, 0x1.5555555555555p-5);
+    pc = fma(z, pc, -0.5);
+    const double cr = fma(z, pc, 1.0);
+    const int q = (int)k & 3;
+    const double a = (q & 1) ? cr : sr;
+    const double b = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -a : a;
+    *c = (q == 1 || q == 2) ? -b : b;
+}
+
+void orc_sincos64(const double* x, double* s, double* c, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) sincos64(x[k], &s[k], &c[k]);
+}
+
+void orc_sqrt_cr(const double* x, double* y, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) y[k] = sqrt_cr(x[k]);
+}
+
+/* B = b0 + J r for one record (b0[3], J row major) */
+static inline d3 field_at(const double* rec, double rx, double ry, double rz) {
+    d3 B;
+    B.x = fma(rec[5], rz, fma(rec[4], ry, fma(rec[3], rx, rec[0])));
+    B.y = fma(rec[8], rz, fma(rec[7], ry, fma(rec[6], rx, rec[1])));
+    B.z = fma(rec[11], rz, fma(rec[10], ry, fma(rec[9], rx, rec[2])));
+    return B;
+}
+
+/* rotation about B by -gh |B| (updateMag_noT2_rotate, blochdiff.cuh:186-199) */
+static inline void mrotate(d3* M, d3 B, double gh) {
+    const double b2 = fma(B.z, B.z, fma(B.y, B.y, B.x * B.x));
+    if (!(b2 > 0.0)) return;
+    const double bn = sqrt_cr(b2);
+    const double ib = 1.0 / bn;
+    double s, c;
+    sincos64(-(gh * bn), &s, &c);
+    const double f = (fma(B.z, M->z, fma(B.y, M->y, B.x * M->x)) * ib) * ib;
+    const double sb = s * ib;
+    const double cx = f * B.x, cy = f * B.y, cz = f * B.z;
+    const double wx = fma(B.y, M->z, -(B.z * M->y));
+    const double wy = fma(B.z, M->x, -(B.x * M->z));
+    const double wz = fma(B.x, M->y, -(B.y * M->x));
+    M->x = fma(wx, sb, fma(M->x - cx, c, cx));
+    M->y = fma(wy, sb, fma(M->y - cy, c, cy));
+    M->z = fma(wz, sb, fma(M->z - cz, c, cz));
+}
+
+/* h (gamma M x B - relaxation) (bloch, blochdiff.cuh:7-22) */
+static inline d3 bloch_h(d3 M, d3 B, double gam, double h, int relax, double i1, double i2) {
+    d3 d;
+    d.x = gam * fma(M.y, B.z, -(M.z * B.y));
+    d.y = gam * fma(M.z, B.x, -(M.x * B.z));
+    d.z = gam * fma(M.x, B.y, -(M.y * B.x));
+    if (relax) {
+        d.x = fma(-M.x, i2, d.x);
+        d.y = fma(-M.y, i2, d.y);
+        d.z = fma(-(M.z - 1.0), i1, d.z);
+    }
+    d.x = d.x * h; d.y = d.y * h; d.z = d.z * h;
+    return d;
+}
+
+static inline d3 axpy(double a, d3 x, d3 y) {
+    d3 r = {fma(a, x.x, y.x), fma(a, x.y, y.y), fma(a, x.z, y.z)};
+    return r;
+}
+
+/* classical RK4 (updateMagRK4, blochdiff.cuh:24-67) */
+static inline void mrk4(d3* M, d3 B0, d3 Bh, d3 B1, double gam, double h, int relax, double i1, double i2) {
+    d3 k1 = bloch_h(*M, B0, gam, h, relax, i1, i2);
+    d3 k2 = bloch_h(axpy(0.5, k1, *M), Bh, gam, h, relax, i1, i2);
+    d3 k3 = bloch_h(axpy(0.5, k2, *M), Bh, gam, h, relax, i1, i2);
+    d3 k4 = bloch_h(axpy(1.0, k3, *M), B1, gam, h, relax, i1, i2);
+    d3 t = axpy(2.0, k3, axpy(2.0, k2, k1));
+    d3 u = {t.x + k4.x, t.y + k4.y, t.z + k4.z};
+    *M = axpy(1.0 / 6.0, u, *M);
+}
+
+/* Dormand-Prince fifth-order step (updateMagDORP, blochdiff.cuh:68-118),
+ * with the published a63 = +46732/5247 (the reference has -46732/5247) */
+static inline void mdopri5(d3* M, const d3* B, double gam, double h, int relax, double i1, double i2) {
+    d3 k1 = bloch_h(*M, B[0], gam, h, relax, i1, i2);
+    d3 k2 = bloch_h(axpy(1.0 / 5.0, k1, *M), B[1], gam, h, relax, i1, i2);
+    d3 k3 = bloch_h(axpy(9.0 / 40.0, k2, axpy(3.0 / 40.0, k1, *M)), B[2], gam, h, relax, i1, i2);
+    d3 k4 = bloch_h(axpy(32.0 / 9.0, k3, axpy(-56.0 / 15.0, k2, axpy(44.0 / 45.0, k1, *M))), B[3], gam, h,
+                    relax, i1, i2);
+    d3 k5 = bloch_h(axpy(-212.0 / 729.0, k4,
+                         axpy(64448.0 / 6561.0, k3, axpy(-25360.0 / 2187.0, k2, axpy(19372.0 / 6561.0, k1, *M)))),
+                    B[4], gam, h, relax, i1, i2);
+    d3 k6 = bloch_h(axpy(-5103.0 / 18656.0, k5,
+                         axpy(49.0 / 176.0, k4,
+                              axpy(46732.0 / 5247.0, k3, axpy(-355.0 / 33.0, k2, axpy(9017.0 / 3168.0, k1, *M))))),
+                    B[5], gam, h, relax, i1, i2);
+    *M = axpy(11.0 / 84.0, k6,
+              axpy(-2187.0 / 6784.0, k5,
+                   axpy(125.0 / 192.0, k4, axpy(500.0 / 1113.0, k3, axpy(35.0 / 384.0, k1, *M)))));
+}
+
+/* samples per step for s substeps */
+static inline int mag_samples(int integ, int s) {
+    return integ == AXX_MAG_ROTATE ? s : (integ == AXX_MAG_RK4 ? 2 * s + 1 : 6 * s);
+}
+
+/* What a walk of one group of fields needs: fields g0 .. g0+F-1 of f. */
+typedef struct {
+    const axx_field* f;
+    int g0, F;
+    const int* off;       /* [n_steps + 1] first sample of each step */
+    const double* iT1;    /* per compartment */
+    const double* iT2;
+    int relax;
+} magjob_t;
+
+/* M of every field after step i at unwrapped position r0 + d (kernelMag.cuh:93-136) */
+static void mag_update(const magjob_t* G, d3* M, int i, float x0, float y0, float z0, float dx, float dy,
+                       float dz, int comp) {
+    const axx_field* f = G->f;
+    const double rx = ((double)x0 + (double)dx) - f->origin[0];
+    const double ry = ((double)y0 + (double)dy) - f->origin[1];
+    const double rz = ((double)z0 + (double)dz) - f->origin[2];
+    const int s0 = G->off[i];
+    const int ns = f->substeps ? f->substeps[i] : 1;
+    const double h = f->dt / (double)ns;
+    double i1 = 0.0, i2 = 0.0;
+    if (f->integrator != AXX_MAG_ROTATE && G->relax) { i1 = G->iT1[comp]; i2 = G->iT2[comp]; }
+    const int NF = f->n_fields;
+#define REC(k, j) (f->samples + ((size_t)(k) * NF + G->g0 + (j)) * 12)
+    for (int j = 0; j < G->F; ++j) {
+        d3 m = M[j];
+        for (int k = 0; k < ns; ++k) {
+            if (f->integrator == AXX_MAG_ROTATE) {
+                mrotate(&m, field_at(REC(s0 + k, j), rx, ry, rz), f->gamma * h);
+            } else if (f->integrator == AXX_MAG_RK4) {
+                int q = s0 + 2 * k;
+                mrk4(&m, field_at(REC(q, j), rx, ry, rz), field_at(REC(q + 1, j), rx, ry, rz),
+                     field_at(REC(q + 2, j), rx, ry, rz), f->gamma, h, G->relax, i1, i2);
+            } else {
+                d3 B[6];
+                for (int c = 0; c < 6; ++c) B[c] = field_at(REC(s0 + 6 * k + c, j), rx, ry, rz);
+                mdopri5(&m, B, f->gamma, h, G->relax, i1, i2);
+            }
+        }
+        M[j] = m;
+    }
+#undef REC
+}
+
 /* ------------------------------------------------------------------ */
 /* Prepared substrate: primitives with their periodic images (what the
  * reference's two-image test plus correctEdges copies cover,
@@ -602,6 +794,7 @@ typedef struct {
     int i0, i1;
     int use_bins;
     int64_t w_begin, w_end;
+    const magjob_t* mag;   /* magnetization of a group of fields instead of phase moments */
     int err;
 } job_t;
 
@@ -668,6 +861,8 @@ static void* job_run(void* arg) {
     const uint32_t key[2] = {(uint32_t)R->seed, (uint32_t)(R->seed >> 32)};
     const uint32_t inst = R->instance;
     float m[AXX_MAX_PROFILES][3];
+    d3 Mg[AXX_MAG_GROUP];
+    const magjob_t* G = J->mag;
 
     for (int64_t w = J->w_begin; w < J->w_end; ++w) {
         uint32_t wid = R->walker_offset + (uint32_t)w;
@@ -683,6 +878,13 @@ static void* job_run(void* arg) {
             memset(m, 0, sizeof(m));
             FS(AXX_F_X0)[w] = x; FS(AXX_F_Y0)[w] = y; FS(AXX_F_Z0)[w] = z;
             IS(AXX_F_COMP0)[w] = comp;
+            if (G) {
+                /* M = m0, then the update at t = 0 at the start point */
+                for (int j = 0; j < G->F; ++j) {
+                    Mg[j].x = G->f->m0[0]; Mg[j].y = G->f->m0[1]; Mg[j].z = G->f->m0[2];
+                }
+                if (J->i1 > 0) mag_update(G, Mg, 0, x, y, z, 0.0f, 0.0f, 0.0f, comp);
+            }
         } else {
             x = FS(AXX_F_X)[w]; y = FS(AXX_F_Y)[w]; z = FS(AXX_F_Z)[w];
             dx = FS(AXX_F_DX)[w]; dy = FS(AXX_F_DY)[w]; dz = FS(AXX_F_DZ)[w];
@@ -690,9 +892,24 @@ static void* job_run(void* arg) {
             comp = IS(AXX_F_COMP)[w]; li = IS(AXX_F_LI)[w];
             ncoll = (uint32_t)IS(AXX_F_NCOLL)[w]; ntran = (uint32_t)IS(AXX_F_NTRAN)[w];
             ncap = (uint32_t)IS(AXX_F_NCAP)[w];
-            for (int p = 0; p < Pn; ++p)
-                for (int a = 0; a < 3; ++a) m[p][a] = FS(AXX_F_MOM + 3 * p + a)[w];
+            if (G) {
+                const uint32_t* U = J->st;
+                for (int j = 0; j < G->F; ++j) {
+                    double v[3];
+                    for (int c = 0; c < 3; ++c) {
+                        uint64_t lo = U[(size_t)(AXX_F_MOM + 6 * j + 2 * c) * J->N + w];
+                        uint64_t hi = U[(size_t)(AXX_F_MOM + 6 * j + 2 * c + 1) * J->N + w];
+                        uint64_t b = lo | (hi << 32);
+                        memcpy(&v[c], &b, 8);
+                    }
+                    Mg[j].x = v[0]; Mg[j].y = v[1]; Mg[j].z = v[2];
+                }
+            } else {
+                for (int p = 0; p < Pn; ++p)
+                    for (int a = 0; a < 3; ++a) m[p][a] = FS(AXX_F_MOM + 3 * p + a)[w];
+            }
         }
+        const float x0 = FS(AXX_F_X0)[w], y0 = FS(AXX_F_Y0)[w], z0 = FS(AXX_F_Z0)[w];
         int ibeg = J->i0 < 1 ? 1 : J->i0;
         uint32_t o[4] = {0, 0, 0, 0}, og[4] = {0, 0, 0, 0};
         for (int i = ibeg; i < J->i1; ++i) {
@@ -868,6 +1085,10 @@ static void* job_run(void* arg) {
                 y = ey;
                 z = ez;
             }
+            if (G) {
+                mag_update(G, Mg, i, x0, y0, z0, dx, dy, dz, comp);
+                continue;
+            }
             /* phase moments (kernelLattice.cuh:81-85, factored by profile) */
             const float* g = q->profiles + (size_t)i * Pn;
             for (int p = 0; p < Pn; ++p) {
@@ -884,8 +1105,21 @@ static void* job_run(void* arg) {
         IS(AXX_F_NCOLL)[w] = (int32_t)ncoll; IS(AXX_F_NTRAN)[w] = (int32_t)ntran;
         IS(AXX_F_NCAP)[w] = (int32_t)ncap;
         IS(AXX_F_STEP)[w] = J->i1;
-        for (int p = 0; p < Pn; ++p)
-            for (int a = 0; a < 3; ++a) FS(AXX_F_MOM + 3 * p + a)[w] = m[p][a];
+        if (G) {
+            uint32_t* U = J->st;
+            for (int j = 0; j < G->F; ++j) {
+                const double v[3] = {Mg[j].x, Mg[j].y, Mg[j].z};
+                for (int c = 0; c < 3; ++c) {
+                    uint64_t b;
+                    memcpy(&b, &v[c], 8);
+                    U[(size_t)(AXX_F_MOM + 6 * j + 2 * c) * J->N + w] = (uint32_t)b;
+                    U[(size_t)(AXX_F_MOM + 6 * j + 2 * c + 1) * J->N + w] = (uint32_t)(b >> 32);
+                }
+            }
+        } else {
+            for (int p = 0; p < Pn; ++p)
+                for (int a = 0; a < 3; ++a) FS(AXX_F_MOM + 3 * p + a)[w] = m[p][a];
+        }
     }
     return NULL;
 }
@@ -901,8 +1135,8 @@ int orc_axes_mask(const axx_acq* q) {
     return widen[mask];
 }
 
-int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
-             int step_end, uint32_t* state, int nthreads, double bin_width) {
+static int walk_common(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+                       int step_end, uint32_t* state, int nthreads, double bin_width, const magjob_t* mag) {
     if (!s || !q || !run || !state) return AXX_ERR_INVALID;
     if (run->n_walkers < 1) return AXX_ERR_INVALID;
     if (step_begin < 0 || step_end > q->n_steps || step_begin > step_end) return AXX_ERR_INVALID;
@@ -926,6 +1160,7 @@ int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int s
         J->P = &P; J->q = q; J->run = run; J->st = state; J->N = run->n_walkers;
         J->Pn = q->n_profiles; J->axes = axes; J->i0 = step_begin; J->i1 = step_end;
         J->use_bins = bin_width > 0;
+        J->mag = mag;
         J->w_begin = run->n_walkers * t / nthreads;
         J->w_end = run->n_walkers * (t + 1) / nthreads;
         if (nthreads > 1) pthread_create(&th[t], NULL, job_run, J);
@@ -938,6 +1173,71 @@ int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int s
     free(jobs); free(th);
     oprep_free(&P);
     return rc;
+}
+
+int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+             int step_end, uint32_t* state, int nthreads, double bin_width) {
+    return walk_common(s, q, run, step_begin, step_end, state, nthreads, bin_width, NULL);
+}
+
+/* Magnetization walk of fields [group*AXX_MAG_GROUP, +F) (updateWalkersMag,
+ * kernelMag.cuh:1-141): the walk above with M per field in place of the
+ * phase moments; state holds AXX_F_MOM + 6 F words per walker. */
+int orc_walk_mag(const axx_substrate* s, const axx_field* f, const axx_run* run, int group, int step_begin,
+                 int step_end, uint32_t* state, int nthreads, double bin_width) {
+    if (!s || !f || !run || !state || !f->samples) return AXX_ERR_INVALID;
+    if (f->n_steps < 1 || f->n_fields < 1 || f->integrator < 0 || f->integrator > 2) return AXX_ERR_INVALID;
+    if (group < 0 || group * AXX_MAG_GROUP >= f->n_fields) return AXX_ERR_INVALID;
+    magjob_t G;
+    G.f = f;
+    G.g0 = group * AXX_MAG_GROUP;
+    G.F = f->n_fields - G.g0 < AXX_MAG_GROUP ? f->n_fields - G.g0 : AXX_MAG_GROUP;
+    int* off = malloc(((size_t)f->n_steps + 1) * sizeof(int));
+    double* iT1 = calloc((size_t)s->n_prims + 1, sizeof(double));
+    double* iT2 = calloc((size_t)s->n_prims + 1, sizeof(double));
+    int64_t tot = 0;
+    for (int i = 0; i < f->n_steps; ++i) {
+        off[i] = (int)tot;
+        tot += mag_samples(f->integrator, f->substeps ? f->substeps[i] : 1);
+    }
+    off[f->n_steps] = (int)tot;
+    for (int k = 0; k <= s->n_prims; ++k) {
+        double t1 = f->T1 ? f->T1[k] : INFINITY;
+        iT1[k] = (t1 > 0 && isfinite(t1)) ? 1.0 / t1 : 0.0;
+        double t2 = k == 0 ? s->T2_lat : s->T2[k - 1];
+        iT2[k] = (t2 > 0 && isfinite(t2)) ? 1.0 / t2 : 0.0;
+    }
+    G.off = off; G.iT1 = iT1; G.iT2 = iT2;
+    G.relax = run->relaxation != 0 && f->integrator != AXX_MAG_ROTATE;
+    /* the walk core sees a one-profile acquisition (its moments unused) */
+    float* prof = calloc((size_t)f->n_steps, sizeof(float));
+    int wfp = 0;
+    double amp = 0.0, dir[3] = {1.0, 0.0, 0.0};
+    axx_acq q;
+    memset(&q, 0, sizeof(q));
+    q.n_steps = f->n_steps; q.dt = f->dt; q.gamma = f->gamma;
+    q.n_profiles = 1; q.profiles = prof;
+    q.n_waveforms = 1; q.wf_profile = &wfp; q.wf_amplitude = &amp; q.wf_dir = dir;
+    int rc = walk_common(s, &q, run, step_begin, step_end, state, nthreads, bin_width, &G);
+    free(off); free(iT1); free(iT2); free(prof);
+    return rc;
+}
+
+/* Sums over walkers of Mx, My, Mz for the F fields of a group's state. */
+void orc_reduce_mag(const uint32_t* state, int64_t n_walkers, int F, double* out) {
+    for (int j = 0; j < F; ++j)
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int64_t w = 0; w < n_walkers; ++w) {
+                uint64_t lo = state[(size_t)(AXX_F_MOM + 6 * j + 2 * c) * n_walkers + w];
+                uint64_t hi = state[(size_t)(AXX_F_MOM + 6 * j + 2 * c + 1) * n_walkers + w];
+                uint64_t b = lo | (hi << 32);
+                double v;
+                memcpy(&v, &b, 8);
+                acc += v;
+            }
+            out[j * 3 + c] = acc;
+        }
 }
 
 /* ------------------------------------------------------------------ */

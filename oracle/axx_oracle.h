@@ -23,6 +23,14 @@ int   orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int
                int step_end, uint32_t* state, int nthreads, double bin_width);
 int   orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
                  const uint32_t* state, axx_result* out);
+/* Magnetization walk of one group of fields (state: AXX_F_MOM + 6 F words
+ * per walker) and its sums; the double-precision sine/cosine and square
+ * root it uses, elementwise. */
+int   orc_walk_mag(const axx_substrate* s, const axx_field* f, const axx_run* run, int group,
+                   int step_begin, int step_end, uint32_t* state, int nthreads, double bin_width);
+void  orc_reduce_mag(const uint32_t* state, int64_t n_walkers, int F, double* out);
+void  orc_sincos64(const double* x, double* s, double* c, int64_t n);
+void  orc_sqrt_cr(const double* x, double* y, int64_t n);
 int   orc_image_count(const axx_substrate* s, const axx_acq* q);
 int   orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
                    const float u3[3], float ell, float* t);

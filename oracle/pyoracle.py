@@ -35,6 +35,11 @@ def lib():
         L.orc_unif.restype = C.c_float
         L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_neg2log_unif.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_sincos64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_sqrt_cr.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_walk_mag.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Field), C.POINTER(abi.Run),
+                                   C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_double]
+        L.orc_reduce_mag.argtypes = [C.POINTER(C.c_uint32), C.c_int64, C.c_int, C.c_void_p]
         L.orc_validate.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
         L.orc_walk.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                                C.c_int, C.c_int, u32p, C.c_int, C.c_double]
@@ -82,6 +87,62 @@ def neg2log_unif(w) -> np.ndarray:
     out = np.empty(w.shape, np.float32)
     lib().orc_neg2log_unif(w.ctypes.data, out.ctypes.data, w.size)
     return out
+
+
+def sincos64(x):
+    """Double-precision sine and cosine of the Bloch path, elementwise."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, c = np.empty_like(x), np.empty_like(x)
+    lib().orc_sincos64(x.ctypes.data, s.ctypes.data, c.ctypes.data, x.size)
+    return s, c
+
+
+def sqrt_cr(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().orc_sqrt_cr(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def walk_mag(sub, field, run, group=0, step_begin=0, step_end=None, state=None, threads=None,
+             bin_width=0.0) -> np.ndarray:
+    """Magnetization walk of one group of fields; returns the group's state."""
+    if step_end is None:
+        step_end = field.n_steps
+    F = min(abi.MAG_GROUP, field.n_fields - group * abi.MAG_GROUP)
+    if state is None:
+        state = np.zeros(abi.mag_state_fields(F) * run.n_walkers, np.uint32)
+    s, f, r = sub.struct(), field.struct(), run.struct()
+    if threads is None:
+        threads = min(os.cpu_count() or 1, 16)
+    rc = lib().orc_walk_mag(C.byref(s), C.byref(f), C.byref(r), int(group), int(step_begin), int(step_end),
+                            state.ctypes.data_as(C.POINTER(C.c_uint32)), int(threads), float(bin_width))
+    _check(rc, "orc_walk_mag")
+    return state
+
+
+def reduce_mag(state, n_walkers: int, F: int) -> np.ndarray:
+    """[F, 3] sums of Mx, My, Mz over walkers."""
+    out = np.zeros(F * 3)
+    lib().orc_reduce_mag(state.ctypes.data_as(C.POINTER(C.c_uint32)), int(n_walkers), int(F), out.ctypes.data)
+    return out.reshape(F, 3)
+
+
+def run_mag(sub, field, run, threads=None) -> abi.MagSignals:
+    """Every group walked and summed (the oracle of axx_run_mag)."""
+    n_groups = (field.n_fields + abi.MAG_GROUP - 1) // abi.MAG_GROUP
+    sums, coll = [], (0, 0, 0)
+    for g in range(n_groups):
+        F = min(abi.MAG_GROUP, field.n_fields - g * abi.MAG_GROUP)
+        st = walk_mag(sub, field, run, group=g, threads=threads)
+        sums.append(reduce_mag(st, run.n_walkers, F))
+        if g == 0:
+            v = st.reshape(abi.mag_state_fields(F), run.n_walkers)
+            coll = (int(v[abi.F_NCOLL].sum(dtype=np.uint64)), int(v[abi.F_NTRAN].sum(dtype=np.uint64)),
+                    int(v[abi.F_NCAP].sum(dtype=np.uint64)))
+    s = np.concatenate(sums)
+    return abi.MagSignals(mx=s[:, 0].copy(), my=s[:, 1].copy(), mz=s[:, 2].copy(), n_walkers=run.n_walkers,
+                          collisions=coll[0], transmissions=coll[1], cap_hits=coll[2])
 
 
 def validate(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:

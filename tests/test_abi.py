@@ -25,7 +25,8 @@ def declared_functions():
 def test_header_declares_the_boundary():
     names = declared_functions()
     for n in ("axx_create", "axx_destroy", "axx_run_phase", "axx_plan_create", "axx_plan_walk",
-              "axx_plan_reduce", "axx_plan_results", "axx_plan_get_state", "axx_plan_set_state"):
+              "axx_plan_reduce", "axx_plan_results", "axx_plan_get_state", "axx_plan_set_state",
+              "axx_run_mag", "axx_mag_create", "axx_mag_walk", "axx_mag_results", "axx_mag_get_state"):
         assert n in names
 
 
@@ -56,8 +57,9 @@ def test_ctypes_structs_match_header_layout():
 #include <stdio.h>
 #include "axonexa.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(axx_substrate), sizeof(axx_acq), sizeof(axx_run),
-         sizeof(axx_result), sizeof(axx_tuning), offsetof(axx_run, bounce_cap));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(axx_substrate), sizeof(axx_acq), sizeof(axx_run),
+         sizeof(axx_result), sizeof(axx_tuning), offsetof(axx_run, bounce_cap), sizeof(axx_field),
+         offsetof(axx_field, origin), offsetof(axx_field, T1), sizeof(axx_mag_result));
   return 0;
 }
 '''
@@ -69,7 +71,8 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), cf, "-o", exe], check=True)
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     want = [C.sizeof(abi.Substrate), C.sizeof(abi.Acq), C.sizeof(abi.Run), C.sizeof(abi.Result),
-            C.sizeof(abi.Tuning), abi.Run.bounce_cap.offset]
+            C.sizeof(abi.Tuning), abi.Run.bounce_cap.offset, C.sizeof(abi.Field), abi.Field.origin.offset,
+            abi.Field.T1.offset, C.sizeof(abi.MagResult)]
     assert got == want
 
 

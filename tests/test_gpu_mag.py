@@ -1,0 +1,146 @@
+"""Magnetization (Bloch) path on the GPU (axx_mag_*, axx_run_mag) against the
+CPU oracle (orc_walk_mag): every walker's position, counters and fp64
+magnetization bitwise, sums to rounding, for each integrator, both kernel
+geometries (cylinders only / spheres), all boundary policies, step
+generators, several field groups, variable substeps and chunked walks; and
+the B0 = 0 run against the phase path of the same walkers."""
+import numpy as np
+import pytest
+
+from axonexa_amd import abi
+from axonexa_amd import fields as fl
+from axonexa_amd import substrates as sb
+from axonexa_amd import waveforms as wf
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from axonexa_amd import engine
+    return engine
+
+
+def _check(label, sub, field, run, chunks=None):
+    eng = _engine()
+    print(f"[mag] {label}: N={run.n_walkers} steps={field.n_steps} fields={field.n_fields} "
+          f"integ={field.integrator}", flush=True)
+    with eng.Mag(sub, field, run) as m:
+        if chunks:
+            b = 0
+            for e in chunks + [field.n_steps]:
+                m.walk(b, e)
+                b = e
+        else:
+            m.walk()
+        got = m.results()
+        for g in range(m.n_groups):
+            gs = m.get_state(g)
+            os_ = po.walk_mag(sub, field, run, group=g)
+            same = np.array_equal(gs, os_)
+            if not same:
+                F = m.group_fields(g)
+                a = gs.reshape(abi.mag_state_fields(F), run.n_walkers)
+                b_ = os_.reshape(abi.mag_state_fields(F), run.n_walkers)
+                bad = np.nonzero((a != b_).any(axis=0))[0]
+                rows = np.nonzero((a != b_).any(axis=1))[0]
+                raise AssertionError(f"{label} group {g}: {bad.size} walkers differ, fields {rows[:12]}")
+    want = po.run_mag(sub, field, run)
+    n = run.n_walkers
+    for k in ("mx", "my", "mz"):
+        assert np.allclose(getattr(got, k), getattr(want, k), rtol=1e-12, atol=1e-12 * n), (label, k)
+    assert (got.collisions, got.transmissions, got.cap_hits) == (want.collisions, want.transmissions,
+                                                                 want.cap_hits), label
+    print(f"[mag] {label}: bitwise, collisions {got.collisions} transmissions {got.transmissions}", flush=True)
+    return got
+
+
+def test_rotation_hex_lattice_saddle_coil_fields():
+    sub = sb.hex_lattice(0.0025, 0.006, 2, 1, D_in=1e-6, D_ex=2.5e-6, kappa=0.0)
+    fs = [fl.QSC_GRE(5e-4, g * 2e-4, 0.01, 0.004) for g in range(3)]
+    f = fl.build_field(fs, 240, 1e-4, origin=(0.003, 0.003, 0.0))
+    run = abi.RunSpec(n_walkers=3000, seed=5)
+    got = _check("rotate_hex", sub, f, run, chunks=[1, 77])
+    assert got.collisions > 100
+
+
+def test_rk4_relaxation_permeable_spheres():
+    dt = 1e-4
+    sub = sb.single_sphere(0.007, 0.0025, D_in=1e-6, D_ex=2e-6, T2_in=60.0, T2_ex=80.0,
+                           kappa=sb.kappa_for_probability(0.05, 1e-6, dt))
+    fs = [fl.COS_GRE(2e-4, (g * 3e-4, 0.0, g * 1e-4), 0.01, 0.003, 2) for g in range(2)]
+    f = fl.build_field(fs, 230, dt, integrator=abi.MAG_RK4, substeps=np.full(230, 2, np.int32),
+                       origin=(0.0035, 0.0035, 0.0035), T1=np.array([100.0, 70.0]))
+    run = abi.RunSpec(n_walkers=2500, seed=6, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    got = _check("rk4_permeable_spheres", sub, f, run, chunks=[100])
+    assert got.transmissions > 0
+
+
+@pytest.mark.parametrize("smode", [abi.STEP_ISOTROPIC, abi.STEP_GAUSSIAN])
+def test_dopri_single_pore_rejection_rf(smode):
+    sub = sb.single_sphere(0.016, 0.0025, D_in=2.5e-6, D_ex=2.5e-6)
+    fs = [fl.UCN_lowfield(5e-4, 3e-5, 0.006, 2e-5, 1e-4), fl.QSC_GRE(5e-4, 6e-6, 0.004, 0.002)]
+    f = fl.build_field(fs, 120, 1e-4, integrator=abi.MAG_DOPRI, substeps="auto",
+                       origin=(0.008, 0.008, 0.008), m0=(0.0, 0.0, 1.0))
+    assert f.substeps.max() == 5 and f.substeps.min() == 1
+    run = abi.RunSpec(n_walkers=2048, seed=7, boundary_mode=abi.BOUNDARY_REJECTION,
+                      init_mode=abi.INIT_INSIDE_PRIM, step_mode=smode)
+    _check(f"dopri_pore_{smode}", sub, f, run)
+
+
+def test_eleven_fields_two_groups_variable_substeps():
+    sub = sb.single_sphere(0.01, 0.0025, D_in=2.5e-6, D_ex=2.5e-6)
+    fs = [fl.QSC_GRE(5e-4, g * 4e-4, 0.02, 0.01) for g in range(11)]
+    n = 500
+    sub_n = np.where(np.arange(n) % 7 == 0, 3, 1).astype(np.int32)
+    f = fl.build_field(fs, n, 1e-4, substeps=sub_n, origin=(0.005, 0.005, 0.005))
+    run = abi.RunSpec(n_walkers=1500, seed=8, init_mode=abi.INIT_INSIDE_PRIM)
+    got = _check("eleven_fields", sub, f, run)
+    assert got.transverse[10] < got.transverse[0]
+
+
+def test_zero_static_field_matches_phase_path():
+    """B = (0, 0, G x g(t)): the magnetization sums are the phase path's
+    (Mx = sum cos, My = -sum sin) for the same walkers."""
+    eng = _engine()
+    dt = 2.0 ** -10
+    d, s = 16 * dt, 8 * dt
+    n = 48
+    sub = sb.hex_lattice(0.03, 0.07, 2, 1, D_in=2e-4, D_ex=4e-4)
+    run = abi.RunSpec(n_walkers=20000, seed=9)
+    Gx = 2e4
+    f = fl.build_field([fl.RECT_GRE(0.0, (Gx, 0, 0), d, s)], n, dt, gamma=1.0)
+    t = np.arange(n) * dt
+    prof = np.where(t < d, 1.0, np.where((t >= s + d) & (t < 2 * d + s), -1.0, 0.0)).astype(np.float32)
+    acq = abi.AcqSpec(n_steps=n, dt=dt, gamma=1.0, profiles=prof, wf_profile=[0], wf_amplitude=[Gx],
+                      wf_dir=[[1.0, 0.0, 0.0]])
+    mag = eng.run_mag(sub, f, run)
+    ph = eng.run_phase(sub, acq, run)
+    print(f"[mag] phase path {ph.sum_cos[0]:.4f} {ph.sum_sin[0]:.4f} mag {mag.mx[0]:.4f} {mag.my[0]:.4f}",
+          flush=True)
+    assert abs(mag.mx[0] - ph.sum_cos[0]) < 1e-4 * run.n_walkers
+    assert abs(mag.my[0] + ph.sum_sin[0]) < 1e-4 * run.n_walkers
+    assert ph.E[0] < 0.9
+
+
+def test_invalid_field_descriptors_are_errors():
+    eng = _engine()
+    sub = sb.single_sphere(0.01, 0.0025, D_in=2.5e-6, D_ex=2.5e-6)
+    run = abi.RunSpec(n_walkers=64, seed=1, init_mode=abi.INIT_INSIDE_PRIM)
+    good = fl.build_field([fl.QSC_GRE(5e-4, 1e-4, 0.002, 0.001)], 60, 1e-4)
+    bad = []
+    f = fl.build_field([fl.QSC_GRE(5e-4, 1e-4, 0.002, 0.001)], 60, 1e-4)
+    f.samples[3, 0, 5] = np.nan
+    bad.append(f)
+    f = fl.build_field([fl.QSC_GRE(5e-4, 1e-4, 0.002, 0.001)], 60, 1e-4, substeps=np.ones(60, np.int32))
+    f.substeps[7] = 0
+    bad.append(f)
+    f = fl.build_field([fl.QSC_GRE(5e-4, 1e-4, 0.002, 0.001)], 60, 1e-4, integrator=abi.MAG_RK4,
+                       T1=np.array([-1.0, 5.0]))
+    bad.append(f)
+    for k, f in enumerate(bad):
+        with pytest.raises(eng.AxxError) as e:
+            eng.run_mag(sub, f, run)
+        assert e.value.code == -1, k
+    m = eng.run_mag(sub, good, run)
+    assert abs(np.hypot(m.mx[0], m.my[0]) / run.n_walkers - 1.0) < 0.05
