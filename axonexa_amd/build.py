@@ -65,7 +65,7 @@ def build_lib(verbose: bool = False) -> str:
     return lib
 
 
-DRIVERS = ["cos_onecyl", "gammadist", "wccompare"]
+DRIVERS = ["cos_onecyl", "gammadist", "wccompare", "maglf"]
 
 
 def build_drivers() -> list:

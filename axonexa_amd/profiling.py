@@ -63,9 +63,22 @@ def _kernels(con):
     return rows
 
 
-def summarize(db: str, kernel: str = "") -> str:
+def summarize(db: str, kernel: str = "", durations: bool = False) -> str:
     con = sqlite3.connect(db)
     out = []
+    if durations:
+        # every dispatch of the matching kernels in launch order (short
+        # placement dispatches and full walks are told apart here)
+        try:
+            rows = con.execute("select name, duration from kernels order by start").fetchall()
+            out.append(f"{'kernel':<60} {'ms':>12}")
+            for name, dur in rows:
+                if kernel and kernel not in name:
+                    continue
+                out.append(f"{name[:60]:<60} {dur / 1e6:>12.4f}")
+            out.append("")
+        except sqlite3.Error as e:
+            out.append(f"(no kernel table: {e})")
     try:
         rows = con.execute("select name, count(*), avg(duration) from kernels group by name").fetchall()
         out.append(f"{'kernel':<60} {'dispatches':>10} {'avg_ms':>12}")
@@ -114,13 +127,14 @@ def main(argv=None):
     s = sub.add_parser("summarize")
     s.add_argument("db")
     s.add_argument("--kernel", default="")
+    s.add_argument("--durations", action="store_true", help="also list every dispatch's duration")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy(args.mib)
     elif args.cmd == "mlp":
         mlp(args.batch, args.train_steps)
     else:
-        print(summarize(args.db, args.kernel))
+        print(summarize(args.db, args.kernel, args.durations))
 
 
 if __name__ == "__main__":

@@ -215,3 +215,49 @@ def c4_batch(seed: int = 4, n_inst: int = 256, scale: float = 1.0, n_steps: int 
             name = f"gamma theta={theta:.2f} f={frac:.2f} p={p:.1e}"
         out.append(Workload(f"c4_{k}", sub, acq, run, name))
     return out
+
+
+# ------------------------------------------------------------ Bloch path
+# fp64 flops of one rotation of one field (field value 18, |B|^2 5, angle 1,
+# sine/cosine 45, projection 7, sine / |B| 1, C 3, B x M 9, update 15;
+# FMA = 2) plus one square root and one reciprocal.
+MAG_ROTATE_FLOPS = 104.0
+
+
+@dataclass
+class MagWorkload:
+    name: str
+    sub: abi.SubstrateSpec
+    field: abi.FieldSpec
+    run: abi.RunSpec
+    description: str
+
+    @property
+    def walker_steps(self) -> int:
+        return int(self.run.n_walkers) * (self.field.n_steps - 1)
+
+    @property
+    def field_updates(self) -> int:
+        """Field updates of one walk: every walker, every field, t_0 .. t_{n-1}."""
+        return int(self.run.n_walkers) * self.field.n_steps * self.field.n_fields
+
+
+def mag_lf(n_walkers: int = 917504, n_fields: int = 2, integrator: int = abi.MAG_ROTATE) -> MagWorkload:
+    """The longest acquisition of examples/magLF_spherNeuman.cu (mm, ms, T):
+    sphere R = 2.5 um, D = 2.5e-6 mm^2/ms, dt = 1e-4 ms, walkers start
+    inside with M = x; QSC_GRE fields with B0 = 0.5 mT, lobes 0.05 ms,
+    spacing 0.01 ms (1100 steps), gradients 0 .. 6e-6 T/mm over n_fields
+    fields (the reference's two: 0 and 6e-6).  Walkers: 8 x the reference's
+    114,688 per acquisition."""
+    from . import fields as fl
+    R, D, dt = 0.0025, 2.5e-6, 1e-4
+    L = 6.0 * R
+    sub = sb.single_sphere(L, R, D_in=D, D_ex=D)
+    d, s = 0.05, 0.01
+    n = int((2.0 * d + s) / dt)
+    fs = [fl.QSC_GRE(5e-4, 6e-6 * g / max(n_fields - 1, 1), d, s) for g in range(n_fields)]
+    f = fl.build_field(fs, n, dt, integrator=integrator, origin=(L / 2,) * 3, m0=(1.0, 0.0, 0.0))
+    run = abi.RunSpec(n_walkers=n_walkers, seed=11, init_mode=abi.INIT_INSIDE_PRIM)
+    return MagWorkload("mag_lf", sub, f, run,
+                       f"low-field QSC_GRE echo in a 2.5 um sphere, {n_fields} fields, {n} steps of 0.1 us, "
+                       f"{n_walkers} walkers, integrator {integrator}")

@@ -69,7 +69,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2_hex",
                     choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder", "c4_batch",
-                             "c5_surrogate"])
+                             "c5_surrogate", "mag_lf"])
+    ap.add_argument("--mag-fields", type=int, default=2, help="mag_lf: fields per walker")
+    ap.add_argument("--mag-integrator", type=int, default=0, help="mag_lf: 0 rotate, 1 RK4, 2 DOPRI")
     ap.add_argument("--sur-geoms", type=int, default=4096,
                     help="c5_surrogate: training geometries (8 queries each)")
     ap.add_argument("--sur-steps", type=int, default=4000, help="c5_surrogate: Adam steps")
@@ -95,6 +97,8 @@ def main(argv=None) -> int:
         return bench_batch(args)
     if args.workload == "c5_surrogate":
         return bench_surrogate(args)
+    if args.workload == "mag_lf":
+        return bench_mag(args)
 
     import numpy as np
 
@@ -221,6 +225,94 @@ def main(argv=None) -> int:
         }
     print(json.dumps(out))
     plan.close()
+    return 0
+
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64 (spec sheet)
+
+
+def bench_mag(args) -> int:
+    """Magnetization (Bloch) path: field updates/s of the fp64 rotation walk
+    on the low-field saddle-coil workload, with the CPU oracle beside it."""
+    import numpy as np
+
+    from axonexa_amd import abi, engine
+    from axonexa_amd import workloads as W
+
+    kw = {"n_fields": args.mag_fields, "integrator": args.mag_integrator}
+    if args.walkers:
+        kw["n_walkers"] = args.walkers
+    wl = W.mag_lf(**kw)
+    n = wl.field.n_steps
+    m = engine.Mag(wl.sub, wl.field, wl.run)
+    for _ in range(args.warmup):
+        m.walk(0, n)
+        m.results()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(m.time_walk(0, n))
+        res = m.results()
+    elapsed = time.perf_counter() - t0
+    value = wl.field_updates * args.steps / elapsed
+    avg_ms = float(np.mean(kernel_ms))
+    # the timed walk covers steps 1 .. n-1 (placement and t = 0 are outside)
+    upd = wl.run.n_walkers * (n - 1) * wl.field.n_fields
+    achieved = W.MAG_ROTATE_FLOPS * upd / (avg_ms * 1e-3) / 1e12 if args.mag_integrator == 0 else None
+    out = {
+        "metric": "field-updates/s",
+        "value": value,
+        "unit": "field-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl.name + ": " + wl.description, "walkers": wl.run.n_walkers,
+                   "time_steps": n, "fields": wl.field.n_fields, "integrator": args.mag_integrator},
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS if achieved else None,
+            "traffic": None,
+            "flops_per_field_update": W.MAG_ROTATE_FLOPS if args.mag_integrator == 0 else None,
+            "field_updates_per_launch": upd,
+            "kernel_ms": avg_ms,
+            "walker_steps_per_s_kernel": wl.run.n_walkers * (n - 1) / (avg_ms * 1e-3),
+        },
+        "signal": {"transverse": [float(v) for v in res.transverse], "collisions": res.collisions},
+    }
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        nw = min(args.cpu_walkers, wl.run.n_walkers) // 4
+        sample = W.mag_lf(**dict(kw, n_walkers=nw))
+        threads = _cpu_threads()
+        tc = time.perf_counter()
+        ost = po.walk_mag(sample.sub, sample.field, sample.run, threads=threads)
+        tcpu = time.perf_counter() - tc
+        with engine.Mag(sample.sub, sample.field, sample.run) as sm:
+            sm.walk()
+            gst = sm.get_state(0)
+        bit = np.all(gst.reshape(-1, nw) == ost.reshape(-1, nw), axis=0)
+        out["cpu_baseline"] = {
+            "value": sample.field_updates / tcpu,
+            "unit": "field-updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{nw} walkers x {n} steps x {sample.field.n_fields} fields of the same workload, "
+                      f"{threads} threads on {_cpu_model()}",
+            "seconds": tcpu,
+        }
+        out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+        out["sample_bitwise_equal_walkers"] = float(bit.mean())
+    print(json.dumps(out))
+    m.close()
     return 0
 
 

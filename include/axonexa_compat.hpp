@@ -33,6 +33,8 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "axonexa.h"
@@ -769,6 +771,404 @@ class CPUtimer {
 
   private:
     std::chrono::steady_clock::time_point t0{}, t1{};
+};
+
+// ------------------------------------------------- magnetic-field functors
+// bfunctors.cuh:5-476.  Each field is affine in the position:
+// B(r, t) = b0(t) + J(t) r; affine(t, b0, J) gives the decomposition (J row
+// major) that the engine samples, operator() the field itself.
+namespace axx_compat {
+struct AffineField {
+    virtual ~AffineField() {}
+    virtual void affine(real t, double b0[3], double J[9]) const = 0;
+    Vector3 operator()(Vector3 r, real t) const {
+        double b0[3], J[9];
+        affine(t, b0, J);
+        return Vector3(b0[0] + J[0] * r.x + J[1] * r.y + J[2] * r.z, b0[1] + J[3] * r.x + J[4] * r.y + J[5] * r.z,
+                       b0[2] + J[6] * r.x + J[7] * r.y + J[8] * r.z);
+    }
+    static void clear(double b0[3], double J[9]) {
+        for (int k = 0; k < 3; ++k) b0[k] = 0;
+        for (int k = 0; k < 9; ++k) J[k] = 0;
+    }
+};
+}  // namespace axx_compat
+
+// saddle coil, gradient echo: (0, G z, B0 + G y), negative gradient in the second lobe
+class QSC_GRE : public axx_compat::AffineField {
+  public:
+    real B0 = 0, G = 0, gradientDuration = 0, gradientSpacing = 0;
+    QSC_GRE() {}
+    QSC_GRE(real b0, real g, real d, real s) : B0(b0), G(g), gradientDuration(d), gradientSpacing(s) {}
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        b0[2] = B0;
+        const real d = gradientDuration, s = gradientSpacing;
+        const real g = t < d ? G : ((t >= s + d && t < 2.0 * d + s) ? -G : 0.0);
+        J[5] = g;   // By = g z
+        J[7] = g;   // Bz = B0 + g y
+    }
+    real getB() const {
+        return G * G * gradientDuration * gradientDuration * GAMMA * GAMMA *
+               ((2.0 / 3.0) * gradientDuration + gradientSpacing);
+    }
+    real getDiffusionTime() const { return gradientSpacing + gradientDuration; }
+};
+
+// saddle-coil PGSE with a 180-degree RF pulse of rfDuration at TE/2
+class QSC_PGSE : public axx_compat::AffineField {
+  public:
+    real B0 = 0, G = 0, gradientDuration = 0, gradientSpacing = 0, rfDuration = 0, dtmin = 0, dtmax = 0, TE = 0,
+         B1 = 0;
+    QSC_PGSE() {}
+    QSC_PGSE(real b0, real g, real d, real s, real rf, real dmin, real dmax, real te)
+        : B0(b0), G(g), gradientDuration(d), gradientSpacing(s), rfDuration(rf), dtmin(dmin), dtmax(dmax), TE(te) {
+        B1 = PI / std::fabs(GAMMA * rfDuration);
+    }
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        b0[2] = B0;
+        const real d = gradientDuration, s = gradientSpacing;
+        if (t < d) {
+            J[5] = G; J[7] = G;
+        } else if (t >= TE / 2.0 && t < TE / 2.0 + rfDuration) {
+            const real w = GAMMA * B0;
+            b0[0] = B1 * std::cos(w * t);
+            b0[1] = -B1 * std::sin(w * t);
+        } else if (t >= s + d && t < 2.0 * d + s) {
+            J[5] = G; J[7] = G;
+        }
+    }
+    real getStepTimeMin(real t) const { return (t >= TE / 2.0 && t < TE / 2.0 + rfDuration) ? dtmin : dtmax; }
+    real getB() const {
+        return G * G * gradientDuration * gradientDuration * GAMMA * GAMMA *
+               ((2.0 / 3.0) * gradientDuration + gradientSpacing);
+    }
+    real getDiffusionTime() const { return gradientSpacing + gradientDuration; }
+};
+
+// saddle coil, cosine-modulated gradient echo
+class QSC_COS_GRE : public axx_compat::AffineField {
+  public:
+    real B0 = 0, G = 0, gradientDuration = 0, gradientSpacing = 0, period = 1, b = 0;
+    QSC_COS_GRE() {}
+    QSC_COS_GRE(real b0, real g, real d, real s, real n) : B0(b0), G(g), gradientDuration(d), gradientSpacing(s) {
+        b = GAMMA * GAMMA * 0.25 * G * G * d * d * d / (PI * PI * n * n);
+        period = d / n;
+    }
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        b0[2] = B0;
+        const real d = gradientDuration, s = gradientSpacing;
+        real g = 0;
+        if (t < d) g = G * std::cos(2.0 * PI * t / period);
+        else if (t >= s + d && t < 2.0 * d + s) g = -G * std::cos(2.0 * PI * (t - s - d) / period);
+        J[5] = g; J[7] = g;
+    }
+    real getB() const { return b; }
+    real getFreq() const { return 1.0 / period; }
+};
+
+// saddle coil with trapezoidal lobes, as the reference defines them
+// (ramps scale by (G/ramp) t, ramp-down over [ramp + d, 3 ramp + d), second
+// lobe of the same sign)
+class QSC_GRE_TRAP : public axx_compat::AffineField {
+  public:
+    real B0 = 0, G = 0, gradientDuration = 0, gradientSpacing = 0, gradientRampTime = 1;
+    QSC_GRE_TRAP() {}
+    QSC_GRE_TRAP(real b0, real g, real d, real s, real ramp)
+        : B0(b0), G(g), gradientDuration(d), gradientSpacing(s), gradientRampTime(ramp) {}
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        b0[2] = B0;
+        const real r = gradientRampTime, t1 = r, t2 = r + gradientDuration, t3 = 2.0 * t1 + t2,
+                   t5 = gradientSpacing + t3;
+        auto up = [&](real u) { return (G / r) * u; };
+        auto down = [&](real u) { return G - (G / r) * u; };
+        real g = 0;
+        if (t < t1) g = G * up(t);
+        else if (t >= t1 && t < t2) g = G;
+        else if (t >= t2 && t < t3) g = G * down(t - t2);
+        else if (t >= t5 && t < t5 + t1) g = G * up(t - t5);
+        else if (t >= t5 + t1 && t < t5 + t2) g = G;
+        else if (t >= t5 + t2 && t < t5 + t3) g = G * down(t - t5 - t2);
+        J[5] = g; J[7] = g;
+    }
+};
+
+// B = (0, 0, B0 + m(t) G.r) in the lobes (SIN_GRE, COS_GRE, RECT_GRE)
+namespace axx_compat {
+class LongitudinalLobes : public AffineField {
+  public:
+    real B0 = 0, gradientDuration = 0, gradientSpacing = 0;
+    Vector3 G;
+    virtual real mod1(real) const { return 1.0; }
+    virtual real mod2(real) const { return 1.0; }
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        b0[2] = B0;
+        const real d = gradientDuration, s = gradientSpacing;
+        real m = 0;
+        if (t < d) m = mod1(t);
+        else if (t >= s + d && t < 2.0 * d + s) m = -mod2(t);
+        J[6] = m * G.x; J[7] = m * G.y; J[8] = m * G.z;
+    }
+};
+}  // namespace axx_compat
+
+class SIN_GRE : public axx_compat::LongitudinalLobes {
+  public:
+    real period = 1;
+    SIN_GRE() {}
+    SIN_GRE(real b0, Vector3 g, real d, real s, real T) {
+        B0 = b0; G = g; gradientDuration = d; gradientSpacing = s; period = T;
+    }
+    real mod1(real t) const override { return std::sin(2.0 * PI * t / period); }
+    real mod2(real t) const override {
+        return std::sin(2.0 * PI * (t - gradientSpacing - gradientDuration) / period);
+    }
+};
+
+class COS_GRE : public axx_compat::LongitudinalLobes {
+  public:
+    real period = 1, b = 0;
+    COS_GRE() {}
+    COS_GRE(real b0, Vector3 g, real d, real s, int n) {
+        B0 = b0; G = g; gradientDuration = d; gradientSpacing = s;
+        period = d / (real)n;
+        b = GAMMA * GAMMA * 0.25 * g.magnitude() * g.magnitude() * d * d * d / (PI * PI * n * n);
+    }
+    COS_GRE(real b0, real g, Vector3 ghat, real d, real s, int n) : COS_GRE(b0, ghat * g, d, s, n) {
+        b = GAMMA * GAMMA * 0.25 * g * g * d * d * d / (PI * PI * n * n);
+    }
+    real mod1(real t) const override { return std::cos(2.0 * PI * t / period); }
+    real mod2(real t) const override {
+        return std::cos(2.0 * PI * (t - gradientSpacing - gradientDuration) / period);
+    }
+    real getB() const { return b; }
+    real getFreq() const { return 1.0 / period; }
+};
+
+class RECT_GRE : public axx_compat::LongitudinalLobes {
+  public:
+    RECT_GRE() {}
+    RECT_GRE(real b0, Vector3 g, real d, real s) {
+        B0 = b0; G = g; gradientDuration = d; gradientSpacing = s;
+    }
+};
+
+// low-field cell with a pi/2 RF pulse over [0, rfDuration)
+class UCN_lowfield : public axx_compat::AffineField {
+  public:
+    real B0 = 0, G = 0, rfDuration = 0, dtmin = 0, dtmax = 0, w = 0, B1 = 0;
+    UCN_lowfield() {}
+    UCN_lowfield(real b0, real g, real rf, real dmin, real dmax)
+        : B0(b0), G(g), rfDuration(rf), dtmin(dmin), dtmax(dmax) {
+        B1 = PI / (2.0 * std::fabs(GAMMA * rfDuration));
+        w = GAMMA * B0;
+    }
+    void affine(real t, double b0[3], double J[9]) const override {
+        clear(b0, J);
+        if (t < rfDuration) {
+            b0[0] = B1 * std::cos(w * t);
+            b0[1] = -B1 * std::sin(w * t);
+        }
+        b0[2] = B0;
+        J[0] = -0.5 * G; J[4] = -0.5 * G; J[8] = G;
+    }
+    real getStepTimeMin(real t) const { return t <= rfDuration ? dtmin : dtmax; }
+};
+
+// --------------------------------------------------- magnetization acquisitions
+// magAcquisition<F> (magAcquisition.cuh:1-79): the fields of one acquisition
+// and the summed magnetization per field.
+template <class F>
+class magAcquisition {
+  public:
+    int numOfMeasurements = 0, numOfSteps = 0, lastAlloc = 0, seed = 0;
+    std::vector<real> mx, my, mz;
+    std::vector<F> fieldFunctors;
+    real ADC = 0, ADCmeansquare = 0;
+    int points = 0;
+    magAcquisition(int nmeas, int nsteps, int s)
+        : numOfMeasurements(nmeas), numOfSteps(nsteps), seed(s), mx(nmeas, 0.0), my(nmeas, 0.0), mz(nmeas, 0.0),
+          fieldFunctors(nmeas) {}
+    void addMeasurement(F& f) { fieldFunctors[lastAlloc++] = f; }
+    std::vector<F>& getFieldFunctors() { return fieldFunctors; }
+    std::vector<real>& getMx() { return mx; }
+    std::vector<real>& getMy() { return my; }
+    std::vector<real>& getMz() { return mz; }
+    int getNumOfMeas() { return numOfMeasurements; }
+    int getNumOfSteps() { return numOfSteps; }
+    int& getSeed() { return seed; }
+    void calcADC() {
+        points++;
+        std::vector<real> ln, b;
+        for (int i = 0; i < numOfMeasurements; ++i) {
+            b.push_back(fieldFunctors[i].getB());
+            ln.push_back(std::log(std::sqrt(mx[i] * mx[i] + my[i] * my[i])));
+        }
+        const real a = -compat_linear_regression(ln, b);
+        ADC = (ADC * (points - 1) + a) / points;
+        ADCmeansquare = (ADCmeansquare * (points - 1) + a * a) / points;
+    }
+    real getADC() { return ADC; }
+    real getADCError() { return std::sqrt(std::max<real>(ADCmeansquare - ADC * ADC, 0)); }
+};
+
+namespace axx_compat {
+template <class F, class = void>
+struct has_step_time_min : std::false_type {};
+template <class F>
+struct has_step_time_min<F, decltype((void)std::declval<const F&>().getStepTimeMin(real(0)))> : std::true_type {};
+
+template <class F>
+int substeps_at(const std::vector<F>& fs, real t, real dt, std::true_type) {
+    int s = 1;
+    for (auto& f : fs) s = std::max(s, (int)std::ceil(dt / f.getStepTimeMin(t) - 1e-9));
+    return std::min(s, 4096);
+}
+template <class F>
+int substeps_at(const std::vector<F>&, real, real, std::false_type) { return 1; }
+}  // namespace axx_compat
+
+// magAcquisitionStream<F> (magAcquisitionStream.cuh:1-342).  The reference
+// walks inside one basis primitive; here the pore sits in a periodic cell
+// six radii wide (as the phase path's single-pore runs) and the fields are
+// evaluated at the walker's position relative to the pore's own centre.
+// Integrator macros as in the reference (kernelMag.cuh:29-68, 93-136):
+// RK4_RELAXATION / USE_RELAXATION (RK4 with T1, T2 of the pore),
+// RK4_NORELAXATION, DORP_NORELAXATION (Dormand-Prince, substeps from
+// getStepTimeMin), ROTATE_MULTISTEP (rotations, substeps from
+// getStepTimeMin); none: one exact rotation per step.
+template <class F>
+class magAcquisitionStream {
+  public:
+    std::vector<magAcquisition<F>> acqSet;
+    int number_of_particles = 0;
+    explicit magAcquisitionStream(int num) : number_of_particles(num) {}
+    void addAcquisition(magAcquisition<F> a) { acqSet.push_back(a); }
+    magAcquisition<F>& getAcquisition(int num) { return acqSet[num]; }
+
+    template <class B>
+    void runAcquisition(int num, B& basis, Vector3& initialM, real timestep, int blocks, int threads, int devNum,
+                        int numOfSM) {
+        (void)blocks; (void)threads; (void)devNum; (void)numOfSM;
+        run(num, basis, initialM, timestep);
+    }
+    // The reference's serial CPU loop (updateWalkersMagCPU); the same walk runs
+    // on the engine here.
+    template <class B>
+    void runCPUAcquisition(int num, B& basis, Vector3& initialM, real timestep) {
+        run(num, basis, initialM, timestep);
+    }
+    template <class B>
+    void runAcquisitionStream(B& basis, Vector3& initialM, real timestep, int blocks, int threads, int numOfDevices,
+                              std::vector<int>& plan, std::vector<int>& numOfSMPerDevice) {
+        (void)blocks; (void)threads; (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
+        for (int i = 0; i < (int)acqSet.size(); ++i) run(i, basis, initialM, timestep);
+    }
+    void changeSeeds(long s) {
+        for (int i = 0; i < (int)acqSet.size(); ++i) acqSet[i].getSeed() = (int)(s * (i + 1));
+    }
+    void calcEveryADC() { for (auto& a : acqSet) a.calcADC(); }
+
+  private:
+    static int integrator() {
+#if defined RK4_RELAXATION || defined USE_RELAXATION || defined RK4_NORELAXATION
+        return AXX_MAG_RK4;
+#elif defined DORP_NORELAXATION
+        return AXX_MAG_DOPRI;
+#else
+        return AXX_MAG_ROTATE;
+#endif
+    }
+    static bool relaxing() {
+#if defined RK4_RELAXATION || defined USE_RELAXATION
+        return true;
+#else
+        return false;
+#endif
+    }
+    static bool multistep() {
+#if defined ROTATE_MULTISTEP || defined DORP_NORELAXATION
+        return true;
+#else
+        return false;
+#endif
+    }
+    template <class B>
+    void run(int num, B& basis, Vector3& initialM, real dt) {
+        magAcquisition<F>& acq = acqSet[num];
+        const int n = acq.getNumOfSteps();
+        const int nf = acq.getNumOfMeas();
+        const int integ = integrator();
+        axx_compat::Prim p = basis.p;
+        const real L = 6.0 * p.R;
+        const double origin[3] = {L / 2 - p.cx, L / 2 - p.cy, L / 2 - p.cz};
+        p.cx = L / 2; p.cy = L / 2; p.cz = L / 2;
+        // substeps and sample times per step
+        std::vector<int> sub(n, 1);
+        if (multistep())
+            for (int i = 0; i < n; ++i)
+                sub[i] = axx_compat::substeps_at(acq.getFieldFunctors(), dt * i, dt,
+                                                 axx_compat::has_step_time_min<F>());
+        std::vector<double> times;
+        static const double nodes[6] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0};
+        for (int i = 0; i < n; ++i) {
+            const double ti = (double)i * dt, h = dt / sub[i];
+            if (integ == AXX_MAG_ROTATE)
+                for (int k = 0; k < sub[i]; ++k) times.push_back(ti + k * h);
+            else if (integ == AXX_MAG_RK4)
+                for (int k = 0; k <= 2 * sub[i]; ++k) times.push_back(ti + k * (h / 2.0));
+            else
+                for (int k = 0; k < sub[i]; ++k)
+                    for (int c = 0; c < 6; ++c) times.push_back(ti + (k + nodes[c]) * h);
+        }
+        std::vector<double> samples(times.size() * nf * 12);
+        for (size_t k = 0; k < times.size(); ++k)
+            for (int j = 0; j < nf; ++j) {
+                double* rec = &samples[(k * nf + j) * 12];
+                acq.getFieldFunctors()[j].affine(times[k], rec, rec + 3);
+            }
+        int type = p.type;
+        double center[3] = {p.cx, p.cy, p.cz}, radius = p.R, D = p.D, T2 = p.T2, kappa = p.kappa;
+        double normal[3] = {p.nx, p.ny, p.nz};
+        axx_substrate s{};
+        s.a = L; s.b = L; s.c = L; s.D_lat = p.D; s.T2_lat = p.T2;
+        s.n_prims = 1;
+        s.type = &type; s.center = center; s.radius = &radius; s.D = &D; s.T2 = &T2; s.kappa = &kappa;
+        s.normal = normal;
+        const double t1 = p.T1 > 0 ? (double)p.T1 : INFINITY;
+        double T1[2] = {t1, t1};
+        axx_field f{};
+        f.n_steps = n; f.dt = dt; f.gamma = GAMMA; f.n_fields = nf; f.integrator = integ;
+        f.substeps = sub.data(); f.samples = samples.data();
+        for (int a = 0; a < 3; ++a) f.origin[a] = origin[a];
+        f.m0[0] = initialM.x; f.m0[1] = initialM.y; f.m0[2] = initialM.z;
+        f.T1 = T1;
+        axx_run r{};
+        r.n_walkers = number_of_particles;
+        r.seed = (uint64_t)(uint32_t)acq.getSeed();
+        r.instance = (uint32_t)num;
+        r.init_mode = AXX_INIT_INSIDE_PRIM;
+        r.init_arg = 0;
+#if defined SPECULAR_REFLECTION
+        r.boundary_mode = AXX_BOUNDARY_SPECULAR;
+#else
+        r.boundary_mode = AXX_BOUNDARY_REJECTION;
+#endif
+        r.relaxation = relaxing() ? 1 : 0;
+        std::vector<double> mx(nf), my(nf), mz(nf);
+        axx_mag_result out{};
+        out.mx = mx.data(); out.my = my.data(); out.mz = mz.data();
+        axx_compat::check(axx_run_mag(axx_compat::context(), &s, &f, &r, &out), "axx_run_mag");
+        for (int j = 0; j < nf; ++j) {
+            acq.getMx()[j] = (real)mx[j];
+            acq.getMy()[j] = (real)my[j];
+            acq.getMz()[j] = (real)mz[j];
+        }
+    }
 };
 
 // ------------------------------------------------- random packing (host)

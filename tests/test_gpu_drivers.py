@@ -68,3 +68,39 @@ def test_wccompare_stream_equals_single_runs():
     assert [int(r[0]) for r in rows] == [2, 4]
     assert all(float(x) > 0 for r in rows for x in r[1:])
     assert lines[-1] == "max |stream - single| = 0", lines[-1]
+
+
+def test_maglf_equals_direct_magnetization_runs():
+    """The low-field driver (magAcquisitionStream over axx_run_mag) gives the
+    ADC that direct runs of the same fields, seeds and pore give."""
+    from axonexa_amd import engine
+    from axonexa_amd import fields as fl
+    from axonexa_amd import substrates as sb
+    from axonexa_amd import analysis
+    N, NOA, REP, SEED = 2048, 3, 2, 11
+    print(f"[gpu] driver maglf {N} {NOA} {REP} {SEED}", flush=True)
+    out = subprocess.run([os.path.join(BIN, "maglf"), str(N), str(NOA), str(REP), str(SEED)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rows = [[float(x) for x in l.split()] for l in out.stdout.splitlines() if l.strip() and "B0" not in l]
+    assert len(rows) == NOA
+    R, D, dt, B0 = 0.0025, 2.5e-6, 1e-4, 5e-4
+    L = 6.0 * R
+    sub = sb.single_sphere(L, R, D_in=D, D_ex=D)
+    for i in range(NOA):
+        d = (i + 1) * 0.05 / NOA
+        n = int((2.0 * d + 0.01) / dt)
+        fs = [fl.QSC_GRE(B0, 0.0, d, 0.01), fl.QSC_GRE(B0, 6e-6, d, 0.01)]
+        f = fl.build_field(fs, n, dt, origin=(L / 2,) * 3, m0=(1.0, 0.0, 0.0))
+        adcs = []
+        for r in range(REP):
+            seed = i * SEED if r == 0 else SEED * (r + 1) * (i + 1)
+            m = engine.run_mag(sub, f, abi.RunSpec(n_walkers=N, seed=seed, instance=i,
+                                                   init_mode=abi.INIT_INSIDE_PRIM))
+            adcs.append(analysis.adc_fit([x.getB() for x in fs], np.hypot(m.mx, m.my)))
+        assert abs(rows[i][0] - (0.01 + d)) < 1e-12
+        # ln|S| moves by ~1e-9 between the two b-values, so the normal-equation
+        # fit (n sum xy - sum x sum y) cancels ~12 digits: the two fits agree
+        # to ~1e-5 relative although the walkers are the same
+        assert np.isclose(rows[i][1], np.mean(adcs), rtol=1e-4, atol=0.0), (i, rows[i], adcs)
+        assert np.isclose(rows[i][2], np.std(adcs), rtol=1e-2, atol=0.0), (i, rows[i], adcs)
