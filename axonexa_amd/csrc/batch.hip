@@ -317,220 +317,268 @@ extern "C" void axx_batch_destroy(axx_batch* b) {
 
 extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* inst, const axx_tuning* tune,
                                 axx_batch** out) {
-    if (!ctx || !inst || !out || n_inst < 1) return fail(ctx, AXX_ERR_INVALID, "batch: bad arguments");
-    *out = nullptr;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    axx_batch* b = new (std::nothrow) axx_batch();
-    if (!b) return AXX_ERR_NOMEM;
-    b->ctx = ctx;
-    axx_tuning t{};
-    if (tune) t = *tune;
-    if (t.lds_budget <= 0) t.lds_budget = 72 * 1024;   // room for two workgroups per CU
-    t.block = 0;
-    char sig[64];
-    std::snprintf(sig, sizeof(sig), "n%d", n_inst);
-    b->signature = sig;
-    std::vector<WalkArgs> args;
-    std::vector<int> nmom;
-    std::vector<RedArgs> red;
-    std::vector<int4> rblocks;
-    std::vector<unsigned long long*> cnt;
-    std::vector<double*> outs;
-    for (int i = 0; i < n_inst; ++i) {
-        const axx_instance& I = inst[i];
-        if (!I.substrate || !I.acq || !I.run) {
-            batch_free(b);
-            return fail(ctx, AXX_ERR_INVALID, "instance %d: null descriptor", i);
+    axx_batch* b = nullptr;   // freed by the handler if construction throws
+    axx_plan* p_new = nullptr;
+    try {
+        if (!ctx || !inst || !out || n_inst < 1) return fail(ctx, AXX_ERR_INVALID, "batch: bad arguments");
+        *out = nullptr;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        b = new (std::nothrow) axx_batch();
+        if (!b) return AXX_ERR_NOMEM;
+        b->plans.reserve((size_t)n_inst);
+        b->ctx = ctx;
+        axx_tuning t{};
+        if (tune) t = *tune;
+        if (t.lds_budget <= 0) t.lds_budget = 72 * 1024;   // room for two workgroups per CU
+        t.block = 0;
+        char sig[64];
+        std::snprintf(sig, sizeof(sig), "n%d", n_inst);
+        b->signature = sig;
+        std::vector<WalkArgs> args;
+        std::vector<int> nmom;
+        std::vector<RedArgs> red;
+        std::vector<int4> rblocks;
+        std::vector<unsigned long long*> cnt;
+        std::vector<double*> outs;
+        for (int i = 0; i < n_inst; ++i) {
+            const axx_instance& I = inst[i];
+            if (!I.substrate || !I.acq || !I.run) {
+                batch_free(b);
+                return fail(ctx, AXX_ERR_INVALID, "instance %d: null descriptor", i);
+            }
+            if (I.acq->n_profiles > kBatchMaxProfiles) {
+                batch_free(b);
+                return fail(ctx, AXX_ERR_LIMIT, "instance %d: %d profiles (batch limit %d)", i, I.acq->n_profiles,
+                            kBatchMaxProfiles);
+            }
+            if (I.run->n_walkers >= (1ll << 31)) {
+                batch_free(b);
+                return fail(ctx, AXX_ERR_LIMIT, "instance %d: n_walkers >= 2^31", i);
+            }
+            axx_plan* p = new (std::nothrow) axx_plan();
+            if (!p) {
+                batch_free(b);
+                return AXX_ERR_NOMEM;
+            }
+            p->ctx = ctx;
+            p_new = p;
+            const int rc = plan_create_impl(ctx, I.substrate, I.acq, I.run, &t, p, min_pt_for(I.acq->n_profiles));
+            if (rc) {
+                std::string msg = "instance " + std::to_string(i) + ": " + ctx->err;
+                plan_free(p);
+                batch_free(b);
+                return fail(ctx, rc, "%s", msg.c_str());
+            }
+            b->plans.push_back(p);   // capacity reserved: does not throw
+            p_new = nullptr;
+            b->cls.push_back(class_of(p));
+            WalkArgs A = plan_walk_args(p, 1, p->n_steps);
+            args.push_back(A);
+            nmom.push_back(3 * p->Pt);
+            b->lds_walk = std::max(b->lds_walk, p->lds_bytes);
+            b->lds_init = std::max(b->lds_init, p->init_lds);
+            RedArgs R = plan_red_args(p);
+            red.push_back(R);
+            const int groups = (p->M + kRedJB - 1) / kRedJB;
+            for (int c = 0; c < p->n_chunks; ++c)
+                for (int g = 0; g < groups; ++g) rblocks.push_back(make_int4(i, c, g, 0));
+            cnt.push_back(p->d_cnt);
+            outs.push_back(p->d_out);
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), ";c%dN%lldn%di%dm%d", b->cls.back(), (long long)p->N, p->n_steps,
+                          p->n_img, p->M);
+            b->signature += buf;
         }
-        if (I.acq->n_profiles > kBatchMaxProfiles) {
+        b->n_red = (int)rblocks.size();
+        int rc;
+        if ((rc = upload(ctx, &b->d_args, args)) || (rc = upload(ctx, &b->d_cls, b->cls)) ||
+            (rc = upload(ctx, &b->d_nmom, nmom)) || (rc = upload(ctx, &b->d_red, red)) ||
+            (rc = upload(ctx, &b->d_red_blocks, rblocks)) || (rc = upload(ctx, &b->d_cnt, cnt)) ||
+            (rc = upload(ctx, &b->d_out, outs))) {
             batch_free(b);
-            return fail(ctx, AXX_ERR_LIMIT, "instance %d: %d profiles (batch limit %d)", i, I.acq->n_profiles,
-                        kBatchMaxProfiles);
+            return rc;
         }
-        if (I.run->n_walkers >= (1ll << 31)) {
+        if (hipMalloc((void**)&b->d_next, sizeof(unsigned int)) != hipSuccess ||
+            hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
             batch_free(b);
-            return fail(ctx, AXX_ERR_LIMIT, "instance %d: n_walkers >= 2^31", i);
+            return fail(ctx, AXX_ERR_DEVICE, "batch: device allocation");
         }
-        axx_plan* p = new (std::nothrow) axx_plan();
-        if (!p) {
+        if (b->lds_walk > ctx->lds_max || b->lds_init > ctx->lds_max) {
             batch_free(b);
-            return AXX_ERR_NOMEM;
+            return fail(ctx, AXX_ERR_LIMIT, "batch: substrate tables exceed LDS");
         }
-        p->ctx = ctx;
-        const int rc = plan_create_impl(ctx, I.substrate, I.acq, I.run, &t, p, min_pt_for(I.acq->n_profiles));
-        if (rc) {
-            std::string msg = "instance " + std::to_string(i) + ": " + ctx->err;
-            plan_free(p);
+        // configuration: explicit, cached, or a default; the autotuner may change it
+        int blk = 256, wpt = 1;
+        load_file_cache();
+        auto it = tune_cache().find(b->signature);
+        if (tune && tune->block) blk = tune->block;
+        else if (it != tune_cache().end()) { blk = it->second.first; wpt = it->second.second; }
+        if (blk < 64 || blk > kBatchBlockMax || blk % 64) {
             batch_free(b);
-            return fail(ctx, rc, "%s", msg.c_str());
+            return fail(ctx, AXX_ERR_INVALID, "batch block must be a multiple of 64 in [64, %d]", kBatchBlockMax);
         }
-        b->plans.push_back(p);
-        b->cls.push_back(class_of(p));
-        WalkArgs A = plan_walk_args(p, 1, p->n_steps);
-        args.push_back(A);
-        nmom.push_back(3 * p->Pt);
-        b->lds_walk = std::max(b->lds_walk, p->lds_bytes);
-        b->lds_init = std::max(b->lds_init, p->init_lds);
-        RedArgs R = plan_red_args(p);
-        red.push_back(R);
-        const int groups = (p->M + kRedJB - 1) / kRedJB;
-        for (int c = 0; c < p->n_chunks; ++c)
-            for (int g = 0; g < groups; ++g) rblocks.push_back(make_int4(i, c, g, 0));
-        cnt.push_back(p->d_cnt);
-        outs.push_back(p->d_out);
-        char buf[96];
-        std::snprintf(buf, sizeof(buf), ";c%dN%lldn%di%dm%d", b->cls.back(), (long long)p->N, p->n_steps,
-                      p->n_img, p->M);
-        b->signature += buf;
+        if ((rc = build_tiles(b, blk, wpt)) || (rc = capture(b))) {
+            batch_free(b);
+            return rc;
+        }
+        *out = b;
+        return AXX_OK;
+    } catch (...) {
+        if (p_new) plan_free(p_new);
+        if (b) batch_free(b);
+        if (out) *out = nullptr;
+        return api_exception();
     }
-    b->n_red = (int)rblocks.size();
-    int rc;
-    if ((rc = upload(ctx, &b->d_args, args)) || (rc = upload(ctx, &b->d_cls, b->cls)) ||
-        (rc = upload(ctx, &b->d_nmom, nmom)) || (rc = upload(ctx, &b->d_red, red)) ||
-        (rc = upload(ctx, &b->d_red_blocks, rblocks)) || (rc = upload(ctx, &b->d_cnt, cnt)) ||
-        (rc = upload(ctx, &b->d_out, outs))) {
-        batch_free(b);
-        return rc;
-    }
-    if (hipMalloc((void**)&b->d_next, sizeof(unsigned int)) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
-        batch_free(b);
-        return fail(ctx, AXX_ERR_DEVICE, "batch: device allocation");
-    }
-    if (b->lds_walk > ctx->lds_max || b->lds_init > ctx->lds_max) {
-        batch_free(b);
-        return fail(ctx, AXX_ERR_LIMIT, "batch: substrate tables exceed LDS");
-    }
-    // configuration: explicit, cached, or a default; the autotuner may change it
-    int blk = 256, wpt = 1;
-    load_file_cache();
-    auto it = tune_cache().find(b->signature);
-    if (tune && tune->block) blk = tune->block;
-    else if (it != tune_cache().end()) { blk = it->second.first; wpt = it->second.second; }
-    if (blk < 64 || blk > kBatchBlockMax || blk % 64) {
-        batch_free(b);
-        return fail(ctx, AXX_ERR_INVALID, "batch block must be a multiple of 64 in [64, %d]", kBatchBlockMax);
-    }
-    if ((rc = build_tiles(b, blk, wpt)) || (rc = capture(b))) {
-        batch_free(b);
-        return rc;
-    }
-    *out = b;
-    return AXX_OK;
 }
 
 extern "C" int axx_batch_run(axx_batch* b) {
-    if (!b) return AXX_ERR_INVALID;
-    axx_ctx* ctx = b->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream));
-    for (axx_plan* p : b->plans) {
-        p->next_step = p->n_steps;
-        p->reduced = true;
+    try {
+        if (!b) return AXX_ERR_INVALID;
+        axx_ctx* ctx = b->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream));
+        for (axx_plan* p : b->plans) {
+            p->next_step = p->n_steps;
+            p->reduced = true;
+        }
+        b->ran = true;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    b->ran = true;
-    return AXX_OK;
 }
 
 extern "C" int axx_batch_time(axx_batch* b, float* ms) {
-    if (!b || !ms) return AXX_ERR_INVALID;
-    axx_ctx* ctx = b->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
-    int rc = axx_batch_run(b);
-    if (rc) return rc;
-    HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
-    HIP_TRY(ctx, hipEventSynchronize(b->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
-    return AXX_OK;
+    try {
+        if (!b || !ms) return AXX_ERR_INVALID;
+        axx_ctx* ctx = b->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
+        int rc = axx_batch_run(b);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(b->ev1));
+        HIP_TRY(ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_sync(axx_batch* b) {
-    if (!b) return AXX_ERR_INVALID;
-    HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
-    return AXX_OK;
+    try {
+        if (!b) return AXX_ERR_INVALID;
+        HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_results(axx_batch* b, int i, axx_result* out) {
-    if (!b || !out || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch results: bad index");
-    if (!b->ran) return fail(b->ctx, AXX_ERR_STATE, "batch results requested before a run");
-    return axx_plan_results(b->plans[i], out);
+    try {
+        if (!b || !out || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch results: bad index");
+        if (!b->ran) return fail(b->ctx, AXX_ERR_STATE, "batch results requested before a run");
+        return axx_plan_results(b->plans[i], out);
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_get_state(axx_batch* b, int i, uint32_t* host) {
-    if (!b || !host || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch state: bad index");
-    return axx_plan_get_state(b->plans[i], host);
+    try {
+        if (!b || !host || i < 0 || i >= (int)b->plans.size()) return fail(b ? b->ctx : nullptr, AXX_ERR_INVALID, "batch state: bad index");
+        return axx_plan_get_state(b->plans[i], host);
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_set_config(axx_batch* b, int block, int wpt) {
-    if (!b) return AXX_ERR_INVALID;
-    if (block < 64 || block > kBatchBlockMax || block % 64 || wpt < 1 || wpt > 64)
-        return fail(b->ctx, AXX_ERR_INVALID, "batch config: block multiple of 64 in [64, %d], wpt in [1, 64]",
-                    kBatchBlockMax);
-    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
-    HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
-    int rc = build_tiles(b, block, wpt);
-    if (rc) return rc;
-    return capture(b);
+    try {
+        if (!b) return AXX_ERR_INVALID;
+        if (block < 64 || block > kBatchBlockMax || block % 64 || wpt < 1 || wpt > 64)
+            return fail(b->ctx, AXX_ERR_INVALID, "batch config: block multiple of 64 in [64, %d], wpt in [1, 64]",
+                        kBatchBlockMax);
+        HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+        HIP_TRY(b->ctx, hipStreamSynchronize(b->ctx->stream));
+        int rc = build_tiles(b, block, wpt);
+        if (rc) return rc;
+        return capture(b);
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_get_config(const axx_batch* b, int* block, int* wpt, int* grid, int* n_tiles) {
-    if (!b) return AXX_ERR_INVALID;
-    if (block) *block = b->block;
-    if (wpt) *wpt = b->wpt;
-    if (grid) *grid = b->grid;
-    if (n_tiles) *n_tiles = b->n_tiles;
-    return AXX_OK;
+    try {
+        if (!b) return AXX_ERR_INVALID;
+        if (block) *block = b->block;
+        if (wpt) *wpt = b->wpt;
+        if (grid) *grid = b->grid;
+        if (n_tiles) *n_tiles = b->n_tiles;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_batch_autotune(axx_batch* b, int probe_steps, int* best_block, int* best_wpt, float* best_ms) {
-    if (!b || probe_steps < 1) return AXX_ERR_INVALID;
-    axx_ctx* ctx = b->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    // probe arguments: every instance walks steps [1, min(n, probe + 1))
-    std::vector<WalkArgs> probe;
-    for (axx_plan* p : b->plans) probe.push_back(plan_walk_args(p, 1, std::min(p->n_steps, probe_steps + 1)));
-    int rc = upload(ctx, &b->d_probe, probe);
-    if (rc) return rc;
-    static const int blocks[] = {256, 512};
-    static const int wpts[] = {1, 2, 4};
-    float best = 1e30f;
-    int bb = b->block, bw = b->wpt;
-    for (int blk : blocks)
-        for (int w : wpts) {
-            if ((rc = build_tiles(b, blk, w))) return rc;
-            if ((rc = launch_walk_part(b, b->d_probe))) return rc;   // warm
-            HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
-            if ((rc = launch_walk_part(b, b->d_probe))) return rc;
-            HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
-            HIP_TRY(ctx, hipEventSynchronize(b->ev1));
-            float ms = 0;
-            HIP_TRY(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best) {
-                best = ms;
-                bb = blk;
-                bw = w;
+    try {
+        if (!b || probe_steps < 1) return AXX_ERR_INVALID;
+        axx_ctx* ctx = b->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        // probe arguments: every instance walks steps [1, min(n, probe + 1))
+        std::vector<WalkArgs> probe;
+        for (axx_plan* p : b->plans) probe.push_back(plan_walk_args(p, 1, std::min(p->n_steps, probe_steps + 1)));
+        int rc = upload(ctx, &b->d_probe, probe);
+        if (rc) return rc;
+        static const int blocks[] = {256, 512};
+        static const int wpts[] = {1, 2, 4};
+        float best = 1e30f;
+        int bb = b->block, bw = b->wpt;
+        for (int blk : blocks)
+            for (int w : wpts) {
+                if ((rc = build_tiles(b, blk, w))) return rc;
+                if ((rc = launch_walk_part(b, b->d_probe))) return rc;   // warm
+                HIP_TRY(ctx, hipEventRecord(b->ev0, ctx->stream));
+                if ((rc = launch_walk_part(b, b->d_probe))) return rc;
+                HIP_TRY(ctx, hipEventRecord(b->ev1, ctx->stream));
+                HIP_TRY(ctx, hipEventSynchronize(b->ev1));
+                float ms = 0;
+                HIP_TRY(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (ms < best) {
+                    best = ms;
+                    bb = blk;
+                    bw = w;
+                }
             }
-        }
-    if ((rc = build_tiles(b, bb, bw)) || (rc = capture(b))) return rc;
-    for (axx_plan* p : b->plans) p->reduced = false;   // the probe overwrote the walker state
-    b->ran = false;
-    tune_cache()[b->signature] = {bb, bw};
-    store_file_cache(b->signature, bb, bw);
-    if (best_block) *best_block = bb;
-    if (best_wpt) *best_wpt = bw;
-    if (best_ms) *best_ms = best;
-    return AXX_OK;
+        if ((rc = build_tiles(b, bb, bw)) || (rc = capture(b))) return rc;
+        for (axx_plan* p : b->plans) p->reduced = false;   // the probe overwrote the walker state
+        b->ran = false;
+        tune_cache()[b->signature] = {bb, bw};
+        store_file_cache(b->signature, bb, bw);
+        if (best_block) *best_block = bb;
+        if (best_wpt) *best_wpt = bw;
+        if (best_ms) *best_ms = best;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_run_batch(axx_ctx* ctx, int n_inst, const axx_instance* inst, axx_result* out) {
-    if (!out) return AXX_ERR_INVALID;
-    axx_batch* b = nullptr;
-    int rc = axx_batch_create(ctx, n_inst, inst, nullptr, &b);
-    if (rc) return rc;
-    rc = axx_batch_run(b);
-    for (int i = 0; !rc && i < n_inst; ++i) rc = axx_batch_results(b, i, &out[i]);
-    axx_batch_destroy(b);
-    return rc;
+    try {
+        if (!out) return AXX_ERR_INVALID;
+        axx_batch* b = nullptr;
+        int rc = axx_batch_create(ctx, n_inst, inst, nullptr, &b);
+        if (rc) return rc;
+        rc = axx_batch_run(b);
+        for (int i = 0; !rc && i < n_inst; ++i) rc = axx_batch_results(b, i, &out[i]);
+        axx_batch_destroy(b);
+        return rc;
+    } catch (...) {
+        return api_exception();
+    }
 }

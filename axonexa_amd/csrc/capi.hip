@@ -42,25 +42,29 @@ extern "C" const char* axx_strerror(int code) {
 extern "C" const char* axx_last_error(const axx_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
 
 extern "C" int axx_create(axx_ctx** out, int device) {
-    if (!out) return AXX_ERR_INVALID;
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return AXX_ERR_NODEVICE;
-    if (device < 0 || device >= n) return AXX_ERR_INVALID;
-    axx_ctx* c = new (std::nothrow) axx_ctx();
-    if (!c) return AXX_ERR_NOMEM;
-    c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-        delete c;
-        return AXX_ERR_DEVICE;
+    try {
+        if (!out) return AXX_ERR_INVALID;
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return AXX_ERR_NODEVICE;
+        if (device < 0 || device >= n) return AXX_ERR_INVALID;
+        axx_ctx* c = new (std::nothrow) axx_ctx();
+        if (!c) return AXX_ERR_NOMEM;
+        c->device = device;
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return AXX_ERR_DEVICE;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+            c->n_cu = prop.multiProcessorCount;
+            c->lds_max = prop.sharedMemPerBlock;
+        }
+        *out = c;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
-        c->n_cu = prop.multiProcessorCount;
-        c->lds_max = prop.sharedMemPerBlock;
-    }
-    *out = c;
-    return AXX_OK;
 }
 
 extern "C" void axx_destroy(axx_ctx* ctx) {
@@ -451,7 +455,11 @@ void plan_free(axx_plan* p) {
 
 extern "C" int axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
                                const axx_run* r, axx_plan** out) {
-    return axx_plan_create_ex(ctx, s, q, r, nullptr, out);
+    try {
+        return axx_plan_create_ex(ctx, s, q, r, nullptr, out);
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" void axx_plan_destroy(axx_plan* plan) {
@@ -586,18 +594,28 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
 
 extern "C" int axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
                                   const axx_run* r, const axx_tuning* tune, axx_plan** out) {
-    if (!ctx || !s || !q || !r || !out) return fail(ctx, AXX_ERR_INVALID, "null argument");
-    *out = nullptr;
-    axx_plan* p = new (std::nothrow) axx_plan();
-    if (!p) return AXX_ERR_NOMEM;
-    p->ctx = ctx;
-    int rc = plan_create_impl(ctx, s, q, r, tune, p, 1);
-    if (rc) {
-        plan_free(p);
-        return rc;
+    try {
+        if (!ctx || !s || !q || !r || !out) return fail(ctx, AXX_ERR_INVALID, "null argument");
+        *out = nullptr;
+        axx_plan* p = new (std::nothrow) axx_plan();
+        if (!p) return AXX_ERR_NOMEM;
+        p->ctx = ctx;
+        int rc;
+        try {
+            rc = plan_create_impl(ctx, s, q, r, tune, p, 1);
+        } catch (...) {
+            plan_free(p);
+            throw;
+        }
+        if (rc) {
+            plan_free(p);
+            return rc;
+        }
+        *out = p;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    *out = p;
-    return AXX_OK;
 }
 
 WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
@@ -648,39 +666,47 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
 }
 
 extern "C" int axx_plan_walk(axx_plan* p, int step_begin, int step_end) {
-    if (!p) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    if (step_begin < 0 || step_end > p->n_steps || step_begin > step_end)
-        return fail(ctx, AXX_ERR_INVALID, "step range [%d, %d) outside [0, %d)", step_begin, step_end,
-                    p->n_steps);
-    if (step_begin != 0 && step_begin != p->next_step)
-        return fail(ctx, AXX_ERR_STATE, "walk must restart at 0 or continue at step %d", p->next_step);
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    int rc = launch_walk(p, step_begin, step_end);
-    if (rc) return rc;
-    p->next_step = step_end;
-    p->reduced = false;
-    return AXX_OK;
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        if (step_begin < 0 || step_end > p->n_steps || step_begin > step_end)
+            return fail(ctx, AXX_ERR_INVALID, "step range [%d, %d) outside [0, %d)", step_begin, step_end,
+                        p->n_steps);
+        if (step_begin != 0 && step_begin != p->next_step)
+            return fail(ctx, AXX_ERR_STATE, "walk must restart at 0 or continue at step %d", p->next_step);
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        int rc = launch_walk(p, step_begin, step_end);
+        if (rc) return rc;
+        p->next_step = step_end;
+        p->reduced = false;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_time_walk(axx_plan* p, int step_begin, int step_end, float* ms) {
-    if (!p || !ms) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    int rc;
-    if (step_begin == 0) {
-        // placement first, outside the timed region: the time is the walk
-        // kernel's alone
-        if ((rc = axx_plan_walk(p, 0, 1))) return rc;
-        step_begin = 1;
+    try {
+        if (!p || !ms) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        int rc;
+        if (step_begin == 0) {
+            // placement first, outside the timed region: the time is the walk
+            // kernel's alone
+            if ((rc = axx_plan_walk(p, 0, 1))) return rc;
+            step_begin = 1;
+        }
+        HIP_TRY(ctx, hipEventRecord(p->ev0, ctx->stream));
+        rc = axx_plan_walk(p, step_begin, step_end);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipEventRecord(p->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(p->ev1));
+        HIP_TRY(ctx, hipEventElapsedTime(ms, p->ev0, p->ev1));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    HIP_TRY(ctx, hipEventRecord(p->ev0, ctx->stream));
-    rc = axx_plan_walk(p, step_begin, step_end);
-    if (rc) return rc;
-    HIP_TRY(ctx, hipEventRecord(p->ev1, ctx->stream));
-    HIP_TRY(ctx, hipEventSynchronize(p->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(ms, p->ev0, p->ev1));
-    return AXX_OK;
 }
 
 RedArgs plan_red_args(const axx_plan* p) {
@@ -704,139 +730,171 @@ RedArgs plan_red_args(const axx_plan* p) {
 }
 
 extern "C" int axx_plan_reduce(axx_plan* p) {
-    if (!p) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    if (p->next_step != p->n_steps)
-        return fail(ctx, AXX_ERR_STATE, "walk not finished (at step %d of %d)", p->next_step, p->n_steps);
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const RedArgs R = plan_red_args(p);
-    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_cnt, 4);
-    const int groups = (p->M + kRedJB - 1) / kRedJB;
-    hipLaunchKernelGGL(reduce_kernel, dim3(p->n_chunks, groups), dim3(kRedThreads), 0, ctx->stream, R);
-    HIP_TRY(ctx, hipGetLastError());
-    const int tot = p->M * kRedVals;
-    hipLaunchKernelGGL(finalize_kernel, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const double*)p->d_partial, p->M, p->n_chunks, p->d_out);
-    HIP_TRY(ctx, hipGetLastError());
-    p->reduced = true;
-    return AXX_OK;
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        if (p->next_step != p->n_steps)
+            return fail(ctx, AXX_ERR_STATE, "walk not finished (at step %d of %d)", p->next_step, p->n_steps);
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        const RedArgs R = plan_red_args(p);
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_cnt, 4);
+        const int groups = (p->M + kRedJB - 1) / kRedJB;
+        hipLaunchKernelGGL(reduce_kernel, dim3(p->n_chunks, groups), dim3(kRedThreads), 0, ctx->stream, R);
+        HIP_TRY(ctx, hipGetLastError());
+        const int tot = p->M * kRedVals;
+        hipLaunchKernelGGL(finalize_kernel, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const double*)p->d_partial, p->M, p->n_chunks, p->d_out);
+        HIP_TRY(ctx, hipGetLastError());
+        p->reduced = true;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_sync(axx_plan* p) {
-    if (!p) return AXX_ERR_INVALID;
-    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
-    return AXX_OK;
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_results(axx_plan* p, axx_result* out) {
-    if (!p || !out) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    if (!p->reduced) return fail(ctx, AXX_ERR_STATE, "results requested before reduce");
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    std::vector<double> h((size_t)p->M * kRedVals);
-    unsigned long long cnt[4];
-    int err = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(h.data(), p->d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(cnt, p->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
-    for (int j = 0; j < p->M; ++j) {
-        const double* v = &h[(size_t)j * kRedVals];
-        if (out->sum_cos) out->sum_cos[j] = v[0];
-        if (out->sum_sin) out->sum_sin[j] = v[1];
-        if (out->sum_cos2) out->sum_cos2[j] = v[2];
-        if (out->sum_sin2) out->sum_sin2[j] = v[3];
-        if (out->intra_cos) out->intra_cos[j] = v[4];
-        if (out->intra_sin) out->intra_sin[j] = v[5];
+    try {
+        if (!p || !out) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        if (!p->reduced) return fail(ctx, AXX_ERR_STATE, "results requested before reduce");
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        std::vector<double> h((size_t)p->M * kRedVals);
+        unsigned long long cnt[4];
+        int err = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(h.data(), p->d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(cnt, p->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+        for (int j = 0; j < p->M; ++j) {
+            const double* v = &h[(size_t)j * kRedVals];
+            if (out->sum_cos) out->sum_cos[j] = v[0];
+            if (out->sum_sin) out->sum_sin[j] = v[1];
+            if (out->sum_cos2) out->sum_cos2[j] = v[2];
+            if (out->sum_sin2) out->sum_sin2[j] = v[3];
+            if (out->intra_cos) out->intra_cos[j] = v[4];
+            if (out->intra_sin) out->intra_sin[j] = v[5];
+        }
+        out->n_walkers = p->N;
+        out->collisions = cnt[0];
+        out->transmissions = cnt[1];
+        out->cap_hits = cnt[2];
+        out->n_intra = (int64_t)cnt[3];
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    out->n_walkers = p->N;
-    out->collisions = cnt[0];
-    out->transmissions = cnt[1];
-    out->cap_hits = cnt[2];
-    out->n_intra = (int64_t)cnt[3];
-    return AXX_OK;
 }
 
 // State is exchanged in the public layout (AXX_STATE_FIELDS(n_profiles));
 // internally moments are kept for the padded profile width.
 extern "C" int axx_plan_get_state(axx_plan* p, uint32_t* host) {
-    if (!p || !host) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    const size_t nf = AXX_STATE_FIELDS(p->P);
-    HIP_TRY(ctx, hipMemcpy(host, p->d_state, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    int err = 0;
-    HIP_TRY(ctx, hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
-    return AXX_OK;
+    try {
+        if (!p || !host) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        const size_t nf = AXX_STATE_FIELDS(p->P);
+        HIP_TRY(ctx, hipMemcpy(host, p->d_state, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        int err = 0;
+        HIP_TRY(ctx, hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_set_state(axx_plan* p, const uint32_t* host) {
-    if (!p || !host) return AXX_ERR_INVALID;
-    axx_ctx* ctx = p->ctx;
-    // the kernels index per-compartment tables with the compartment and
-    // assume positions within a cell length of the cell and one common step
-    {
-        const size_t N = (size_t)p->N;
-        const int* is = reinterpret_cast<const int*>(host);
-        const float* fs = reinterpret_cast<const float*>(host);
-        const int step = is[(size_t)F_STEP * N];
-        if (step < 1 || step > p->n_steps) return fail(ctx, AXX_ERR_INVALID, "state: step %d", step);
-        const float lo[3] = {-p->cell[0], -p->cell[1], -p->cell[2]};
-        const float hi[3] = {2 * p->cell[0], 2 * p->cell[1], 2 * p->cell[2]};
-        for (size_t w = 0; w < N; ++w) {
-            const int c = is[(size_t)F_COMP * N + w], c0 = is[(size_t)F_COMP0 * N + w];
-            if (c < 0 || c > p->n_prims || c0 < 0 || c0 > p->n_prims)
-                return fail(ctx, AXX_ERR_INVALID, "state: walker %zu compartment out of range", w);
-            if (is[(size_t)F_STEP * N + w] != step)
-                return fail(ctx, AXX_ERR_INVALID, "state: walkers at different steps");
-            for (int a = 0; a < 3; ++a) {
-                const float v = fs[(size_t)(F_X + a) * N + w];
-                if (!(v >= lo[a] && v <= hi[a])) return fail(ctx, AXX_ERR_INVALID, "state: walker %zu position", w);
+    try {
+        if (!p || !host) return AXX_ERR_INVALID;
+        axx_ctx* ctx = p->ctx;
+        // the kernels index per-compartment tables with the compartment and
+        // assume positions within a cell length of the cell and one common step
+        {
+            const size_t N = (size_t)p->N;
+            const int* is = reinterpret_cast<const int*>(host);
+            const float* fs = reinterpret_cast<const float*>(host);
+            const int step = is[(size_t)F_STEP * N];
+            if (step < 1 || step > p->n_steps) return fail(ctx, AXX_ERR_INVALID, "state: step %d", step);
+            const float lo[3] = {-p->cell[0], -p->cell[1], -p->cell[2]};
+            const float hi[3] = {2 * p->cell[0], 2 * p->cell[1], 2 * p->cell[2]};
+            for (size_t w = 0; w < N; ++w) {
+                const int c = is[(size_t)F_COMP * N + w], c0 = is[(size_t)F_COMP0 * N + w];
+                if (c < 0 || c > p->n_prims || c0 < 0 || c0 > p->n_prims)
+                    return fail(ctx, AXX_ERR_INVALID, "state: walker %zu compartment out of range", w);
+                if (is[(size_t)F_STEP * N + w] != step)
+                    return fail(ctx, AXX_ERR_INVALID, "state: walkers at different steps");
+                for (int a = 0; a < 3; ++a) {
+                    const float v = fs[(size_t)(F_X + a) * N + w];
+                    if (!(v >= lo[a] && v <= hi[a])) return fail(ctx, AXX_ERR_INVALID, "state: walker %zu position", w);
+                }
             }
         }
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        const size_t nf = AXX_STATE_FIELDS(p->P);
+        HIP_TRY(ctx, hipMemcpy(p->d_state, host, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyHostToDevice));
+        const int* is = reinterpret_cast<const int*>(host);
+        p->next_step = is[(size_t)F_STEP * p->N];
+        p->reduced = false;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    const size_t nf = AXX_STATE_FIELDS(p->P);
-    HIP_TRY(ctx, hipMemcpy(p->d_state, host, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyHostToDevice));
-    const int* is = reinterpret_cast<const int*>(host);
-    p->next_step = is[(size_t)F_STEP * p->N];
-    p->reduced = false;
-    return AXX_OK;
 }
 
 extern "C" int axx_plan_get_config(const axx_plan* p, int* block, int* wpt, double* bin_width, int* lds) {
-    if (!p) return AXX_ERR_INVALID;
-    if (block) *block = std::min(p->block, walk_max_block(p->Pt));   // as launched
-    if (wpt) *wpt = p->wpt;
-    if (bin_width) *bin_width = p->bin_width;
-    if (lds) *lds = (int)p->lds_bytes;
-    return AXX_OK;
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        if (block) *block = std::min(p->block, walk_max_block(p->Pt));   // as launched
+        if (wpt) *wpt = p->wpt;
+        if (bin_width) *bin_width = p->bin_width;
+        if (lds) *lds = (int)p->lds_bytes;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_set_config(axx_plan* p, int block, int wpt) {
-    if (!p) return AXX_ERR_INVALID;
-    if (block < 64 || block > 1024 || (block % 64) != 0) return fail(p->ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
-    if (wpt != 1) return fail(p->ctx, AXX_ERR_INVALID, "walkers per thread must be 1");
-    p->block = block;
-    p->wpt = wpt;
-    return AXX_OK;
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        if (block < 64 || block > 1024 || (block % 64) != 0) return fail(p->ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+        if (wpt != 1) return fail(p->ctx, AXX_ERR_INVALID, "walkers per thread must be 1");
+        p->block = block;
+        p->wpt = wpt;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_plan_profile_count(const axx_plan* p) { return p ? p->P : AXX_ERR_INVALID; }
 
 extern "C" int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
                              axx_result* out) {
-    if (!ctx || !out) return AXX_ERR_INVALID;
-    axx_plan* p = nullptr;
-    int rc = axx_plan_create(ctx, s, q, r, &p);
-    if (rc) return rc;
-    rc = axx_plan_walk(p, 0, q->n_steps);
-    if (!rc) rc = axx_plan_reduce(p);
-    if (!rc) rc = axx_plan_results(p, out);
-    axx_plan_destroy(p);
-    return rc;
+    try {
+        if (!ctx || !out) return AXX_ERR_INVALID;
+        axx_plan* p = nullptr;
+        int rc = axx_plan_create(ctx, s, q, r, &p);
+        if (rc) return rc;
+        rc = axx_plan_walk(p, 0, q->n_steps);
+        if (!rc) rc = axx_plan_reduce(p);
+        if (!rc) rc = axx_plan_results(p, out);
+        axx_plan_destroy(p);
+        return rc;
+    } catch (...) {
+        return api_exception();
+    }
 }

@@ -85,6 +85,19 @@ static int fail(axx_ctx* ctx, int code, const char* fmt, ...) {
     return code;
 }
 
+// Every int entry point of the C ABI turns a C++ exception (in practice a
+// failed host allocation for an oversized input) into an error code instead
+// of letting it cross the C boundary.
+static inline int api_exception() {
+    try {
+        throw;
+    } catch (const std::bad_alloc&) {
+        return AXX_ERR_NOMEM;
+    } catch (...) {
+        return AXX_ERR_INVALID;
+    }
+}
+
 #define HIP_TRY(ctx, call)                                                                  \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \

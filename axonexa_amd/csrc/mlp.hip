@@ -494,275 +494,314 @@ extern "C" void axx_mlp_destroy(axx_mlp* m) {
 }
 
 extern "C" int axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int max_batch, axx_mlp** out) {
-    if (!ctx || !dims || !out || n_layers < 1 || n_layers > 64 || max_batch < 1)
-        return fail(ctx, AXX_ERR_INVALID, "mlp: bad arguments");
-    *out = nullptr;
-    for (int l = 0; l <= n_layers; ++l)
-        if (dims[l] < 1 || dims[l] > 16384) return fail(ctx, AXX_ERR_INVALID, "mlp: dims[%d] = %d", l, dims[l]);
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    axx_mlp* m = new (std::nothrow) axx_mlp();
-    if (!m) return AXX_ERR_NOMEM;
-    m->ctx = ctx;
-    m->max_batch = pad_to(max_batch, kBK * kSplitK);
-    int maxdim = 0;
-    for (int l = 0; l < n_layers; ++l) {
-        Layer y;
-        y.in = dims[l];
-        y.out = dims[l + 1];
-        // inputs of layer l are the padded outputs of layer l - 1
-        y.inp = l == 0 ? pad_to(y.in, kBN) : m->L[l - 1].outp;
-        y.outp = pad_to(y.out, kBN);
-        y.gelu = l + 1 < n_layers;
-        maxdim = std::max(maxdim, std::max(y.inp, y.outp));
-        m->L.push_back(y);
-    }
-    const size_t B = (size_t)m->max_batch;
-    int rc = AXX_OK;
-    for (Layer& y : m->L) {
-        const size_t nw = (size_t)y.outp * y.inp;
-        if ((rc = dalloc(ctx, &y.w, nw)) || (rc = dalloc(ctx, &y.mw, nw)) || (rc = dalloc(ctx, &y.vw, nw)) ||
-            (rc = dalloc(ctx, &y.b, y.outp)) || (rc = dalloc(ctx, &y.mb, y.outp)) ||
-            (rc = dalloc(ctx, &y.vb, y.outp)) || (rc = dalloc(ctx, &y.gpart, nw * kSplitK)) ||
-            (rc = dalloc(ctx, &y.gb, (size_t)y.outp * kBiasSlices)) || (rc = dalloc(ctx, &y.wt_bf, nw)) ||
-            (rc = dalloc(ctx, &y.w_bf, nw)))
-            break;
-    }
-    m->H.assign(n_layers + 1, nullptr);
-    m->Z.assign(n_layers, nullptr);
-    for (int l = 0; !rc && l < n_layers; ++l) {
-        rc = dalloc(ctx, &m->H[l], B * m->L[l].inp);
-        if (!rc && m->L[l].gelu) {
-            rc = dalloc(ctx, &m->Z[l], B * m->L[l].outp);
-            if (!rc) rc = dalloc(ctx, &m->H[l + 1], B * m->L[l].outp);
+    axx_mlp* m = nullptr;   // freed by the handler if construction throws
+    try {
+        if (!ctx || !dims || !out || n_layers < 1 || n_layers > 64 || max_batch < 1)
+            return fail(ctx, AXX_ERR_INVALID, "mlp: bad arguments");
+        *out = nullptr;
+        for (int l = 0; l <= n_layers; ++l)
+            if (dims[l] < 1 || dims[l] > 16384) return fail(ctx, AXX_ERR_INVALID, "mlp: dims[%d] = %d", l, dims[l]);
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        m = new (std::nothrow) axx_mlp();
+        if (!m) return AXX_ERR_NOMEM;
+        m->ctx = ctx;
+        m->max_batch = pad_to(max_batch, kBK * kSplitK);
+        int maxdim = 0;
+        for (int l = 0; l < n_layers; ++l) {
+            Layer y;
+            y.in = dims[l];
+            y.out = dims[l + 1];
+            // inputs of layer l are the padded outputs of layer l - 1
+            y.inp = l == 0 ? pad_to(y.in, kBN) : m->L[l - 1].outp;
+            y.outp = pad_to(y.out, kBN);
+            y.gelu = l + 1 < n_layers;
+            maxdim = std::max(maxdim, std::max(y.inp, y.outp));
+            m->L.push_back(y);
         }
+        const size_t B = (size_t)m->max_batch;
+        int rc = AXX_OK;
+        for (Layer& y : m->L) {
+            const size_t nw = (size_t)y.outp * y.inp;
+            if ((rc = dalloc(ctx, &y.w, nw)) || (rc = dalloc(ctx, &y.mw, nw)) || (rc = dalloc(ctx, &y.vw, nw)) ||
+                (rc = dalloc(ctx, &y.b, y.outp)) || (rc = dalloc(ctx, &y.mb, y.outp)) ||
+                (rc = dalloc(ctx, &y.vb, y.outp)) || (rc = dalloc(ctx, &y.gpart, nw * kSplitK)) ||
+                (rc = dalloc(ctx, &y.gb, (size_t)y.outp * kBiasSlices)) || (rc = dalloc(ctx, &y.wt_bf, nw)) ||
+                (rc = dalloc(ctx, &y.w_bf, nw)))
+                break;
+        }
+        m->H.assign(n_layers + 1, nullptr);
+        m->Z.assign(n_layers, nullptr);
+        for (int l = 0; !rc && l < n_layers; ++l) {
+            rc = dalloc(ctx, &m->H[l], B * m->L[l].inp);
+            if (!rc && m->L[l].gelu) {
+                rc = dalloc(ctx, &m->Z[l], B * m->L[l].outp);
+                if (!rc) rc = dalloc(ctx, &m->H[l + 1], B * m->L[l].outp);
+            }
+        }
+        const Layer& last = m->L.back();
+        if (!rc) rc = dalloc(ctx, &m->Y, B * last.outp);
+        if (!rc) rc = dalloc(ctx, &m->dA, B * maxdim);
+        if (!rc) rc = dalloc(ctx, &m->dB, B * maxdim);
+        if (!rc) rc = dalloc(ctx, &m->HT, B * maxdim);
+        if (!rc) rc = dalloc(ctx, &m->dZT, B * maxdim);
+        if (!rc) rc = dalloc(ctx, &m->T, B * last.out);
+        if (!rc) rc = dalloc(ctx, &m->loss_part, kLossBlocks);
+        if (!rc) rc = dalloc(ctx, &m->xin, B * m->L[0].in);
+        if (rc) {
+            mlp_free(m);
+            return rc;
+        }
+        *out = m;
+        return AXX_OK;
+    } catch (...) {
+        if (m && !(out && *out == m)) mlp_free(m);
+        if (out) *out = nullptr;
+        return api_exception();
     }
-    const Layer& last = m->L.back();
-    if (!rc) rc = dalloc(ctx, &m->Y, B * last.outp);
-    if (!rc) rc = dalloc(ctx, &m->dA, B * maxdim);
-    if (!rc) rc = dalloc(ctx, &m->dB, B * maxdim);
-    if (!rc) rc = dalloc(ctx, &m->HT, B * maxdim);
-    if (!rc) rc = dalloc(ctx, &m->dZT, B * maxdim);
-    if (!rc) rc = dalloc(ctx, &m->T, B * last.out);
-    if (!rc) rc = dalloc(ctx, &m->loss_part, kLossBlocks);
-    if (!rc) rc = dalloc(ctx, &m->xin, B * m->L[0].in);
-    if (rc) {
-        mlp_free(m);
-        return rc;
-    }
-    *out = m;
-    return AXX_OK;
 }
 
 // Parameters, packed per layer: W[out][in] row major, then b[out] (fp32).
 extern "C" int axx_mlp_param_count(const axx_mlp* m, int64_t* n) {
-    if (!m || !n) return AXX_ERR_INVALID;
-    int64_t s = 0;
-    for (const Layer& y : m->L) s += (int64_t)y.out * y.in + y.out;
-    *n = s;
-    return AXX_OK;
+    try {
+        if (!m || !n) return AXX_ERR_INVALID;
+        int64_t s = 0;
+        for (const Layer& y : m->L) s += (int64_t)y.out * y.in + y.out;
+        *n = s;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 extern "C" int axx_mlp_set_params(axx_mlp* m, const float* p) {
-    if (!m || !p) return AXX_ERR_INVALID;
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    for (Layer& y : m->L) {
-        std::vector<float> w((size_t)y.outp * y.inp, 0.0f), b(y.outp, 0.0f);
-        for (int o = 0; o < y.out; ++o)
-            for (int i = 0; i < y.in; ++i) w[(size_t)o * y.inp + i] = *p++;
-        for (int o = 0; o < y.out; ++o) b[o] = *p++;
-        HIP_TRY(ctx, hipMemcpy(y.w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
-        HIP_TRY(ctx, hipMemcpy(y.b, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
-        HIP_TRY(ctx, hipMemset(y.mw, 0, w.size() * sizeof(float)));
-        HIP_TRY(ctx, hipMemset(y.vw, 0, w.size() * sizeof(float)));
-        HIP_TRY(ctx, hipMemset(y.mb, 0, b.size() * sizeof(float)));
-        HIP_TRY(ctx, hipMemset(y.vb, 0, b.size() * sizeof(float)));
+    try {
+        if (!m || !p) return AXX_ERR_INVALID;
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (Layer& y : m->L) {
+            std::vector<float> w((size_t)y.outp * y.inp, 0.0f), b(y.outp, 0.0f);
+            for (int o = 0; o < y.out; ++o)
+                for (int i = 0; i < y.in; ++i) w[(size_t)o * y.inp + i] = *p++;
+            for (int o = 0; o < y.out; ++o) b[o] = *p++;
+            HIP_TRY(ctx, hipMemcpy(y.w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIP_TRY(ctx, hipMemcpy(y.b, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIP_TRY(ctx, hipMemset(y.mw, 0, w.size() * sizeof(float)));
+            HIP_TRY(ctx, hipMemset(y.vw, 0, w.size() * sizeof(float)));
+            HIP_TRY(ctx, hipMemset(y.mb, 0, b.size() * sizeof(float)));
+            HIP_TRY(ctx, hipMemset(y.vb, 0, b.size() * sizeof(float)));
+        }
+        m->adam_t = 0;
+        int rc = refresh_bf16(m);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    m->adam_t = 0;
-    int rc = refresh_bf16(m);
-    if (rc) return rc;
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return AXX_OK;
 }
 
 extern "C" int axx_mlp_get_params(axx_mlp* m, float* p) {
-    if (!m || !p) return AXX_ERR_INVALID;
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    for (Layer& y : m->L) {
-        std::vector<float> w((size_t)y.outp * y.inp), b(y.outp);
-        HIP_TRY(ctx, hipMemcpy(w.data(), y.w, w.size() * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(ctx, hipMemcpy(b.data(), y.b, b.size() * sizeof(float), hipMemcpyDeviceToHost));
-        for (int o = 0; o < y.out; ++o)
-            for (int i = 0; i < y.in; ++i) *p++ = w[(size_t)o * y.inp + i];
-        for (int o = 0; o < y.out; ++o) *p++ = b[o];
+    try {
+        if (!m || !p) return AXX_ERR_INVALID;
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (Layer& y : m->L) {
+            std::vector<float> w((size_t)y.outp * y.inp), b(y.outp);
+            HIP_TRY(ctx, hipMemcpy(w.data(), y.w, w.size() * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_TRY(ctx, hipMemcpy(b.data(), y.b, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int o = 0; o < y.out; ++o)
+                for (int i = 0; i < y.in; ++i) *p++ = w[(size_t)o * y.inp + i];
+            for (int o = 0; o < y.out; ++o) *p++ = b[o];
+        }
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    return AXX_OK;
 }
 
 // y[n][out] = network(x[n][in]), host buffers; n <= max_batch.
 extern "C" int axx_mlp_forward(axx_mlp* m, const float* x, int n, float* y) {
-    if (!m || !x || !y || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp forward: n");
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int n_pad = pad_to(n, kBM);
-    int rc = load_inputs(m, x, n, n_pad);
-    if (!rc) rc = forward(m, n_pad);
-    if (rc) return rc;
-    const Layer& last = m->L.back();
-    float* tmp = m->xin;   // reuse the staging buffer for the unpadded output
-    if ((size_t)n * last.out > (size_t)m->max_batch * m->L[0].in) {
-        std::vector<float> full((size_t)n_pad * last.outp);
-        HIP_TRY(ctx, hipMemcpyAsync(full.data(), m->Y, full.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    try {
+        if (!m || !x || !y || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp forward: n");
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        const int n_pad = pad_to(n, kBM);
+        int rc = load_inputs(m, x, n, n_pad);
+        if (!rc) rc = forward(m, n_pad);
+        if (rc) return rc;
+        const Layer& last = m->L.back();
+        float* tmp = m->xin;   // reuse the staging buffer for the unpadded output
+        if ((size_t)n * last.out > (size_t)m->max_batch * m->L[0].in) {
+            std::vector<float> full((size_t)n_pad * last.outp);
+            HIP_TRY(ctx, hipMemcpyAsync(full.data(), m->Y, full.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            for (int r = 0; r < n; ++r)
+                for (int c = 0; c < last.out; ++c) y[(size_t)r * last.out + c] = full[(size_t)r * last.outp + c];
+            return AXX_OK;
+        }
+        const long long tot = (long long)n * last.out;
+        hipLaunchKernelGGL(unpad_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const float*)m->Y, n, last.outp, last.out, tmp);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipMemcpyAsync(y, tmp, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        for (int r = 0; r < n; ++r)
-            for (int c = 0; c < last.out; ++c) y[(size_t)r * last.out + c] = full[(size_t)r * last.outp + c];
         return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    const long long tot = (long long)n * last.out;
-    hipLaunchKernelGGL(unpad_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const float*)m->Y, n, last.outp, last.out, tmp);
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipMemcpyAsync(y, tmp, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return AXX_OK;
 }
 
 // Forward over the n_pad rows already resident in the input buffer, timed
 // with HIP events (inputs staged by a previous axx_mlp_forward).
 extern "C" int axx_mlp_time_forward(axx_mlp* m, int n, float* ms) {
-    if (!m || !ms || n < 1 || n > m->max_batch) return AXX_ERR_INVALID;
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    hipEvent_t e0, e1;
-    HIP_TRY(ctx, hipEventCreate(&e0));
-    HIP_TRY(ctx, hipEventCreate(&e1));
-    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
-    int rc = forward(m, pad_to(n, kBM));
-    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
-    HIP_TRY(ctx, hipEventSynchronize(e1));
-    HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return rc;
+    try {
+        if (!m || !ms || n < 1 || n > m->max_batch) return AXX_ERR_INVALID;
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        hipEvent_t e0, e1;
+        HIP_TRY(ctx, hipEventCreate(&e0));
+        HIP_TRY(ctx, hipEventCreate(&e1));
+        HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+        int rc = forward(m, pad_to(n, kBM));
+        HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(e1));
+        HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 // Loss (mean squared error over the n x out targets) and gradients of the
 // mean loss w.r.t. the packed parameters, for the batch (x, t).
 extern "C" int axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int n, float* loss, float* grad) {
-    if (!m || !x || !t || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp grad: n");
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int n_pad = pad_to(n, kBK * kSplitK);   // M tiles and split-K slices
-    const Layer& last = m->L.back();
-    int rc = load_inputs(m, x, n, n_pad);
-    if (rc) return rc;
-    HIP_TRY(ctx, hipMemsetAsync(m->T, 0, (size_t)n_pad * last.out * sizeof(float), ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(m->T, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = forward(m, n_pad))) return rc;
-    if (n_pad != n) {
-        // padded rows must not contribute: their targets equal their outputs
-        HIP_TRY(ctx, hipMemcpy2DAsync(m->T + (size_t)n * last.out, last.out * sizeof(float),
-                                      m->Y + (size_t)n * last.outp, last.outp * sizeof(float),
-                                      last.out * sizeof(float), n_pad - n, hipMemcpyDeviceToDevice,
-                                      ctx->stream));
-    }
-    if ((rc = backward(m, n_pad))) return rc;
-    std::vector<float> part(kLossBlocks);
-    HIP_TRY(ctx, hipMemcpyAsync(part.data(), m->loss_part, kLossBlocks * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    double s = 0.0;
-    for (float v : part) s += v;
-    if (loss) *loss = (float)(s / ((double)n * last.out));
-    // the scale of dY used n_pad rows; rescale to the mean over n rows
-    const float rescale = (float)n_pad / (float)n;
-    if (grad) {
-        for (Layer& y : m->L) {
-            const size_t nw = (size_t)y.outp * y.inp;
-            std::vector<float> gp(nw * kSplitK), gb((size_t)y.outp * kBiasSlices);
-            HIP_TRY(ctx, hipMemcpy(gp.data(), y.gpart, gp.size() * sizeof(float), hipMemcpyDeviceToHost));
-            HIP_TRY(ctx, hipMemcpy(gb.data(), y.gb, gb.size() * sizeof(float), hipMemcpyDeviceToHost));
-            for (int o = 0; o < y.out; ++o)
-                for (int i = 0; i < y.in; ++i) {
+    try {
+        if (!m || !x || !t || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp grad: n");
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        const int n_pad = pad_to(n, kBK * kSplitK);   // M tiles and split-K slices
+        const Layer& last = m->L.back();
+        int rc = load_inputs(m, x, n, n_pad);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemsetAsync(m->T, 0, (size_t)n_pad * last.out * sizeof(float), ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(m->T, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = forward(m, n_pad))) return rc;
+        if (n_pad != n) {
+            // padded rows must not contribute: their targets equal their outputs
+            HIP_TRY(ctx, hipMemcpy2DAsync(m->T + (size_t)n * last.out, last.out * sizeof(float),
+                                          m->Y + (size_t)n * last.outp, last.outp * sizeof(float),
+                                          last.out * sizeof(float), n_pad - n, hipMemcpyDeviceToDevice,
+                                          ctx->stream));
+        }
+        if ((rc = backward(m, n_pad))) return rc;
+        std::vector<float> part(kLossBlocks);
+        HIP_TRY(ctx, hipMemcpyAsync(part.data(), m->loss_part, kLossBlocks * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        double s = 0.0;
+        for (float v : part) s += v;
+        if (loss) *loss = (float)(s / ((double)n * last.out));
+        // the scale of dY used n_pad rows; rescale to the mean over n rows
+        const float rescale = (float)n_pad / (float)n;
+        if (grad) {
+            for (Layer& y : m->L) {
+                const size_t nw = (size_t)y.outp * y.inp;
+                std::vector<float> gp(nw * kSplitK), gb((size_t)y.outp * kBiasSlices);
+                HIP_TRY(ctx, hipMemcpy(gp.data(), y.gpart, gp.size() * sizeof(float), hipMemcpyDeviceToHost));
+                HIP_TRY(ctx, hipMemcpy(gb.data(), y.gb, gb.size() * sizeof(float), hipMemcpyDeviceToHost));
+                for (int o = 0; o < y.out; ++o)
+                    for (int i = 0; i < y.in; ++i) {
+                        float g = 0.0f;
+                        for (int sl = 0; sl < kSplitK; ++sl) g += gp[(size_t)sl * nw + (size_t)o * y.inp + i];
+                        *grad++ = g * rescale;
+                    }
+                for (int o = 0; o < y.out; ++o) {
                     float g = 0.0f;
-                    for (int sl = 0; sl < kSplitK; ++sl) g += gp[(size_t)sl * nw + (size_t)o * y.inp + i];
+                    for (int sl = 0; sl < kBiasSlices; ++sl) g += gb[(size_t)sl * y.outp + o];
                     *grad++ = g * rescale;
                 }
-            for (int o = 0; o < y.out; ++o) {
-                float g = 0.0f;
-                for (int sl = 0; sl < kBiasSlices; ++sl) g += gb[(size_t)sl * y.outp + o];
-                *grad++ = g * rescale;
             }
         }
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    return AXX_OK;
 }
 
 // Keep a dataset on the device for training: x[n][in], t[n][out].
 extern "C" int axx_mlp_set_data(axx_mlp* m, const float* x, const float* t, int n) {
-    if (!m || !x || !t || n < 1) return AXX_ERR_INVALID;
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (m->dX) (void)hipFree(m->dX);
-    if (m->dT) (void)hipFree(m->dT);
-    m->dX = nullptr;
-    m->dT = nullptr;
-    const Layer& f = m->L[0];
-    const Layer& last = m->L.back();
-    int rc = dalloc(ctx, &m->dX, (size_t)n * f.inp, false);
-    if (!rc) rc = dalloc(ctx, &m->dT, (size_t)n * last.out, false);
-    if (rc) return rc;
-    float* tmp = nullptr;
-    if ((rc = dalloc(ctx, &tmp, (size_t)n * f.in, false))) return rc;
-    HIP_TRY(ctx, hipMemcpy(tmp, x, (size_t)n * f.in * sizeof(float), hipMemcpyHostToDevice));
-    const long long tot = (long long)n * f.inp;
-    hipLaunchKernelGGL(to_bf16_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const float*)tmp, n, f.in, f.inp, m->dX);
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipMemcpy(m->dT, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(tmp);
-    m->n_data = n;
-    return AXX_OK;
+    try {
+        if (!m || !x || !t || n < 1) return AXX_ERR_INVALID;
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (m->dX) (void)hipFree(m->dX);
+        if (m->dT) (void)hipFree(m->dT);
+        m->dX = nullptr;
+        m->dT = nullptr;
+        const Layer& f = m->L[0];
+        const Layer& last = m->L.back();
+        int rc = dalloc(ctx, &m->dX, (size_t)n * f.inp, false);
+        if (!rc) rc = dalloc(ctx, &m->dT, (size_t)n * last.out, false);
+        if (rc) return rc;
+        float* tmp = nullptr;
+        if ((rc = dalloc(ctx, &tmp, (size_t)n * f.in, false))) return rc;
+        HIP_TRY(ctx, hipMemcpy(tmp, x, (size_t)n * f.in * sizeof(float), hipMemcpyHostToDevice));
+        const long long tot = (long long)n * f.inp;
+        hipLaunchKernelGGL(to_bf16_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const float*)tmp, n, f.in, f.inp, m->dX);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipMemcpy(m->dT, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(tmp);
+        m->n_data = n;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
 }
 
 // Adam training on the resident dataset: `steps` minibatches of `batch`
 // rows drawn with Philox (key = seed, counter = (row, global step)).
 // loss_trace[s] (optional) receives each step's minibatch loss.
 extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_t seed, float* loss_trace) {
-    if (!m || steps < 1 || batch < 1 || batch > m->max_batch || !m->dX)
-        return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp train: arguments or no data");
-    if (batch % (kBK * kSplitK) || batch % kBM)
-        return fail(m->ctx, AXX_ERR_INVALID, "mlp train: batch must be a multiple of %d", kBK * kSplitK);
-    axx_ctx* ctx = m->ctx;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const Layer& f = m->L[0];
-    const Layer& last = m->L.back();
-    float* d_loss = nullptr;
-    int rc0 = dalloc(ctx, &d_loss, (size_t)steps);
-    if (rc0) return rc0;
-    for (int s = 0; s < steps; ++s) {
-        hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(128), 0, ctx->stream, (const __bf16*)m->dX, f.inp,
-                           (const float*)m->dT, last.out, m->n_data, batch, (uint32_t)(m->adam_t + 1),
-                           (uint32_t)seed, (uint32_t)(seed >> 32), m->H[0], m->T);
-        HIP_TRY(ctx, hipGetLastError());
-        int rc = forward(m, batch);
-        if (!rc) rc = backward(m, batch);
-        if (!rc) rc = adam_step(m, lr);
-        if (rc) {
-            (void)hipFree(d_loss);
-            return rc;
+    try {
+        if (!m || steps < 1 || batch < 1 || batch > m->max_batch || !m->dX)
+            return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp train: arguments or no data");
+        if (batch % (kBK * kSplitK) || batch % kBM)
+            return fail(m->ctx, AXX_ERR_INVALID, "mlp train: batch must be a multiple of %d", kBK * kSplitK);
+        axx_ctx* ctx = m->ctx;
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        const Layer& f = m->L[0];
+        const Layer& last = m->L.back();
+        float* d_loss = nullptr;
+        int rc0 = dalloc(ctx, &d_loss, (size_t)steps);
+        if (rc0) return rc0;
+        for (int s = 0; s < steps; ++s) {
+            hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(128), 0, ctx->stream, (const __bf16*)m->dX, f.inp,
+                               (const float*)m->dT, last.out, m->n_data, batch, (uint32_t)(m->adam_t + 1),
+                               (uint32_t)seed, (uint32_t)(seed >> 32), m->H[0], m->T);
+            HIP_TRY(ctx, hipGetLastError());
+            int rc = forward(m, batch);
+            if (!rc) rc = backward(m, batch);
+            if (!rc) rc = adam_step(m, lr);
+            if (rc) {
+                (void)hipFree(d_loss);
+                return rc;
+            }
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, ctx->stream, (const float*)m->loss_part,
+                               kLossBlocks, 1.0f / ((float)batch * last.out), d_loss + s);
         }
-        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, ctx->stream, (const float*)m->loss_part,
-                           kLossBlocks, 1.0f / ((float)batch * last.out), d_loss + s);
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess && loss_trace)
+            e = hipMemcpy(loss_trace, d_loss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipFree(d_loss);
+        if (e != hipSuccess) return fail(ctx, AXX_ERR_DEVICE, "mlp train: %s", hipGetErrorString(e));
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
     }
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess && loss_trace)
-        e = hipMemcpy(loss_trace, d_loss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
-    (void)hipFree(d_loss);
-    if (e != hipSuccess) return fail(ctx, AXX_ERR_DEVICE, "mlp train: %s", hipGetErrorString(e));
-    return AXX_OK;
 }

@@ -727,8 +727,8 @@ class phaseAcquisitionStream {
         store(d, num);
     }
     // Every acquisition in one batched launch (axx_run_batch) when the batch
-    // takes them (no rejection policy, at most AXX_BATCH_MAX_PROFILES
-    // distinct waveform shapes each); otherwise one run per acquisition.
+    // takes them (at most AXX_BATCH_MAX_PROFILES distinct waveform shapes
+    // each); otherwise one run per acquisition.
     void run_all(const std::vector<axx_compat::Prim>& prims, Lattice& lat, real dt, int imode, int iarg) {
         const int n = (int)acqSet.size();
         if (n == 0) return;
@@ -737,8 +737,7 @@ class phaseAcquisitionStream {
         for (int i = 0; i < n; ++i) {
             ds.emplace_back(new Desc);
             describe(*ds.back(), i, prims, lat, dt, imode, iarg, step_mode(), -1);
-            batched = batched && ds.back()->r.boundary_mode != AXX_BOUNDARY_REJECTION &&
-                      ds.back()->q.n_profiles <= AXX_BATCH_MAX_PROFILES;
+            batched = batched && ds.back()->q.n_profiles <= AXX_BATCH_MAX_PROFILES;
         }
         if (batched) {
             std::vector<axx_instance> inst(n);

@@ -144,3 +144,10 @@ def test_invalid_field_descriptors_are_errors():
         assert e.value.code == -1, k
     m = eng.run_mag(sub, good, run)
     assert abs(np.hypot(m.mx[0], m.my[0]) / run.n_walkers - 1.0) < 0.05
+    # a corrupt substrate descriptor is an error code, not a crash
+    import ctypes as C
+    ss, ff, rr = sub.struct(), good.struct(), run.struct()
+    ss.n_prims = -3
+    res = abi.MagResult()
+    ctx = eng.default_context()
+    assert eng.lib().axx_run_mag(ctx.handle, C.byref(ss), C.byref(ff), C.byref(rr), C.byref(res)) == -1
