@@ -397,3 +397,7 @@ def mag_from_state(state: np.ndarray, n_walkers: int, F: int) -> np.ndarray:
     words = st[F_MOM:].reshape(F, 3, 2, n_walkers)
     both = (words[:, :, 0].astype(np.uint64) | (words[:, :, 1].astype(np.uint64) << np.uint64(32)))
     return both.view(np.float64).transpose(2, 0, 1)
+
+
+class MagInstance(C.Structure):
+    _fields_ = [("substrate", C.POINTER(Substrate)), ("field", C.POINTER(Field)), ("run", C.POINTER(Run))]

@@ -27,7 +27,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = (["capi.hip", "batch.hip", "mlp.hip", "mag.hip", "mag_rotate.hip", "mag_rk4.hip", "mag_dopri.hip"]
+SOURCES = (["capi.hip", "batch.hip", "mlp.hip", "mag.hip", "mag_rotate.hip", "mag_rk4.hip", "mag_dopri.hip",
+            "mag_batch.hip", "mag_batch_rotate.hip", "mag_batch_rk4.hip", "mag_batch_dopri.hip"]
            + [f"walk_p{p}.hip" for p in (1, 2, 4, 8, 16, 32, 64)])
 LIBNAME = "libaxonexa.so"
 

@@ -3,14 +3,10 @@
 // the walk core of the phase path.  One plan per group of up to eight fields
 // (its own walker state, same walkers); field tables, substep offsets and
 // per-compartment relaxation rates on the device; deterministic fp64 sums.
-#include "internal.h"
+#include "mag_internal.h"
 #include "init_kernel.h"
-#include "mag_kernel.h"
 
 namespace axx {
-
-constexpr int kMagChunk = 4096;
-constexpr int kMagRedThreads = 256;
 
 // M = m0 for every field slot of every walker (after placement).
 static __global__ void mag_fill_kernel(uint32_t* __restrict__ state, long long N, int mb, double m0x, double m0y,
@@ -30,49 +26,12 @@ static __global__ void mag_zero_kernel(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-// Block (chunk c, field j): sums of Mx, My, Mz over the chunk's walkers in a
-// fixed order (strided per thread, then a fixed tree); field 0's blocks also
-// add the event counters.
+// Block (chunk c, field j): see mag_reduce_block.
 static __global__ __launch_bounds__(kMagRedThreads) void mag_reduce_kernel(const uint32_t* __restrict__ state,
                                                                            long long N, int n_chunks,
                                                                            double* __restrict__ partial,
                                                                            unsigned long long* __restrict__ cnt) {
-    __shared__ double sh[3][kMagRedThreads];
-    __shared__ unsigned long long shc[3][kMagRedThreads];
-    const int c = blockIdx.x, j = blockIdx.y, t = threadIdx.x;
-    const long long w0 = (long long)c * kMagChunk;
-    const long long w1 = w0 + kMagChunk < N ? w0 + kMagChunk : N;
-    double s[3] = {0.0, 0.0, 0.0};
-    unsigned long long e[3] = {0ull, 0ull, 0ull};
-    for (long long w = w0 + t; w < w1; w += kMagRedThreads) {
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t lo = state[(size_t)(F_MOM + 6 * j + 2 * k) * N + w];
-            const uint32_t hi = state[(size_t)(F_MOM + 6 * j + 2 * k + 1) * N + w];
-            s[k] += __hiloint2double((int)hi, (int)lo);
-        }
-        if (j == 0 && cnt) {
-            e[0] += state[(size_t)F_NCOLL * N + w];
-            e[1] += state[(size_t)F_NTRAN * N + w];
-            e[2] += state[(size_t)F_NCAP * N + w];
-        }
-    }
-    for (int k = 0; k < 3; ++k) {
-        sh[k][t] = s[k];
-        shc[k][t] = e[k];
-    }
-    __syncthreads();
-    for (int h = kMagRedThreads / 2; h > 0; h >>= 1) {
-        if (t < h)
-            for (int k = 0; k < 3; ++k) {
-                sh[k][t] += sh[k][t + h];
-                shc[k][t] += shc[k][t + h];
-            }
-        __syncthreads();
-    }
-    if (t < 3) {
-        partial[((size_t)j * n_chunks + c) * 3 + t] = sh[t][0];
-        if (j == 0 && cnt) atomicAdd(&cnt[t], shc[t][0]);
-    }
+    mag_reduce_block(state, N, n_chunks, blockIdx.x, blockIdx.y, partial, cnt);
 }
 
 // Chunk-ordered totals: out[j * 3 + k].
@@ -80,36 +39,12 @@ static __global__ void mag_finalize_kernel(const double* __restrict__ partial, i
                                            double* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= F * 3) return;
-    const int j = t / 3, k = t % 3;
-    double s = 0.0;
-    for (int c = 0; c < n_chunks; ++c) s += partial[((size_t)j * n_chunks + c) * 3 + k];
-    out[t] = s;
+    out[t] = mag_chunk_total(partial, n_chunks, t / 3, t % 3);
 }
 
 }  // namespace axx
 
-struct axx_mag {
-    axx_ctx* ctx = nullptr;
-    int n_fields = 0, integ = 0, n_steps = 0;
-    long long N = 0;
-    std::vector<axx_plan*> plans;    // one per group
-    std::vector<int> gF, gMB;
-    std::vector<double*> d_tab;
-    int* d_off = nullptr;
-    int* d_nsub = nullptr;
-    double* d_iT1 = nullptr;
-    double* d_iT2 = nullptr;
-    double* d_part = nullptr;        // [group][F][n_chunks][3]
-    double* d_out = nullptr;         // [n_fields][3]
-    unsigned long long* d_cnt = nullptr;
-    MagArgs base{};
-    int n_chunks = 0;
-    int next_step = 0;
-    bool reduced = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
-
-static void mag_free(axx_mag* m) {
+void mag_free(axx_mag* m) {
     if (!m) return;
     for (axx_plan* p : m->plans) plan_free(p);
     for (double* t : m->d_tab)
@@ -161,8 +96,8 @@ extern "C" int axx_mag_sample_count(int integrator, int n_steps, const int* subs
     }
 }
 
-extern "C" int axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
-                              axx_mag** out) {
+int mag_create_internal(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
+                        int lds_budget, axx_mag** out) {
     axx_mag* m = nullptr;   // freed by the handler if construction throws
     axx_plan* p_new = nullptr;
     try {
@@ -232,6 +167,7 @@ extern "C" int axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_fi
             p_new = p;
             axx_tuning t{};
             t.block = kMagBlock;
+            t.lds_budget = lds_budget;
             rc = plan_create_impl(ctx, s, &q, r, &t, p, 2 * mb);
             if (rc) {
                 plan_free(p);
@@ -281,6 +217,11 @@ extern "C" int axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_fi
         if (out) *out = nullptr;
         return api_exception();
     }
+}
+
+extern "C" int axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
+                              axx_mag** out) {
+    return mag_create_internal(ctx, s, f, r, kMagLdsBudget, out);
 }
 
 static mag_fn pick_mag(int integ, int mb, bool gen, bool rej) {

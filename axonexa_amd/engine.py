@@ -101,6 +101,14 @@ def lib():
     L.axx_mag_get_state.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32)]
     L.axx_mag_destroy.argtypes = [vp]
     L.axx_run_mag.argtypes = [vp, S, Fp, R, MR]
+    MI = C.POINTER(abi.MagInstance)
+    L.axx_mag_batch_create.argtypes = [vp, C.c_int, MI, C.POINTER(vp)]
+    L.axx_mag_batch_run.argtypes = [vp]
+    L.axx_mag_batch_time.argtypes = [vp, C.POINTER(C.c_float)]
+    L.axx_mag_batch_results.argtypes = [vp, C.c_int, MR]
+    L.axx_mag_batch_get_state.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32)]
+    L.axx_mag_batch_destroy.argtypes = [vp]
+    L.axx_run_mag_batch.argtypes = [vp, C.c_int, MI, MR]
     _LIB = L
     return L
 
@@ -323,6 +331,63 @@ def run_mag(sub: abi.SubstrateSpec, field: abi.FieldSpec, run: abi.RunSpec,
     res, bufs = _mag_result(field.n_fields)
     ctx.check(lib().axx_run_mag(ctx.handle, C.byref(s), C.byref(f), C.byref(r), C.byref(res)), "axx_run_mag")
     return _mag_signals(res, bufs)
+
+
+class MagBatch:
+    """Many (substrate, field, run) magnetization runs in one persistent
+    launch (axx_mag_batch_*); each run gives exactly its own run_mag."""
+
+    def __init__(self, items, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.items = list(items)
+        self._structs = [(s.struct(), f.struct(), r.struct()) for s, f, r in self.items]
+        arr = (abi.MagInstance * len(self.items))()
+        for k, (s, f, r) in enumerate(self._structs):
+            arr[k].substrate = C.pointer(s)
+            arr[k].field = C.pointer(f)
+            arr[k].run = C.pointer(r)
+        self._arr = arr
+        self._h = C.c_void_p()
+        self.ctx.check(lib().axx_mag_batch_create(self.ctx.handle, len(self.items), arr, C.byref(self._h)),
+                       "axx_mag_batch_create")
+
+    def run(self):
+        self.ctx.check(lib().axx_mag_batch_run(self._h), "axx_mag_batch_run")
+        return self
+
+    def time(self) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().axx_mag_batch_time(self._h, C.byref(ms)), "axx_mag_batch_time")
+        return ms.value
+
+    def results(self, i: int) -> abi.MagSignals:
+        res, bufs = _mag_result(self.items[i][1].n_fields)
+        self.ctx.check(lib().axx_mag_batch_results(self._h, int(i), C.byref(res)), "axx_mag_batch_results")
+        return _mag_signals(res, bufs)
+
+    def get_state(self, i: int) -> np.ndarray:
+        _, f, r = self.items[i]
+        st = np.zeros(abi.mag_state_fields(f.n_fields) * r.n_walkers, np.uint32)
+        self.ctx.check(lib().axx_mag_batch_get_state(self._h, int(i), st.ctypes.data_as(C.POINTER(C.c_uint32))),
+                       "axx_mag_batch_get_state")
+        return st
+
+    def close(self):
+        if self._h:
+            lib().axx_mag_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Batch:

@@ -261,3 +261,23 @@ def mag_lf(n_walkers: int = 917504, n_fields: int = 2, integrator: int = abi.MAG
     return MagWorkload("mag_lf", sub, f, run,
                        f"low-field QSC_GRE echo in a 2.5 um sphere, {n_fields} fields, {n} steps of 0.1 us, "
                        f"{n_walkers} walkers, integrator {integrator}")
+
+
+def mag_lf_stream(n_acq: int = 1000, n_walkers: int = 114688):
+    """Every acquisition of examples/magLF_spherNeuman.cu: acquisition i has
+    lobes of (i + 1) 0.05 / n_acq ms, spacing 0.01 ms, two QSC_GRE fields
+    (B0 = 0.5 mT, G = 0 and 6e-6 T/mm), 114,688 walkers in a 2.5 um sphere,
+    dt = 1e-4 ms; seeds i * 11.  Returns [(substrate, field, run)]."""
+    from . import fields as fl
+    R, D, dt = 0.0025, 2.5e-6, 1e-4
+    L = 6.0 * R
+    sub = sb.single_sphere(L, R, D_in=D, D_ex=D)
+    items = []
+    for i in range(n_acq):
+        d = (i + 1) * 0.05 / n_acq
+        n = int((2.0 * d + 0.01) / dt)
+        fs = [fl.QSC_GRE(5e-4, 0.0, d, 0.01), fl.QSC_GRE(5e-4, 6e-6, d, 0.01)]
+        f = fl.build_field(fs, n, dt, origin=(L / 2,) * 3, m0=(1.0, 0.0, 0.0))
+        items.append((sub, f, abi.RunSpec(n_walkers=n_walkers, seed=i * 11, instance=i,
+                                          init_mode=abi.INIT_INSIDE_PRIM)))
+    return items

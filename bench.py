@@ -69,7 +69,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2_hex",
                     choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder", "c4_batch",
-                             "c5_surrogate", "mag_lf"])
+                             "c5_surrogate", "mag_lf", "mag_stream"])
+    ap.add_argument("--mag-acqs", type=int, default=1000, help="mag_stream: acquisitions")
     ap.add_argument("--mag-fields", type=int, default=2, help="mag_lf: fields per walker")
     ap.add_argument("--mag-integrator", type=int, default=0, help="mag_lf: 0 rotate, 1 RK4, 2 DOPRI")
     ap.add_argument("--sur-geoms", type=int, default=4096,
@@ -99,6 +100,8 @@ def main(argv=None) -> int:
         return bench_surrogate(args)
     if args.workload == "mag_lf":
         return bench_mag(args)
+    if args.workload == "mag_stream":
+        return bench_mag_stream(args)
 
     import numpy as np
 
@@ -313,6 +316,63 @@ def bench_mag(args) -> int:
         out["sample_bitwise_equal_walkers"] = float(bit.mean())
     print(json.dumps(out))
     m.close()
+    return 0
+
+
+def bench_mag_stream(args) -> int:
+    """Every acquisition of the low-field experiment in one batched launch
+    (axx_mag_batch_*), against the same acquisitions run one by one."""
+    import numpy as np
+
+    from axonexa_amd import engine
+    from axonexa_amd import workloads as W
+
+    items = W.mag_lf_stream(n_acq=args.mag_acqs)
+    upd = sum(r.n_walkers * f.n_steps * f.n_fields for _, f, r in items)
+    walker_steps = sum(r.n_walkers * (f.n_steps - 1) for _, f, r in items)
+    tc = time.perf_counter()
+    b = engine.MagBatch(items)
+    t_create = time.perf_counter() - tc
+    for _ in range(args.warmup):
+        b.run()
+        b.results(0)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(b.time())
+        last = b.results(len(items) - 1)
+    elapsed = time.perf_counter() - t0
+    value = upd * args.steps / elapsed
+    out = {
+        "metric": "field-updates/s",
+        "value": value,
+        "unit": "field-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"mag_stream: {len(items)} low-field QSC_GRE acquisitions (magLF_spherNeuman), "
+                               f"114688 walkers each, 2 fields, one batched launch",
+                   "acquisitions": len(items), "field_updates": upd, "walker_steps": walker_steps},
+        "batch_ms": float(np.mean(kernel_ms)),
+        "create_s": t_create,
+        "walker_steps_per_s": walker_steps * args.steps / elapsed,
+        "signal_last": [float(v) for v in last.transverse],
+    }
+    b.close()
+    if not args.no_unbatched:
+        tu = time.perf_counter()
+        for it in items:
+            engine.run_mag(*it)
+        tun = time.perf_counter() - tu
+        out["unbatched_s"] = tun
+        out["batched_vs_unbatched"] = tun / (elapsed / args.steps)
+    print(json.dumps(out))
     return 0
 
 

@@ -373,6 +373,28 @@ void axx_mag_destroy(axx_mag* m);
 int  axx_run_mag(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, const axx_run* r,
                  axx_mag_result* out);
 
+/*
+ * Many magnetization runs in one persistent launch: the acquisitions of
+ * magAcquisitionStream<F>::runAcquisitionStream (magAcquisitionStream.cuh:
+ * 105-342).  Every run keeps its own substrate, fields, seed, instance id and
+ * walkers, and gives bitwise the walkers and sums of its own axx_run_mag.
+ * Limits: all runs of a batch carry the same number of fields (1 to
+ * AXX_MAG_GROUP) and the same integrator; n_walkers < 2^31 per run.
+ */
+typedef struct axx_mag_instance {
+    const axx_substrate* substrate;
+    const axx_field*     field;
+    const axx_run*       run;
+} axx_mag_instance;
+typedef struct axx_mag_batch axx_mag_batch;
+int  axx_mag_batch_create(axx_ctx* ctx, int n_inst, const axx_mag_instance* inst, axx_mag_batch** out);
+int  axx_mag_batch_run(axx_mag_batch* b);                /* async: placement, walk, sums */
+int  axx_mag_batch_time(axx_mag_batch* b, float* ms);    /* run once, HIP-event timed   */
+int  axx_mag_batch_results(axx_mag_batch* b, int i, axx_mag_result* out);   /* synchronises */
+int  axx_mag_batch_get_state(axx_mag_batch* b, int i, uint32_t* host_state);
+void axx_mag_batch_destroy(axx_mag_batch* b);
+int  axx_run_mag_batch(axx_ctx* ctx, int n_inst, const axx_mag_instance* inst, axx_mag_result* out);
+
 /* Library version string. */
 const char* axx_version(void);
 

@@ -1031,8 +1031,9 @@ template <class F>
 int substeps_at(const std::vector<F>&, real, real, std::false_type) { return 1; }
 }  // namespace axx_compat
 
-// magAcquisitionStream<F> (magAcquisitionStream.cuh:1-342).  The reference
-// walks inside one basis primitive; here the pore sits in a periodic cell
+// magAcquisitionStream<F> (magAcquisitionStream.cuh:1-342);
+// runAcquisitionStream runs every acquisition in one batched launch
+// (axx_run_mag_batch).  The reference walks inside one basis primitive; here the pore sits in a periodic cell
 // six radii wide (as the phase path's single-pore runs) and the fields are
 // evaluated at the walker's position relative to the pore's own centre.
 // Integrator macros as in the reference (kernelMag.cuh:29-68, 93-136):
@@ -1065,7 +1066,7 @@ class magAcquisitionStream {
     void runAcquisitionStream(B& basis, Vector3& initialM, real timestep, int blocks, int threads, int numOfDevices,
                               std::vector<int>& plan, std::vector<int>& numOfSMPerDevice) {
         (void)blocks; (void)threads; (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
-        for (int i = 0; i < (int)acqSet.size(); ++i) run(i, basis, initialM, timestep);
+        run_all(basis, initialM, timestep);
     }
     void changeSeeds(long s) {
         for (int i = 0; i < (int)acqSet.size(); ++i) acqSet[i].getSeed() = (int)(s * (i + 1));
@@ -1096,8 +1097,24 @@ class magAcquisitionStream {
         return false;
 #endif
     }
+    // Acquisition num described for the C ABI (the descriptors point into
+    // the vectors, so a MagDesc is neither copied nor moved once filled).
+    struct MagDesc {
+        std::vector<int> sub;
+        std::vector<double> samples;
+        int type = 0;
+        double center[3] = {0, 0, 0}, radius = 0, D = 0, T2 = 0, kappa = 0, normal[3] = {0, 0, 0}, T1[2] = {0, 0};
+        axx_substrate s{};
+        axx_field f{};
+        axx_run r{};
+        std::vector<double> mx, my, mz;
+        axx_mag_result out{};
+        MagDesc() = default;
+        MagDesc(const MagDesc&) = delete;
+        MagDesc& operator=(const MagDesc&) = delete;
+    };
     template <class B>
-    void run(int num, B& basis, Vector3& initialM, real dt) {
+    void describe(MagDesc& d, int num, B& basis, Vector3& initialM, real dt) {
         magAcquisition<F>& acq = acqSet[num];
         const int n = acq.getNumOfSteps();
         const int nf = acq.getNumOfMeas();
@@ -1107,46 +1124,47 @@ class magAcquisitionStream {
         const double origin[3] = {L / 2 - p.cx, L / 2 - p.cy, L / 2 - p.cz};
         p.cx = L / 2; p.cy = L / 2; p.cz = L / 2;
         // substeps and sample times per step
-        std::vector<int> sub(n, 1);
+        d.sub.assign(n, 1);
         if (multistep())
             for (int i = 0; i < n; ++i)
-                sub[i] = axx_compat::substeps_at(acq.getFieldFunctors(), dt * i, dt,
-                                                 axx_compat::has_step_time_min<F>());
+                d.sub[i] = axx_compat::substeps_at(acq.getFieldFunctors(), dt * i, dt,
+                                                   axx_compat::has_step_time_min<F>());
         std::vector<double> times;
         static const double nodes[6] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0};
         for (int i = 0; i < n; ++i) {
-            const double ti = (double)i * dt, h = dt / sub[i];
+            const double ti = (double)i * dt, h = dt / d.sub[i];
             if (integ == AXX_MAG_ROTATE)
-                for (int k = 0; k < sub[i]; ++k) times.push_back(ti + k * h);
+                for (int k = 0; k < d.sub[i]; ++k) times.push_back(ti + k * h);
             else if (integ == AXX_MAG_RK4)
-                for (int k = 0; k <= 2 * sub[i]; ++k) times.push_back(ti + k * (h / 2.0));
+                for (int k = 0; k <= 2 * d.sub[i]; ++k) times.push_back(ti + k * (h / 2.0));
             else
-                for (int k = 0; k < sub[i]; ++k)
+                for (int k = 0; k < d.sub[i]; ++k)
                     for (int c = 0; c < 6; ++c) times.push_back(ti + (k + nodes[c]) * h);
         }
-        std::vector<double> samples(times.size() * nf * 12);
+        d.samples.assign(times.size() * nf * 12, 0.0);
         for (size_t k = 0; k < times.size(); ++k)
             for (int j = 0; j < nf; ++j) {
-                double* rec = &samples[(k * nf + j) * 12];
+                double* rec = &d.samples[(k * nf + j) * 12];
                 acq.getFieldFunctors()[j].affine(times[k], rec, rec + 3);
             }
-        int type = p.type;
-        double center[3] = {p.cx, p.cy, p.cz}, radius = p.R, D = p.D, T2 = p.T2, kappa = p.kappa;
-        double normal[3] = {p.nx, p.ny, p.nz};
-        axx_substrate s{};
+        d.type = p.type;
+        d.center[0] = p.cx; d.center[1] = p.cy; d.center[2] = p.cz;
+        d.radius = p.R; d.D = p.D; d.T2 = p.T2; d.kappa = p.kappa;
+        d.normal[0] = p.nx; d.normal[1] = p.ny; d.normal[2] = p.nz;
+        axx_substrate& s = d.s;
         s.a = L; s.b = L; s.c = L; s.D_lat = p.D; s.T2_lat = p.T2;
         s.n_prims = 1;
-        s.type = &type; s.center = center; s.radius = &radius; s.D = &D; s.T2 = &T2; s.kappa = &kappa;
-        s.normal = normal;
+        s.type = &d.type; s.center = d.center; s.radius = &d.radius; s.D = &d.D; s.T2 = &d.T2;
+        s.kappa = &d.kappa; s.normal = d.normal;
         const double t1 = p.T1 > 0 ? (double)p.T1 : INFINITY;
-        double T1[2] = {t1, t1};
-        axx_field f{};
+        d.T1[0] = t1; d.T1[1] = t1;
+        axx_field& f = d.f;
         f.n_steps = n; f.dt = dt; f.gamma = GAMMA; f.n_fields = nf; f.integrator = integ;
-        f.substeps = sub.data(); f.samples = samples.data();
+        f.substeps = d.sub.data(); f.samples = d.samples.data();
         for (int a = 0; a < 3; ++a) f.origin[a] = origin[a];
         f.m0[0] = initialM.x; f.m0[1] = initialM.y; f.m0[2] = initialM.z;
-        f.T1 = T1;
-        axx_run r{};
+        f.T1 = d.T1;
+        axx_run& r = d.r;
         r.n_walkers = number_of_particles;
         r.seed = (uint64_t)(uint32_t)acq.getSeed();
         r.instance = (uint32_t)num;
@@ -1158,15 +1176,53 @@ class magAcquisitionStream {
         r.boundary_mode = AXX_BOUNDARY_REJECTION;
 #endif
         r.relaxation = relaxing() ? 1 : 0;
-        std::vector<double> mx(nf), my(nf), mz(nf);
-        axx_mag_result out{};
-        out.mx = mx.data(); out.my = my.data(); out.mz = mz.data();
-        axx_compat::check(axx_run_mag(axx_compat::context(), &s, &f, &r, &out), "axx_run_mag");
-        for (int j = 0; j < nf; ++j) {
-            acq.getMx()[j] = (real)mx[j];
-            acq.getMy()[j] = (real)my[j];
-            acq.getMz()[j] = (real)mz[j];
+        d.mx.assign(nf, 0.0); d.my.assign(nf, 0.0); d.mz.assign(nf, 0.0);
+        d.out.mx = d.mx.data(); d.out.my = d.my.data(); d.out.mz = d.mz.data();
+    }
+    void store(const MagDesc& d, int num) {
+        magAcquisition<F>& acq = acqSet[num];
+        for (int j = 0; j < d.f.n_fields; ++j) {
+            acq.getMx()[j] = (real)d.mx[j];
+            acq.getMy()[j] = (real)d.my[j];
+            acq.getMz()[j] = (real)d.mz[j];
         }
+    }
+    template <class B>
+    void run(int num, B& basis, Vector3& initialM, real dt) {
+        MagDesc d;
+        describe(d, num, basis, initialM, dt);
+        axx_compat::check(axx_run_mag(axx_compat::context(), &d.s, &d.f, &d.r, &d.out), "axx_run_mag");
+        store(d, num);
+    }
+    // Every acquisition in one batched launch when they carry the same number
+    // of fields (at most AXX_MAG_GROUP); otherwise one run each.
+    template <class B>
+    void run_all(B& basis, Vector3& initialM, real dt) {
+        const int n = (int)acqSet.size();
+        if (n == 0) return;
+        std::vector<std::unique_ptr<MagDesc>> ds;
+        bool batched = true;
+        for (int i = 0; i < n; ++i) {
+            ds.emplace_back(new MagDesc);
+            describe(*ds.back(), i, basis, initialM, dt);
+            batched = batched && ds.back()->f.n_fields <= AXX_MAG_GROUP && ds.back()->f.n_fields == ds[0]->f.n_fields;
+        }
+        if (batched) {
+            std::vector<axx_mag_instance> inst(n);
+            std::vector<axx_mag_result> out(n);
+            for (int i = 0; i < n; ++i) {
+                inst[i].substrate = &ds[i]->s; inst[i].field = &ds[i]->f; inst[i].run = &ds[i]->r;
+                out[i] = ds[i]->out;
+            }
+            axx_compat::check(axx_run_mag_batch(axx_compat::context(), n, inst.data(), out.data()),
+                              "axx_run_mag_batch");
+        } else {
+            for (int i = 0; i < n; ++i) {
+                MagDesc& d = *ds[i];
+                axx_compat::check(axx_run_mag(axx_compat::context(), &d.s, &d.f, &d.r, &d.out), "axx_run_mag");
+            }
+        }
+        for (int i = 0; i < n; ++i) store(*ds[i], i);
     }
 };
 

@@ -151,3 +151,39 @@ def test_invalid_field_descriptors_are_errors():
     res = abi.MagResult()
     ctx = eng.default_context()
     assert eng.lib().axx_run_mag(ctx.handle, C.byref(ss), C.byref(ff), C.byref(rr), C.byref(res)) == -1
+
+
+def test_batch_of_runs_equals_single_runs():
+    """Acquisitions of a magAcquisitionStream in one launch: every run's
+    walkers and sums are those of its own run_mag (and of the oracle)."""
+    eng = _engine()
+    items = []
+    R = 0.0025
+    for k in range(7):
+        d = 0.004 * (k + 1)
+        n = int((2 * d + 0.01) / 1e-4)
+        L = 6 * R * (1.0 + 0.1 * k)
+        sub = sb.single_sphere(L, R, D_in=2.5e-6, D_ex=2.5e-6) if k % 3 else \
+            sb.hex_lattice(R, 0.006, 2, 1, D_in=1e-6, D_ex=2.5e-6)
+        f = fl.build_field([fl.QSC_GRE(5e-4, 0.0, d, 0.01), fl.QSC_GRE(5e-4, 6e-5 * (k + 1), d, 0.01)], n, 1e-4,
+                           origin=(L / 2,) * 3)
+        run = abi.RunSpec(n_walkers=700 + 900 * k, seed=100 + k, instance=k,
+                          init_mode=abi.INIT_INSIDE_PRIM if k % 3 else abi.INIT_UNIFORM,
+                          boundary_mode=abi.BOUNDARY_REJECTION if k == 4 else abi.BOUNDARY_SPECULAR)
+        items.append((sub, f, run))
+    with eng.MagBatch(items) as b:
+        b.run()
+        for i, (sub, f, run) in enumerate(items):
+            single = eng.run_mag(sub, f, run)
+            got = b.results(i)
+            assert np.array_equal(got.mx, single.mx) and np.array_equal(got.my, single.my), i
+            assert np.array_equal(got.mz, single.mz) and got.collisions == single.collisions, i
+            assert np.array_equal(b.get_state(i), po.walk_mag(sub, f, run)), i
+        # a second run of the same batch gives the same bits
+        b.run()
+        assert np.array_equal(b.results(3).mx, eng.run_mag(*items[3]).mx)
+    print(f"[mag] batch of {len(items)} runs bitwise equal to single runs and the oracle", flush=True)
+    # mixed field counts are refused
+    f3 = fl.build_field([fl.QSC_GRE(5e-4, 0.0, 0.004, 0.01)] * 3, 180, 1e-4)
+    with pytest.raises(eng.AxxError):
+        eng.MagBatch([items[0], (items[0][0], f3, items[0][2])])
