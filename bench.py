@@ -71,6 +71,8 @@ def main(argv=None) -> int:
                     choices=["c2_hex", "c3_gamma", "c1_free", "c1_cylinder", "c4_batch",
                              "c5_surrogate", "mag_lf", "mag_stream"])
     ap.add_argument("--mag-acqs", type=int, default=1000, help="mag_stream: acquisitions")
+    ap.add_argument("--lds-budget", type=int, default=0,
+                    help="c4_batch: LDS bytes per workgroup for substrate tables (0: library default)")
     ap.add_argument("--mag-fields", type=int, default=2, help="mag_lf: fields per walker")
     ap.add_argument("--mag-integrator", type=int, default=0, help="mag_lf: 0 rotate, 1 RK4, 2 DOPRI")
     ap.add_argument("--sur-geoms", type=int, default=4096,
@@ -391,7 +393,7 @@ def bench_batch(args) -> int:
     n = ws[0].acq.n_steps
     insts = [(w.sub, w.acq, w.run) for w in ws]
     t0 = time.perf_counter()
-    b = engine.Batch(insts, block=args.block)
+    b = engine.Batch(insts, block=args.block, lds_budget=args.lds_budget)
     t_create = time.perf_counter() - t0
     tune = b.autotune(probe_steps=200) if not args.block else {}
     cfg = b.config()
@@ -434,6 +436,7 @@ def bench_batch(args) -> int:
             "time_steps": n,
             "block": cfg["block"], "walkers_per_thread": cfg["wpt"], "grid": cfg["grid"],
             "tiles": cfg["tiles"], "autotune": tune, "create_s": t_create,
+            "lds_budget": args.lds_budget or "default",
         },
         "roofline": {
             "bound": "valu",
