@@ -543,6 +543,40 @@ Sampled<F> sample(std::vector<F>& fs, int n, double dt) {
     return S;
 }
 
+// The reference's lattices list periodic copies of a primitive explicitly
+// (all eight corner spheres of a cubic cell, say: its lattice test only sees
+// the basis it is given); the engine builds periodic images itself, so
+// copies that coincide modulo the cell (same type and radius, centres equal
+// to 1e-9 of the cell under the minimum image; z ignored for cylinders)
+// are passed once.  map[i] = index of basis i's primitive among those kept.
+template <class B>
+std::vector<Prim> unique_modulo_cell(const B* basis, const Lattice& lat, std::vector<int>& map) {
+    std::vector<Prim> out;
+    const int n = lat.getBasisSize();
+    map.assign(n, -1);
+    const real L[3] = {lat.getA(), lat.getB(), lat.getC()};
+    const real tol = 1e-9 * std::max(L[0], std::max(L[1], L[2]));
+    for (int i = 0; i < n; ++i) {
+        const Prim& p = basis[i].p;
+        for (int k = 0; k < (int)out.size() && map[i] < 0; ++k) {
+            const Prim& q = out[k];
+            if (p.type != q.type || p.type == AXX_PRIM_PLANE || std::fabs(p.R - q.R) > tol) continue;
+            const real d[3] = {p.cx - q.cx, p.cy - q.cy, p.type == AXX_PRIM_CYLINDER_XY ? 0.0 : p.cz - q.cz};
+            real r2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                const real m = d[a] - L[a] * std::nearbyint(d[a] / L[a]);
+                r2 += m * m;
+            }
+            if (r2 <= tol * tol) map[i] = k;
+        }
+        if (map[i] < 0) {
+            map[i] = (int)out.size();
+            out.push_back(p);
+        }
+    }
+    return out;
+}
+
 inline void check(int rc, const char* what) {
     if (rc) {
         std::string msg = std::string(what) + ": " + axx_strerror(rc) + " — " + axx_last_error(context());
@@ -565,9 +599,9 @@ class phaseAcquisitionStream {
     template <class B>
     void runAcquisitionLattice(int num, B* basis, Lattice& lat, real dt, int blocks, int threads, int numOfSM) {
         (void)blocks; (void)threads; (void)numOfSM;
-        std::vector<axx_compat::Prim> prims;
-        for (int i = 0; i < lat.getBasisSize(); ++i) prims.push_back(basis[i].p);
-        run(num, prims, lat, dt, init_mode(), init_arg());
+        std::vector<int> map;
+        const std::vector<axx_compat::Prim> prims = axx_compat::unique_modulo_cell(basis, lat, map);
+        run(num, prims, lat, dt, init_mode(), init_arg(map));
     }
     // Single pore: walkers start inside basis (phaseAcquisitionStream.cuh:37-88);
     // the pore sits in a cell three radii wider than it.
@@ -611,9 +645,9 @@ class phaseAcquisitionStream {
     void runAcquisitionStreamLattice(B* basis, Lattice& lat, real dt, int blocks, int threads, int numOfDevices,
                                      std::vector<int>& plan, std::vector<int>& numOfSMPerDevice) {
         (void)blocks; (void)threads; (void)numOfDevices; (void)plan; (void)numOfSMPerDevice;
-        std::vector<axx_compat::Prim> prims;
-        for (int i = 0; i < lat.getBasisSize(); ++i) prims.push_back(basis[i].p);
-        run_all(prims, lat, dt, init_mode(), init_arg());
+        std::vector<int> map;
+        const std::vector<axx_compat::Prim> prims = axx_compat::unique_modulo_cell(basis, lat, map);
+        run_all(prims, lat, dt, init_mode(), init_arg(map));
     }
     void changeSeeds(long s) {
         for (auto& a : acqSet) a.getSeed() = (int)s;
@@ -633,10 +667,14 @@ class phaseAcquisitionStream {
         return AXX_INIT_UNIFORM;
 #endif
     }
-    static int init_arg() {
+    // basis index of the starting primitive, renumbered by map when periodic
+    // copies were merged
+    static int init_arg(const std::vector<int>& map = std::vector<int>()) {
 #if defined USE_INITIALIZE_IN_REGION && USE_INITIALIZE_IN_REGION > 0
-        return USE_INITIALIZE_IN_REGION - 1;
+        const int k = USE_INITIALIZE_IN_REGION - 1;
+        return k < (int)map.size() ? map[k] : k;
 #else
+        (void)map;
         return 0;
 #endif
     }

@@ -104,3 +104,29 @@ def test_maglf_equals_direct_magnetization_runs():
         # to ~1e-5 relative although the walkers are the same
         assert np.isclose(rows[i][1], np.mean(adcs), rtol=1e-4, atol=0.0), (i, rows[i], adcs)
         assert np.isclose(rows[i][2], np.std(adcs), rtol=1e-2, atol=0.0), (i, rows[i], adcs)
+
+
+def test_cos_onesphere_merges_periodic_copies():
+    """The FCC driver lists 14 spheres (corners and face centres); the
+    compat layer passes the four distinct ones, and the signals equal direct
+    runs of those four."""
+    from axonexa_amd import engine
+    N, frac, seed = 4096, 0.02, 1234
+    lines, rows = _run("cos_onesphere", N, frac, seed)
+    assert rows.shape == (300, 3)
+    R, f = .0015, .4
+    a = ((16. / 3.) * math.pi * R ** 3 / f) ** (1. / 3.)
+    centers = [[0, 0, 0], [0.5 * a, 0, 0.5 * a], [a, 0.5 * a, 0.5 * a], [0.5 * a, 0.5 * a, a]]
+    sub = abi.SubstrateSpec(a=a, b=a, c=a, D_lat=2.5e-6, T2_lat=0.0, types=[abi.PRIM_SPHERE] * 4,
+                            centers=centers, radii=[R] * 4, D=1e-6, T2=0.0, kappa=0.0)
+    steps = max(2, int(42.0 / 0.001 * frac))
+    for j in (0, 7):
+        n_per = 1 + 10 * j
+        fs = [wf.CosGFunc(i * 0.0000025 * n_per, 20.0, 2.0, n_per, (1, 0, 0)) for i in range(30)]
+        acq = wf.build_acq(fs, steps, 0.001, 267500.0)
+        sig = engine.run_phase(sub, acq, abi.RunSpec(n_walkers=N, seed=j * seed, instance=j,
+                                                     init_mode=abi.INIT_COMPARTMENT, init_arg=0))
+        want = np.hypot(sig.sum_cos, sig.sum_sin)
+        got = rows[30 * j:30 * (j + 1), 2]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-9), (j, np.max(np.abs(got - want)))
+        assert sig.n_intra == 0 and sig.collisions > 0
