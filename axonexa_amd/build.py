@@ -66,7 +66,7 @@ def build_lib(verbose: bool = False) -> str:
     return lib
 
 
-DRIVERS = ["cos_onecyl", "gammadist", "wccompare", "maglf", "cos_onesphere"]
+DRIVERS = ["cos_onecyl", "gammadist", "wccompare", "maglf", "cos_onesphere", "hex_phase", "plane_phase"]
 
 
 def build_drivers() -> list:

@@ -548,7 +548,9 @@ Sampled<F> sample(std::vector<F>& fs, int n, double dt) {
 // the basis it is given); the engine builds periodic images itself, so
 // copies that coincide modulo the cell (same type and radius, centres equal
 // to 1e-9 of the cell under the minimum image; z ignored for cylinders)
-// are passed once.  map[i] = index of basis i's primitive among those kept.
+// are passed once; so are planes that coincide up to a cell translation
+// (a slab bounded by x = 0 and x = l in a cell of length l is one plane).
+// map[i] = index of basis i's primitive among those kept.
 template <class B>
 std::vector<Prim> unique_modulo_cell(const B* basis, const Lattice& lat, std::vector<int>& map) {
     std::vector<Prim> out;
@@ -560,7 +562,26 @@ std::vector<Prim> unique_modulo_cell(const B* basis, const Lattice& lat, std::ve
         const Prim& p = basis[i].p;
         for (int k = 0; k < (int)out.size() && map[i] < 0; ++k) {
             const Prim& q = out[k];
-            if (p.type != q.type || p.type == AXX_PRIM_PLANE || std::fabs(p.R - q.R) > tol) continue;
+            if (p.type != q.type) continue;
+            if (p.type == AXX_PRIM_PLANE) {
+                // parallel normals (either sense) and no offset along them
+                // up to a cell translation
+                const real lp = std::sqrt(p.nx * p.nx + p.ny * p.ny + p.nz * p.nz);
+                const real lq = std::sqrt(q.nx * q.nx + q.ny * q.ny + q.nz * q.nz);
+                if (!(lp > 0 && lq > 0) ||
+                    std::fabs((p.nx * q.nx + p.ny * q.ny + p.nz * q.nz) / (lp * lq)) < 1.0 - 1e-12)
+                    continue;
+                const real d[3] = {p.cx - q.cx, p.cy - q.cy, p.cz - q.cz};
+                for (int sx = -1; sx <= 1 && map[i] < 0; ++sx)
+                    for (int sy = -1; sy <= 1 && map[i] < 0; ++sy)
+                        for (int sz = -1; sz <= 1 && map[i] < 0; ++sz) {
+                            const real off = ((d[0] - sx * L[0]) * q.nx + (d[1] - sy * L[1]) * q.ny +
+                                              (d[2] - sz * L[2]) * q.nz) / lq;
+                            if (std::fabs(off) <= tol) map[i] = k;
+                        }
+                continue;
+            }
+            if (std::fabs(p.R - q.R) > tol) continue;
             const real d[3] = {p.cx - q.cx, p.cy - q.cy, p.type == AXX_PRIM_CYLINDER_XY ? 0.0 : p.cz - q.cz};
             real r2 = 0;
             for (int a = 0; a < 3; ++a) {
@@ -649,8 +670,9 @@ class phaseAcquisitionStream {
         const std::vector<axx_compat::Prim> prims = axx_compat::unique_modulo_cell(basis, lat, map);
         run_all(prims, lat, dt, init_mode(), init_arg(map));
     }
+    // acquisition i gets seed s (i + 1) (phaseAcquisitionStream.cuh:623-627)
     void changeSeeds(long s) {
-        for (auto& a : acqSet) a.getSeed() = (int)s;
+        for (int i = 0; i < (int)acqSet.size(); ++i) acqSet[i].getSeed() = (int)(s * (i + 1));
     }
     void calcEveryADC() { for (auto& a : acqSet) a.calcADC(); }
     void flushADC() { for (auto& a : acqSet) { a.ADC = 0; a.ADCmeansquare = 0; a.points = 0; } }

@@ -36,6 +36,16 @@ int main() {
     std::printf("%zu", u2.size());
     for (int m : map) std::printf(" %d", m);
     std::printf("\n");
+    // planes: x = 0 and x = a (opposite normals) are one plane; x = a/2 is another
+    std::vector<Plane> pl(3);
+    pl[0] = Plane(Vector3(0, 0, 0), Vector3(1, 0, 0));
+    pl[1] = Plane(Vector3(a, 0.3, 0), Vector3(-1, 0, 0));
+    pl[2] = Plane(Vector3(a / 2, 0, 0), Vector3(1, 0, 0));
+    Lattice l3(a, a, a, 0, 0, 1, 3);
+    auto u3 = axx_compat::unique_modulo_cell(&pl[0], l3, map);
+    std::printf("%zu", u3.size());
+    for (int m : map) std::printf(" %d", m);
+    std::printf("\n");
     return 0;
 }
 '''
@@ -55,3 +65,5 @@ def test_unique_modulo_cell():
     assert fcc[1:] == [0, 0, 0, 0, 0, 0, 0, 0, 1, 2, 1, 2, 3, 3]
     cyl = [int(x) for x in out[1].split()]
     assert cyl == [3, 0, 0, 1, 2]
+    pl = [int(x) for x in out[2].split()]
+    assert pl == [2, 0, 0, 1]

@@ -130,3 +130,73 @@ def test_cos_onesphere_merges_periodic_copies():
         got = rows[30 * j:30 * (j + 1), 2]
         assert np.allclose(got, want, rtol=1e-12, atol=1e-9), (j, np.max(np.abs(got - want)))
         assert sig.n_intra == 0 and sig.collisions > 0
+
+
+def _blocks(lines, tag):
+    """Numeric rows after each line containing tag."""
+    out, cur = [], None
+    for l in lines:
+        if tag in l:
+            cur = []
+            out.append(cur)
+        elif cur is not None and l.strip():
+            cur.append([float(x) for x in l.split()])
+    return [np.array(b) for b in out]
+
+
+def test_hex_phase_equals_direct_runs():
+    """Four listed cylinders, two distinct: the driver's signals equal direct
+    runs of the two."""
+    from axonexa_amd import engine
+    N, frac, seed = 2048, 0.05, 1234
+    print(f"[gpu] driver hex_phase {N} {frac} {seed}", flush=True)
+    out = subprocess.run([os.path.join(BIN, "hex_phase"), str(N), str(frac), str(seed)], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    blocks = _blocks(out.stdout.splitlines(), "f = ")
+    assert len(blocks) == 4 and all(b.shape == (6, 3) for b in blocks)
+    s3 = 0.86602540378443864676372317075294
+    R, f = .0025, .6
+    d = math.sqrt(2.0 * math.pi * R * R / (math.sqrt(3.0) * f))
+    sub = abi.SubstrateSpec(a=d, b=2.0 * d * s3, c=d, D_lat=2.5e-6, T2_lat=200.0, types=[0, 0],
+                            centers=[[d / 2.0, 0.0, 0.0], [0.0, d * s3, 0.0]], radii=[R, R], D=1e-6, T2=200.0,
+                            kappa=0.0)
+    steps = max(2, int(22.0 / 0.001 * frac))
+    for j in range(3):
+        fs = [wf.CosGFunc(i * 0.0000025 * (1 + j), 10.0, 2.0, 1 + j, (1, 0, 0)) for i in range(2)]
+        acq = wf.build_acq(fs, steps, 0.001, 267500.0)
+        sig = engine.run_phase(sub, acq, abi.RunSpec(n_walkers=N, seed=j * seed, instance=j))
+        want = np.hypot(sig.sum_cos, sig.sum_sin)
+        assert np.allclose(blocks[0][2 * j:2 * j + 2, 2], want, rtol=1e-12, atol=1e-9), j
+
+
+def test_plane_phase_equals_direct_runs():
+    """A slab bounded by x = 0 and x = l (one plane under periodicity): the
+    driver's replica-averaged ADC equals direct runs with the same seeds."""
+    from axonexa_amd import engine
+    from axonexa_amd import analysis
+    N, frac, seed = 2048, 0.05, 1234
+    print(f"[gpu] driver plane_phase {N} {frac} {seed}", flush=True)
+    out = subprocess.run([os.path.join(BIN, "plane_phase"), str(N), str(frac), str(seed)], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    blocks = _blocks(out.stdout.splitlines(), "PLANE LENGTH")
+    assert len(blocks) == 4 and all(b.shape == (10, 2) for b in blocks)
+    # the first length (seeds carry over from one length to the next)
+    l = .001
+    sub = abi.SubstrateSpec(a=l, b=l, c=l, D_lat=2.5e-6, T2_lat=200.0, types=[abi.PRIM_PLANE],
+                            centers=[[0.0, 0.0, 0.0]], radii=[0.0], D=0.0, T2=200.0, kappa=0.0,
+                            normals=[[1.0, 0.0, 0.0]])
+    steps = max(2, int(22.0 / 0.001 * frac))
+    for j in (0, 6):
+        fs = [wf.CosGFunc(i * 0.0000025 * (1 + j), 10.0, 2.0, 1 + j, (1, 0, 0)) for i in range(2)]
+        acq = wf.build_acq(fs, steps, 0.001, 267500.0)
+        adcs = []
+        for r in range(3):
+            sd = j * seed if r == 0 else seed * r * (j + 1)
+            sig = engine.run_phase(sub, acq, abi.RunSpec(n_walkers=N, seed=sd, instance=j))
+            adcs.append(analysis.adc_fit([x.getB(267500.0) for x in fs], np.hypot(sig.sum_cos, sig.sum_sin)))
+        assert np.isclose(blocks[0][j, 1], np.mean(adcs), rtol=1e-6, atol=0.0), (j, blocks[0][j], adcs)
+    # the slab restricts diffusion along x: the ADC of the narrowest slab at
+    # the lowest frequency is well below D
+    assert blocks[0][0, 1] < 0.5 * 2.5e-6
