@@ -912,6 +912,67 @@ extern "C" int axx_plan_set_config(axx_plan* p, int block, int wpt) {
 
 extern "C" int axx_plan_profile_count(const axx_plan* p) { return p ? p->P : AXX_ERR_INVALID; }
 
+// Unwrapped positions of the traced walkers into out[(j * n_steps + i) * 3].
+static __global__ void trace_gather_kernel(const uint32_t* __restrict__ state, long long N,
+                                           const uint32_t* __restrict__ ids, int k, int n_steps, int i,
+                                           float* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    const float* fs = reinterpret_cast<const float*>(state);
+    const size_t w = ids[j];
+    for (int a = 0; a < 3; ++a)
+        out[((size_t)j * n_steps + i) * 3 + a] = fs[(size_t)(F_X0 + a) * N + w] + fs[(size_t)(F_DX + a) * N + w];
+}
+
+extern "C" int axx_trace(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
+                         const uint32_t* walker_ids, int k, float* xyz, uint32_t* counts) {
+    try {
+        if (!ctx || !s || !q || !r || !walker_ids || !xyz || k < 1) return fail(ctx, AXX_ERR_INVALID, "trace: bad arguments");
+        for (int j = 0; j < k; ++j)
+            if ((long long)walker_ids[j] >= r->n_walkers) return fail(ctx, AXX_ERR_INVALID, "trace: walker id %u", walker_ids[j]);
+        axx_plan* p = nullptr;
+        int rc = axx_plan_create(ctx, s, q, r, &p);
+        if (rc) return rc;
+        uint32_t* d_ids = nullptr;
+        float* d_out = nullptr;
+        const size_t n_out = (size_t)k * p->n_steps * 3;
+        if (hipMalloc((void**)&d_ids, (size_t)k * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc((void**)&d_out, n_out * sizeof(float)) != hipSuccess ||
+            hipMemcpy(d_ids, walker_ids, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+            if (d_ids) (void)hipFree(d_ids);
+            if (d_out) (void)hipFree(d_out);
+            axx_plan_destroy(p);
+            return fail(ctx, AXX_ERR_NOMEM, "trace: device buffers");
+        }
+        const int blocks = (k + 255) / 256;
+        for (int i = 0; !rc && i < p->n_steps; ++i) {
+            rc = axx_plan_walk(p, i == 0 ? 0 : i, i + 1);
+            if (!rc) {
+                hipLaunchKernelGGL(trace_gather_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                                   (const uint32_t*)p->d_state, p->N, (const uint32_t*)d_ids, k, p->n_steps, i, d_out);
+                if (hipGetLastError() != hipSuccess) rc = fail(ctx, AXX_ERR_DEVICE, "trace: gather launch");
+            }
+        }
+        if (!rc && hipMemcpy(xyz, d_out, n_out * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ctx, AXX_ERR_DEVICE, "trace: copy back");
+        if (!rc && counts) {
+            std::vector<uint32_t> st((size_t)AXX_STATE_FIELDS(p->P) * (size_t)p->N);
+            rc = axx_plan_get_state(p, st.data());
+            for (int j = 0; !rc && j < k; ++j) {
+                counts[j * 3 + 0] = st[(size_t)F_NCOLL * p->N + walker_ids[j]];
+                counts[j * 3 + 1] = st[(size_t)F_NTRAN * p->N + walker_ids[j]];
+                counts[j * 3 + 2] = st[(size_t)F_NCAP * p->N + walker_ids[j]];
+            }
+        }
+        (void)hipFree(d_ids);
+        (void)hipFree(d_out);
+        axx_plan_destroy(p);
+        return rc;
+    } catch (...) {
+        return api_exception();
+    }
+}
+
 extern "C" int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
                              axx_result* out) {
     try {

@@ -53,6 +53,8 @@ def lib():
     S, Q, R, Res = (C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                     C.POINTER(abi.Result))
     L.axx_run_phase.argtypes = [vp, S, Q, R, Res]
+    L.axx_trace.argtypes = [vp, S, Q, R, C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_float),
+                            C.POINTER(C.c_uint32)]
     L.axx_plan_create.argtypes = [vp, S, Q, R, C.POINTER(vp)]
     L.axx_plan_create_ex.argtypes = [vp, S, Q, R, C.POINTER(abi.Tuning), C.POINTER(vp)]
     L.axx_plan_walk.argtypes = [vp, C.c_int, C.c_int]
@@ -331,6 +333,24 @@ def run_mag(sub: abi.SubstrateSpec, field: abi.FieldSpec, run: abi.RunSpec,
     res, bufs = _mag_result(field.n_fields)
     ctx.check(lib().axx_run_mag(ctx.handle, C.byref(s), C.byref(f), C.byref(r), C.byref(res)), "axx_run_mag")
     return _mag_signals(res, bufs)
+
+
+def trace(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec, walker_ids,
+          ctx: Optional[Context] = None):
+    """Positions of the chosen walkers after every step (axx_trace):
+    xyz [k, n_steps, 3] (start + displacement), counts [k, 3] (collisions,
+    transmissions, bounce-cap hits)."""
+    ctx = ctx or default_context()
+    ids = np.ascontiguousarray(walker_ids, dtype=np.uint32)
+    k = ids.size
+    xyz = np.zeros((k, acq.n_steps, 3), np.float32)
+    counts = np.zeros((k, 3), np.uint32)
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    ctx.check(lib().axx_trace(ctx.handle, C.byref(s), C.byref(q), C.byref(r),
+                              ids.ctypes.data_as(C.POINTER(C.c_uint32)), k,
+                              xyz.ctypes.data_as(C.POINTER(C.c_float)),
+                              counts.ctypes.data_as(C.POINTER(C.c_uint32))), "axx_trace")
+    return xyz, counts
 
 
 class MagBatch:

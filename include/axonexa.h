@@ -194,6 +194,19 @@ int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
                   const axx_run* run, axx_result* out);
 
 /*
+ * Trace (parity hook): the walk of axx_run_phase, step by step, with the
+ * unwrapped positions (start + displacement, fp32) of k chosen walkers after
+ * every step: xyz[(j * n_steps + i) * 3 + a] for walker walker_ids[j]
+ * (index within the run) after step i (i = 0: the start), and counts[j * 3 +
+ * 0..2] = its collisions, transmissions and bounce-cap hits at the end.  The
+ * same kernel walks all run.n_walkers walkers (one launch per step), so the
+ * traced walkers are those of the full run.  No reference counterpart: the
+ * reference offers no way to observe a walker.
+ */
+int axx_trace(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq, const axx_run* run,
+              const uint32_t* walker_ids, int k, float* xyz, uint32_t* counts);
+
+/*
  * Plan API: the same run split into its parts so that the device-resident
  * walk can be timed, chunked (checkpoint/resume) and inspected.
  *   create  : validate, build periodic images + cell bins, upload tables,

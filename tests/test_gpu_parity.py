@@ -309,3 +309,27 @@ def test_coincident_planes_are_a_geometry_error():
     # a parallel plane elsewhere is fine
     sub.centers[1] = [a / 2, 3.0, 1.0]
     assert eng.run_phase(sub, acq, abi.RunSpec(n_walkers=64, seed=1)).collisions >= 0
+
+
+def test_trace_follows_the_oracle_step_by_step():
+    """axx_trace: the chosen walkers' positions after every step equal the
+    oracle's, walked one step at a time."""
+    eng = _engine()
+    dt = 2e-3
+    sub = sb.hex_lattice(2.5, 6.0, 3, 2, D_in=1.7, D_ex=2.0, kappa=sb.kappa_for_probability(0.05, 1.7, dt))
+    acq = _pgse(200, dt)
+    run = abi.RunSpec(n_walkers=512, seed=31, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    ids = np.array([0, 1, 77, 255, 300, 511], np.uint32)
+    xyz, counts = eng.trace(sub, acq, run, ids)
+    st = po.walk(sub, acq, run, step_begin=0, step_end=1)
+    n = run.n_walkers
+    for i in range(acq.n_steps):
+        if i > 0:
+            st = po.walk(sub, acq, run, step_begin=i, step_end=i + 1, state=st)
+        v = abi.state_view(st, n, acq.n_profiles)
+        want = (v["r0"][ids] + v["disp"][ids]).astype(np.float32)
+        assert np.array_equal(xyz[:, i, :], want), i
+    v = abi.state_view(st, n, acq.n_profiles)
+    assert np.array_equal(counts[:, 0], v["ncoll"][ids]) and np.array_equal(counts[:, 1], v["ntran"][ids])
+    assert counts[:, 0].sum() > 0
+    print(f"[parity] trace of {ids.size} walkers x {acq.n_steps} steps equals the oracle", flush=True)
