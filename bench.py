@@ -221,6 +221,23 @@ def main(argv=None) -> int:
             "seconds": tcpu,
         }
         out["vs_cpu"] = value / cpu_rate
+        # the reference-faithful variant: every step scans every primitive
+        # image (lattice.cuh:184-199), on a shorter sample
+        lw = min(256, wl.run.n_walkers)
+        ln = min(n, 20000)
+        lin = make(**dict(kw, n_walkers=lw, n_steps=ln))
+        tl = time.perf_counter()
+        po.walk(lin.sub, lin.acq, lin.run, threads=threads, bin_width=0.0)
+        tlin = time.perf_counter() - tl
+        out["cpu_baseline_linear_scan"] = {
+            "value": lin.walker_steps / tlin,
+            "unit": "walker-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{lw} walkers x {ln} steps of the same workload, oracle scanning all primitive images "
+                      f"every step, {threads} threads",
+            "seconds": tlin,
+        }
         out["signal_error_vs_oracle"] = {
             "sample_walkers": nw,
             "bitwise_equal_walkers": float(bit.mean()),
