@@ -333,3 +333,23 @@ def test_trace_follows_the_oracle_step_by_step():
     assert np.array_equal(counts[:, 0], v["ncoll"][ids]) and np.array_equal(counts[:, 1], v["ntran"][ids])
     assert counts[:, 0].sum() > 0
     print(f"[parity] trace of {ids.size} walkers x {acq.n_steps} steps equals the oracle", flush=True)
+
+
+@pytest.mark.parametrize("n_steps", [1, 2, 3])
+def test_shortest_walks(n_steps):
+    """One time point (placement only), one step, two steps: the walker state
+    and sums equal the oracle's (with one time point the phase is the
+    gradient at t = 0 on the start position; the G = 0 waveform gives N)."""
+    eng = _engine()
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
+    acq = _pgse(n_steps, 1e-3)
+    run = abi.RunSpec(n_walkers=300, seed=5)
+    sig, _ = _run_both(f"short_{n_steps}", sub, acq, run)
+    assert sig.sum_cos[0] == run.n_walkers and sig.sum_sin[0] == 0.0
+    # the magnetization path at the same lengths
+    from axonexa_amd import fields as fl
+    f = fl.build_field([fl.QSC_GRE(1e-3, 1e-4, 0.002, 0.001)], n_steps, 1e-3, gamma=GAMMA,
+                       origin=(10.0, 10.0, 10.0))
+    with eng.Mag(sub, f, run) as m:
+        m.walk()
+        assert np.array_equal(m.get_state(0), po.walk_mag(sub, f, run))
