@@ -112,31 +112,6 @@ static void box_proj(const double n[3], const double lo[3], const double hi[3], 
     }
 }
 
-// Planes k and l (normals checked non-zero) are the same plane up to a cell
-// translation: parallel normals and an offset between their points, less
-// some lattice vector in {-1, 0, 1}^3 cells, with no component along the
-// normal (to 1e-9 of the cell).
-static bool planes_coincide(const axx_substrate* s, int l, int k) {
-    const double* nl = s->normal + 3 * l;
-    const double* nk = s->normal + 3 * k;
-    const double ll = std::sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
-    const double lk = std::sqrt(nk[0] * nk[0] + nk[1] * nk[1] + nk[2] * nk[2]);
-    const double cosang = (nl[0] * nk[0] + nl[1] * nk[1] + nl[2] * nk[2]) / (ll * lk);
-    if (std::fabs(cosang) < 1.0 - 1e-12) return false;
-    const double L[3] = {s->a, s->b, s->c};
-    const double tol = 1e-9 * std::max(s->a, std::max(s->b, s->c));
-    for (int i = -1; i <= 1; ++i)
-        for (int j = -1; j <= 1; ++j)
-            for (int m = -1; m <= 1; ++m) {
-                const int sh[3] = {i, j, m};
-                double d = 0;
-                for (int a = 0; a < 3; ++a)
-                    d += (s->center[3 * k + a] - s->center[3 * l + a] - sh[a] * L[a]) * nl[a] / ll;
-                if (std::fabs(d) <= tol) return true;
-            }
-    return false;
-}
-
 static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     if (!(s->a > 0 && s->b > 0 && s->c > 0) || !std::isfinite(s->a + s->b + s->c))
         return fail(ctx, AXX_ERR_INVALID, "cell lengths must be finite and > 0");
@@ -172,11 +147,6 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
             const double* v = s->normal + 3 * k;
             if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0) || !std::isfinite(v[0] + v[1] + v[2]))
                 return fail(ctx, AXX_ERR_INVALID, "primitive %d: plane normal", k);
-            // a plane that coincides with a periodic image of an earlier one
-            // would reflect a walker twice at the same place
-            for (int l = 0; l < k; ++l)
-                if (s->type[l] == AXX_PRIM_PLANE && planes_coincide(s, l, k))
-                    return fail(ctx, AXX_ERR_GEOMETRY, "plane %d coincides with plane %d under periodicity", k, l);
             continue;
         }
         if (s->type[k] != AXX_PRIM_CYLINDER_XY && s->type[k] != AXX_PRIM_SPHERE)
