@@ -22,6 +22,7 @@
 
 #include <cstdlib>
 #include <map>
+#include <mutex>
 
 namespace axx {
 
@@ -158,9 +159,16 @@ struct axx_batch {
 
 namespace {
 
+// Autotuner choices by batch signature, shared by every context of the
+// process (guarded by tune_mutex; the file named by AXX_TUNE_CACHE is read
+// once and appended to).
 std::map<std::string, std::pair<int, int>>& tune_cache() {
     static std::map<std::string, std::pair<int, int>> m;
     return m;
+}
+std::mutex& tune_mutex() {
+    static std::mutex mu;
+    return mu;
 }
 
 int class_of(const axx_plan* p) {
@@ -283,6 +291,12 @@ int capture(axx_batch* b) {
     return AXX_OK;
 }
 
+bool valid_config(int blk, int wpt) {
+    return blk >= 64 && blk <= kBatchBlockMax && blk % 64 == 0 && wpt >= 1 && wpt <= 64;
+}
+
+// call with tune_mutex held; entries that are not a valid configuration
+// (a damaged file) are skipped
 void load_file_cache() {
     static bool loaded = false;
     if (loaded) return;
@@ -293,11 +307,24 @@ void load_file_cache() {
     if (!f) return;
     char key[4096];
     int blk, wpt;
-    while (std::fscanf(f, "%4095s %d %d", key, &blk, &wpt) == 3) tune_cache()[key] = {blk, wpt};
+    while (std::fscanf(f, "%4095s %d %d", key, &blk, &wpt) == 3)
+        if (valid_config(blk, wpt)) tune_cache()[key] = {blk, wpt};
     std::fclose(f);
 }
 
-void store_file_cache(const std::string& key, int blk, int wpt) {
+bool cache_lookup(const std::string& key, int& blk, int& wpt) {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    load_file_cache();
+    auto it = tune_cache().find(key);
+    if (it == tune_cache().end()) return false;
+    blk = it->second.first;
+    wpt = it->second.second;
+    return true;
+}
+
+void cache_store(const std::string& key, int blk, int wpt) {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    tune_cache()[key] = {blk, wpt};
     const char* path = std::getenv("AXX_TUNE_CACHE");
     if (!path) return;
     FILE* f = std::fopen(path, "a");
@@ -411,10 +438,8 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
         }
         // configuration: explicit, cached, or a default; the autotuner may change it
         int blk = 256, wpt = 1;
-        load_file_cache();
-        auto it = tune_cache().find(b->signature);
         if (tune && tune->block) blk = tune->block;
-        else if (it != tune_cache().end()) { blk = it->second.first; wpt = it->second.second; }
+        else cache_lookup(b->signature, blk, wpt);
         if (blk < 64 || blk > kBatchBlockMax || blk % 64) {
             batch_free(b);
             return fail(ctx, AXX_ERR_INVALID, "batch block must be a multiple of 64 in [64, %d]", kBatchBlockMax);
@@ -559,8 +584,7 @@ extern "C" int axx_batch_autotune(axx_batch* b, int probe_steps, int* best_block
         if ((rc = build_tiles(b, bb, bw)) || (rc = capture(b))) return rc;
         for (axx_plan* p : b->plans) p->reduced = false;   // the probe overwrote the walker state
         b->ran = false;
-        tune_cache()[b->signature] = {bb, bw};
-        store_file_cache(b->signature, bb, bw);
+        cache_store(b->signature, bb, bw);
         if (best_block) *best_block = bb;
         if (best_wpt) *best_wpt = bw;
         if (best_ms) *best_ms = best;
