@@ -209,8 +209,10 @@ __device__ __forceinline__ Quad quad_of(float ox, float oy, float oz, float ux, 
 
 __device__ __forceinline__ bool quad_hits(const Quad& Q, float ell) {
     const bool hit_in = Q.fe > 0.0f;
-    const bool hit_out = Q.B < 0.0f && (Q.Cp == 0.0f || Q.fe < 0.0f || ((-Q.B) < Q.A * ell && Q.disc > 0.0f));
-    return Q.A > 0.0f && (Q.inside ? hit_in : hit_out);
+    // bitwise, not short-circuit: every operand is a compare, and evaluating
+    // them all avoids the exec-mask branches the && / || chain compiles to
+    const bool hit_out = (Q.B < 0.0f) & ((Q.Cp == 0.0f) | (Q.fe < 0.0f) | (((-Q.B) < Q.A * ell) & (Q.disc > 0.0f)));
+    return (Q.A > 0.0f) & (Q.inside ? hit_in : hit_out);
 }
 
 // inside, B < 0: (sq - B) / A;  inside, B >= 0: -Cp / (B + sq);
