@@ -441,8 +441,21 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     ncap++;
                     ell = 0.0f;
                     live = false;
-                } else if (nrel == 1 && excl >= 0) {
-                    live = false;   // the only surface within reach was just left
+                } else if (nrel == 1) {
+                    if (excl >= 0) {
+                        live = false;   // the only surface within reach was just left
+                    } else {
+                        // the only surface within reach, from inside: decide here
+                        // whether the rest of the step hits it again, with the
+                        // operands the next pass's test would use, so a wave
+                        // whose lanes all miss leaves the loop without that pass
+                        const bool same2 = comp == pcomp;
+                        const Quad Q2 = sk ? quad_of<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w * g.w,
+                                                           same2, A.H2s, ell)
+                                           : quad_of<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w,
+                                                            same2, A.H2c, ell);
+                        live = quad_hits(Q2, ell);
+                    }
                 }
             }
         }
