@@ -240,6 +240,11 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     const int cap = A.cap;
     const bool permeable = A.mode == 1;
+    // transmission draws: boundary event n of the walk takes word n mod 4 of
+    // the Philox call at (wid, n / 4, kStreamColl, inst); the last call's
+    // words stay here so one call serves four events (block 0xFFFFFFFF: none)
+    uint32_t cblk = 0xFFFFFFFFu;
+    u32x4 cw = {0u, 0u, 0u, 0u};
     float Lc = A.compL[comp];
 
     // Surfaces within reach of the current start point: the bin's list,
@@ -330,7 +335,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         } else {
             float Axy = fmaf(uy, uy, ux * ux);
             float A3 = fmaf(uz, uz, Axy);
-            int nb = 0, kc = 0;
+            int nb = 0;
             int excl = -1;   // image just left outward: a convex surface cannot be hit again
             bool live = nrel > 0;
             // no lane of the wave has two or more candidates: no list loop
@@ -389,13 +394,15 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 bool transmit = false;
                 if (permeable && !pl) {     // planes never transmit (plane.cuh:39)
                     // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
-                    const u32x4 oc = philox10(wid, (uint32_t)i, kStreamColl | ((uint32_t)kc >> 2), A.inst,
-                                              A.k0, A.k1);
-                    const uint32_t sel = (uint32_t)kc & 3u;
-                    const uint32_t wv = sel == 0 ? oc.x : sel == 1 ? oc.y : sel == 2 ? oc.z : oc.w;
+                    const uint32_t n = ncoll - 1u;   // this event's ordinal in the walk
+                    if ((n >> 2) != cblk) {
+                        cw = philox10(wid, n >> 2, kStreamColl, A.inst, A.k0, A.k1);
+                        cblk = n >> 2;
+                    }
+                    const uint32_t sel = n & 3u;
+                    const uint32_t wv = sel == 0 ? cw.x : sel == 1 ? cw.y : sel == 2 ? cw.z : cw.w;
                     const float thr = ins ? A.thr_out[pcomp - 1] : A.thr_in[pcomp - 1];
                     transmit = unif(wv) <= thr;
-                    kc++;
                 }
                 lastpl = pl ? bk : -1;
                 // left outward (outside reflection, transmission from inside,

@@ -974,7 +974,7 @@ static void* job_run(void* arg) {
             } else {
                 float Axy = fmaf(uy, uy, ux * ux);
                 float A3 = fmaf(uz, uz, Axy);
-                int nb = 0, kc = 0;
+                int nb = 0;
                 int excl = -1;   /* image just left outward: a convex surface
                                   * (or a plane) cannot be hit again */
                 lastplane = -1;
@@ -1006,12 +1006,15 @@ static void* job_run(void* arg) {
                     int inside = bins_;
                     int transmit = 0;
                     if (mode == AXX_BOUNDARY_PERMEABLE && pk->type != AXX_PRIM_PLANE) {
+                        /* boundary event n of the walk (n counts every event,
+                         * planes and earlier launches included): word n mod 4
+                         * of the call at counter (wid, n / 4, 1 << 24, inst) */
                         uint32_t oc[4];
-                        draw(wid, (uint32_t)i, (STREAM_COLL << 24) | ((uint32_t)kc >> 2), inst, key, oc);
-                        float U = unif(oc[kc & 3]);
+                        const uint32_t n = ncoll - 1u;
+                        draw(wid, n >> 2, STREAM_COLL << 24, inst, key, oc);
+                        float U = unif(oc[n & 3u]);
                         float thr = inside ? P->thr_out[pk->orig] : P->thr_in[pk->orig];
                         transmit = U <= thr;
-                        kc++;
                     }
                     lastplane = pk->type == AXX_PRIM_PLANE ? bk : -1;
                     /* left outward: outside reflection, transmission from

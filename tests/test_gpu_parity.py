@@ -132,6 +132,18 @@ def test_chunked_walk_equals_oracle():
     _run_both("chunked", sub, acq, abi.RunSpec(n_walkers=2048, seed=8), chunks=[1, 2, 333, 600])
 
 
+def test_chunked_permeable_walk_equals_oracle():
+    # one Philox call serves four boundary events of a walker, numbered over
+    # the whole walk: launches that start inside a block of four must resume
+    # on the same words
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0,
+                             kappa=sb.kappa_for_probability(0.2, 2.0, 1e-2))
+    acq = _pgse(1000, 1e-2)
+    run = abi.RunSpec(n_walkers=2048, seed=9, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    sig, _ = _run_both("chunked_permeable", sub, acq, run, chunks=[1, 2, 3, 251, 252, 600])
+    assert sig.transmissions > 1000
+
+
 def test_bounce_cap_counted():
     sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
     acq = _pgse(800, 1e-2)

@@ -124,15 +124,23 @@ def test_binned_equals_linear_scan_and_threads(mode):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-def test_chunked_walk_is_bitwise_identical():
-    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0)
+@pytest.mark.parametrize("permeable", [False, True])
+def test_chunked_walk_is_bitwise_identical(permeable):
+    # permeable: transmission draws come four boundary events per Philox call,
+    # numbered over the whole walk, so a chunk may start inside a block of four
+    kappa = sb.kappa_for_probability(0.2, 2.0, 1e-2) if permeable else 0.0
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0, kappa=kappa)
     acq = W.c1_acq(n_steps=2000, dt=1e-2)
-    run = abi.RunSpec(n_walkers=512, seed=8)
+    mode = abi.BOUNDARY_PERMEABLE if permeable else abi.BOUNDARY_SPECULAR
+    run = abi.RunSpec(n_walkers=512, seed=8, boundary_mode=mode)
     one = po.walk(sub, acq, run, bin_width=1.0)
     st = po.walk(sub, acq, run, 0, 1, bin_width=1.0)
     for a, b in ((1, 2), (2, 999), (999, 1500), (1500, 2000)):
         st = po.walk(sub, acq, run, a, b, state=st, bin_width=1.0)
     assert np.array_equal(one, st)
+    v = abi.state_view(one, run.n_walkers, acq.n_profiles)
+    if permeable:
+        assert int(v["ntran"].sum()) > 1000
 
 
 def test_step_length_and_isometry_invariants():
