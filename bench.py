@@ -312,7 +312,9 @@ def bench_mag(args) -> int:
     }
     if not args.no_cpu:
         from oracle import pyoracle as po
-        nw = min(args.cpu_walkers, wl.run.n_walkers) // 4
+        # 28 x 32,768 walkers by default = the whole workload: about 5 s (2 fields)
+        # to 20 s (8 fields) of oracle time on 16 host threads
+        nw = min(28 * args.cpu_walkers, wl.run.n_walkers)
         sample = W.mag_lf(**dict(kw, n_walkers=nw))
         threads = _cpu_threads()
         tc = time.perf_counter()
