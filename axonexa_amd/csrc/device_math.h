@@ -29,9 +29,11 @@ __device__ __forceinline__ u32x4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
     for (int r = 0; r < 10; ++r) {
         uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
         uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
-        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        // three-way xor in one v_bitop3_b32 (truth table 0x96); the
+        // compiler otherwise emits two v_xor_b32
+        uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
         uint32_t n1 = (uint32_t)p1;
-        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         uint32_t n3 = (uint32_t)p0;
         c0 = n0; c1 = n1; c2 = n2; c3 = n3;
         k0 += 0x9E3779B9u;
@@ -47,16 +49,18 @@ constexpr uint32_t kStreamGauss = 3u << 24;   // extra words of Gaussian steps
 
 // Uniform in (0,1): odd multiples of 2^-24 (exactly representable).
 __device__ __forceinline__ float unif(uint32_t w) {
-    return (float)((w >> 8) | 1u) * 0x1p-24f;
+    // ((w >> 8) | 1) 2^-24 = (w >> 9) 2^-23 + 2^-24, exact either way
+    return fmaf((float)(w >> 9), 0x1p-23f, 0x1p-24f);
 }
 
 // sin and cos of 2*pi*u(w): integer quarter-turn reduction, Taylor
 // polynomials of sin(pi/2 r), cos(pi/2 r) for r in [-1/2, 1/2].
 __device__ __forceinline__ void sincos_turn(uint32_t w, float& s, float& c) {
-    uint32_t v = (w >> 8) | 1u;
-    uint32_t q = (v + (1u << 21)) >> 22;
-    int32_t ri = (int32_t)v - (int32_t)(q << 22);
-    float r = (float)ri * 0x1p-22f;
+    // v = u 2^24 + 2^21; quarter turn q = v >> 22 (0..4); r = u 4 - q in
+    // [-1/2, 1/2) from the low 22 bits of v, exact
+    const uint32_t v = ((w >> 8) | 1u) + (1u << 21);
+    const uint32_t q = v >> 22;
+    float r = fmaf((float)(v & 0x3FFFFFu), 0x1p-22f, -0.5f);
     float y = r * r;
     float ps = fmaf(y, 0.00016044118478735982f, -0.004681754135318687f);
     ps = fmaf(y, ps, 0.07969262624616704f);

@@ -15,6 +15,8 @@
 // the O(n_basis) scan, the segment form (start, direction, remaining length)
 // of the reflection loop, side-tracked crossing tests, and true T2 time split.
 #pragma once
+#include <type_traits>
+
 #include "device_math.h"
 
 namespace axx {
@@ -282,7 +284,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
         if (!ALT || A.step_mode == 0) {
             // isotropic direction (sphere point picking, kernelLattice.cuh:52-57)
             // cos(theta) = 2u - 1 and sin^2 = 1 - c^2, each in one rounding
-            const float ct = fmaf((float)((wb >> 8) | 1u), 0x1p-23f, -1.0f);
+            // 2 u(wb) - 1 = ((wb >> 8) | 1) 2^-23 - 1 = (wb >> 9) 2^-22 + (2^-23 - 1):
+            // the same exact value under the one rounding of the fma
+            const float ct = fmaf((float)(wb >> 9), 0x1p-22f, 0x1p-23f - 1.0f);
             const float sth = sqrt_unit(fmaf(-ct, ct, 1.0f));
             float sphi, cphi;
             sincos_turn(wa, sphi, cphi);
@@ -345,12 +349,16 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 float best = __builtin_inff();
                 int bk = -1;
                 bool bins_ = false;
-                auto test = [&](const int k) {
+                // FIRST: the only test of this pass (best is still +inf and
+                // every crossing time is finite), so the t < best compare
+                // and its branch are left out
+                auto test = [&](const int k, auto first_tag) {
+                    constexpr bool FIRST = decltype(first_tag)::value;
                     int mt;
                     const float4 g = image_at<GEN>(T, k, mt);
                     if (GEN && (mt & kPlaneBit)) {
                         float t;
-                        if (plane_hit(g, x, y, z, sx, sy, sz, ux, uy, uz, ell, A.plane_eps, t) && t < best) {
+                        if (plane_hit(g, x, y, z, sx, sy, sz, ux, uy, uz, ell, A.plane_eps, t) && (FIRST || t < best)) {
                             best = t;
                             bk = k;
                             bins_ = false;
@@ -365,7 +373,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                                                        same, A.H2c, ell);
                     if (quad_hits(Q, ell)) {
                         const float t = quad_root(Q, ell);
-                        if (t < best) {
+                        if (FIRST || t < best) {
                             best = t;
                             bk = k;
                             bins_ = Q.inside;
@@ -373,11 +381,11 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     }
                 };
                 if (single) {
-                    if (nrel > 0 && (int)cl[0] != excl) test(cl[0]);
+                    if (nrel > 0 && (int)cl[0] != excl) test(cl[0], std::true_type{});
                 } else {
                     for (int r = 0; r < nrel; ++r) {
                         const int k = cl[r];
-                        if (k != excl) test(k);
+                        if (k != excl) test(k, std::false_type{});
                     }
                 }
                 live = bk >= 0;
