@@ -49,7 +49,12 @@ __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T
     }
 }
 
-__global__ __launch_bounds__(kBatchBlockMax) void batch_walk_kernel(BatchWalkArgs B) {
+// At most 128 VGPRs (four waves per SIMD, four 256-thread workgroups per CU
+// within the 40 KB LDS budget): left to itself the compiler takes 154 for the
+// twelve inlined walk variants and only three workgroups fit a CU (19 % slower);
+// the few spills this costs sit in the per-walker setup, not the step loop.
+__global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(4))) void batch_walk_kernel(
+    BatchWalkArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
     int staged = -1;

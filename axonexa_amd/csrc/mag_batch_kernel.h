@@ -18,8 +18,21 @@ struct MagBatchArgs {
     unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
 };
 
+// Waves per SIMD the register allocation must allow.  The kernel inlines
+// four walk variants (geometry class x boundary policy) with every step
+// generator; left to itself the compiler takes 200-310 VGPRs (one or two
+// waves).  Three waves (168 VGPRs, spills outside the step loop) measured
+// faster for every field count and integrator except Dormand-Prince with one
+// or two fields (rotation, 2 fields: +14 %; 8 fields: +43 %; Dormand-Prince,
+// 8 fields: 2.2x; Dormand-Prince, 2 fields: -9 % at three, so two there).
 template <int MB, int INTEG>
-__global__ __launch_bounds__(kMagBlock) void mag_batch_walk_kernel(MagBatchArgs B) {
+constexpr int mag_batch_waves() {
+    return (INTEG == kMagDopri && MB <= 2) ? 2 : 3;
+}
+
+template <int MB, int INTEG>
+__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_batch_waves<MB, INTEG>())))
+void mag_batch_walk_kernel(MagBatchArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
     int staged = -1;
