@@ -144,7 +144,12 @@ struct MagAcc {
 };
 
 template <int MB, int INTEG, bool GEN, bool REJ>
-__global__ __launch_bounds__(kMagBlock) void mag_walk_kernel(WalkArgs A, MagArgs G) {
+// At most 128 VGPRs (four waves per SIMD).  Up to four fields the kernel fits
+// by itself; with more the compiler would take 129-169 and drop to three
+// waves, which measured slower than the few spills of the four-wave budget
+// (8 fields: rotation +1 %, RK4 +16 %, Dormand-Prince +39 %).
+__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(4))) void mag_walk_kernel(WalkArgs A,
+                                                                                                   MagArgs G) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsTables T = stage_tables<GEN>(A, lds);
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
