@@ -187,3 +187,30 @@ def test_batch_of_runs_equals_single_runs():
     f3 = fl.build_field([fl.QSC_GRE(5e-4, 0.0, 0.004, 0.01)] * 3, 180, 1e-4)
     with pytest.raises(eng.AxxError):
         eng.MagBatch([items[0], (items[0][0], f3, items[0][2])])
+
+
+@pytest.mark.parametrize("integ", [abi.MAG_ROTATE, abi.MAG_RK4, abi.MAG_DOPRI])
+def test_eight_fields_single_and_batched(integ):
+    """Eight fields in one group: the widest kernels, which run under a
+    register budget with spills (single: four waves per SIMD, batched: three),
+    keep every walker bitwise equal to the oracle."""
+    eng = _engine()
+    R = 0.0025
+    L = 6 * R
+    sub = sb.single_sphere(L, R, D_in=2.5e-6, D_ex=2.5e-6, kappa=1e-2)
+    items = []
+    for k in range(2):
+        d = 0.003 * (k + 1)
+        n = int((2 * d + 0.002) / 1e-4)
+        fs = [fl.QSC_GRE(5e-4, 2e-5 * g, d, 0.002) for g in range(8)]
+        f = fl.build_field(fs, n, 1e-4, integrator=integ, origin=(L / 2,) * 3, m0=(1.0, 0.0, 0.0),
+                           substeps=None if integ == abi.MAG_ROTATE else np.full(n, 2, np.int32))
+        run = abi.RunSpec(n_walkers=1500 + 700 * k, seed=40 + k, instance=k,
+                          init_mode=abi.INIT_INSIDE_PRIM, boundary_mode=abi.BOUNDARY_PERMEABLE)
+        items.append((sub, f, run))
+    _check(f"eight_fields_integ{integ}", *items[0], chunks=[37])
+    with eng.MagBatch(items) as b:
+        b.run()
+        for i, (s_, f, run) in enumerate(items):
+            assert np.array_equal(b.get_state(i), po.walk_mag(s_, f, run)), (integ, i)
+    print(f"[mag] eight fields, integrator {integ}: single and batched bitwise", flush=True)
