@@ -49,6 +49,16 @@ def _cpu_model() -> str:
 PROFILE = os.path.join(ROOT, "profiles", "walk_kernel_c2_hex.txt")
 
 
+def _counted(flop, kernel_ms):
+    """Counter-based FP32 throughput of this run's walk kernel."""
+    if flop is None:
+        return None
+    tf = flop / (kernel_ms * 1e-3) / 1e12
+    return {"flop_per_launch": flop, "tflops": tf, "frac": tf / PEAK_FP32_TFLOPS,
+            "source": "64 x SQ_INSTS_VALU_FLOPS_FP32 per launch from profiles/walk_kernel_c2_hex.txt, "
+                      "over this run's kernel time"}
+
+
 def _profiled_traffic(walkers: int, n_steps: int):
     """HBM bytes per walk launch measured with rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE (calibrated against a plain copy) for the default workload
@@ -59,6 +69,24 @@ def _profiled_traffic(walkers: int, n_steps: int):
         for line in f:
             if line.strip().startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":", 1)[1])
+    return None
+
+
+def _profiled_valu_flops(walkers: int, n_steps: int):
+    """FP32 VALU work per walk launch as the counters count it: 64 lanes x
+    SQ_INSTS_VALU_FLOPS_FP32 (ADD + MUL + 2 FMA + transcendental wave
+    instructions) from the rocprofv3 summary of the default workload; None
+    for other sizes or when the summary is absent."""
+    if walkers != 1000000 or n_steps != 250000 or not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == "SQ_INSTS_VALU_FLOPS_FP32":
+                try:
+                    return 64.0 * float(parts[1])
+                except ValueError:   # the counter's name in a command line
+                    continue
     return None
 
 
@@ -182,6 +210,7 @@ def main(argv=None) -> int:
             "flops_per_walker_step": F,
             "walker_steps_per_launch": ws,
             "kernel_ms": avg_ms,
+            "valu_fp32_counted": _counted(_profiled_valu_flops(wl.run.n_walkers, n), avg_ms),
             "collisions_per_walker_step": rho,
         },
         "signal": {
