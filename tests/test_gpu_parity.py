@@ -215,6 +215,31 @@ def test_wide_profile_set_gamma_packing():
     assert sig.collisions > 1000
 
 
+def test_dense_permeable_packing_many_events_per_step():
+    # steps of 0.49 um against radii from 0.25 um with p = 0.6: several
+    # boundary events per step, multi-candidate sub-bins, and transmission
+    # draws that run through many blocks of four per walker
+    dt = 2e-2
+    sub = sb.gamma_packing(30, 5.0, 0.4, 0.6, D_in=1.7, D_ex=2.0, seed=7, T2_in=70.0, T2_ex=50.0,
+                           rmin=0.25, kappa=sb.kappa_for_probability(0.6, 1.7, dt))
+    acq = wf.build_acq([wf.RectGrad(g * 1e-6, 3.0, 2.0, (1, 0, 0)) for g in (0.1, 0.3)], 500, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=3000, seed=21, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both("dense_permeable", sub, acq, run)
+    assert sig.collisions > 100000 and sig.transmissions > 50000
+
+
+def test_permeable_sphere_lattice_many_events():
+    dt = 2e-2
+    cs = [[2.5 + 5 * i, 2.5 + 5 * j, 2.5 + 5 * k] for i in range(2) for j in range(2) for k in range(2)]
+    sub = abi.SubstrateSpec(a=10.0, b=10.0, c=10.0, D_lat=2.0, types=[abi.PRIM_SPHERE] * 8, centers=cs,
+                            radii=[2.2] * 8, D=1.5, T2=60.0, T2_lat=40.0,
+                            kappa=sb.kappa_for_probability(0.5, 1.5, dt))
+    acq = wf.build_acq([wf.RectGrad(g * 1e-6, 3.0, 2.0, (0, 0, 1)) for g in (0.1, 0.3)], 500, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=3000, seed=22, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both("sphere_lattice_permeable", sub, acq, run, chunks=[1, 250])
+    assert sig.collisions > 100000 and sig.transmissions > 50000
+
+
 def test_single_walker_large_ids():
     sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
     acq = _pgse(900, 1e-2)
