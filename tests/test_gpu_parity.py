@@ -287,6 +287,32 @@ def test_invalid_descriptors_are_errors():
             p.walk(5, 10)                    # must restart at 0 or continue
 
 
+def test_acquisition_limits_are_errors():
+    # 64 profile columns is the widest kernel; 65 is refused as a limit, a
+    # waveform naming a missing profile and non-finite samples as invalid
+    eng = _engine()
+    sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
+    n = 50
+    run = abi.RunSpec(n_walkers=64)
+
+    def acq(P, wf_p=None, bad=False):
+        prof = np.zeros((n, P), np.float32)
+        prof[: n // 2] = 1.0
+        prof[n // 2:] = -1.0
+        if bad:
+            prof[3, 0] = np.nan
+        wf_p = np.arange(P, dtype=np.int32) if wf_p is None else np.asarray(wf_p, np.int32)
+        return abi.AcqSpec(n_steps=n, dt=1e-2, gamma=GAMMA, profiles=prof, wf_profile=wf_p,
+                           wf_amplitude=np.full(wf_p.size, 1e-7), wf_dir=np.tile([1.0, 0, 0], (wf_p.size, 1)))
+
+    with eng.Plan(sub, acq(64), run) as p:
+        p.walk()
+    for q, code in ((acq(65), -4), (acq(4, wf_p=[0, 1, 4]), -1), (acq(4, bad=True), -1)):
+        with pytest.raises(eng.AxxError) as e:
+            eng.Plan(sub, q, run)
+        assert e.value.code == code
+
+
 def test_restored_state_is_checked():
     eng = _engine()
     sub = sb.single_cylinder(12.0, 3.0, D_in=2.0, D_ex=2.0)
