@@ -143,13 +143,20 @@ struct MagAcc {
     __device__ __forceinline__ void step(int i, float dx, float dy, float dz, int comp) { update(i, dx, dy, dz, comp); }
 };
 
+// Register budget as waves per SIMD.  Four (at most 128 VGPRs): up to four
+// fields the kernel fits by itself; with more the compiler would take
+// 129-169 and drop to three waves, which measured slower than the few spills
+// of the budget (8 fields: rotation +1 %, RK4 +16 %, Dormand-Prince +39 %).
+// Five (96 VGPRs, 16-36 B of spills) for one or two fields with rotation or
+// RK4: +2-4 % (Dormand-Prince, whose stages hold more live values, loses 3 %).
+template <int MB, int INTEG>
+constexpr int mag_waves() {
+    return (MB <= 2 && INTEG != kMagDopri) ? 5 : 4;
+}
+
 template <int MB, int INTEG, bool GEN, bool REJ>
-// At most 128 VGPRs (four waves per SIMD).  Up to four fields the kernel fits
-// by itself; with more the compiler would take 129-169 and drop to three
-// waves, which measured slower than the few spills of the four-wave budget
-// (8 fields: rotation +1 %, RK4 +16 %, Dormand-Prince +39 %).
-__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(4))) void mag_walk_kernel(WalkArgs A,
-                                                                                                   MagArgs G) {
+__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_waves<MB, INTEG>()))) void
+mag_walk_kernel(WalkArgs A, MagArgs G) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsTables T = stage_tables<GEN>(A, lds);
     const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
