@@ -224,10 +224,10 @@ extern "C" int axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_fi
     return mag_create_internal(ctx, s, f, r, kMagLdsBudget, out);
 }
 
-static mag_fn pick_mag(int integ, int mb, bool gen, bool rej) {
-    if (integ == kMagRotate) return pick_mag_rotate(mb, gen, rej);
-    if (integ == kMagRK4) return pick_mag_rk4(mb, gen, rej);
-    return pick_mag_dopri(mb, gen, rej);
+static mag_fn pick_mag(int integ, int mb, bool gen, bool rej, bool alt) {
+    if (integ == kMagRotate) return pick_mag_rotate(mb, gen, rej, alt);
+    if (integ == kMagRK4) return pick_mag_rk4(mb, gen, rej, alt);
+    return pick_mag_dopri(mb, gen, rej, alt);
 }
 
 extern "C" int axx_mag_walk(axx_mag* m, int step_begin, int step_end) {
@@ -255,7 +255,8 @@ extern "C" int axx_mag_walk(axx_mag* m, int step_begin, int step_end) {
                 HIP_TRY(ctx, hipGetLastError());
             }
             if (step_begin < step_end) {
-                mag_fn fn = pick_mag(m->integ, m->gMB[g], p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION);
+                mag_fn fn = pick_mag(m->integ, m->gMB[g], p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION,
+                                     p->run.step_mode != AXX_STEP_ISOTROPIC);
                 hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kMagBlock), p->lds_bytes, ctx->stream, A, G);
                 HIP_TRY(ctx, hipGetLastError());
             }
