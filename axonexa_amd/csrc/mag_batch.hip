@@ -64,10 +64,10 @@ static __global__ void mag_batch_finalize_kernel(const MagRed* __restrict__ red,
     for (int t = threadIdx.x; t < F * 3; t += blockDim.x) R.out[t] = mag_chunk_total(R.partial, R.n_chunks, t / 3, t % 3);
 }
 
-static mag_batch_fn pick_batch(int integ, int mb) {
-    if (integ == kMagRotate) return pick_mag_batch_rotate(mb);
-    if (integ == kMagRK4) return pick_mag_batch_rk4(mb);
-    return pick_mag_batch_dopri(mb);
+static mag_batch_fn pick_batch(int integ, int mb, int cls) {
+    if (integ == kMagRotate) return pick_mag_batch_rotate(mb, cls);
+    if (integ == kMagRK4) return pick_mag_batch_rk4(mb, cls);
+    return pick_mag_batch_dopri(mb, cls);
 }
 
 }  // namespace axx
@@ -84,6 +84,7 @@ struct axx_mag_batch {
     MagRed* d_red = nullptr;
     int2* d_red_blocks = nullptr;
     int n_tiles = 0, n_red = 0, grid = 0;
+    int cls = -1;   // the class all runs share with isotropic steps, else -1
     size_t lds_walk = 0, lds_init = 0;
     bool ran = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -166,6 +167,8 @@ extern "C" int axx_mag_batch_create(axx_ctx* ctx, int n_inst, const axx_mag_inst
             G.F = m->gF[0];
             margs.push_back(G);
             cls.push_back((p->gen ? 1 : 0) | (p->run.boundary_mode == AXX_BOUNDARY_REJECTION ? 2 : 0));
+            const int ci = p->run.step_mode == AXX_STEP_ISOTROPIC ? cls.back() : -1;
+            b->cls = i == 0 ? ci : (ci == b->cls ? ci : -1);
             red.push_back(MagRed{p->d_state, p->N, m->n_chunks, m->d_part, m->d_out, m->d_cnt});
             for (int c = 0; c < m->n_chunks; ++c) rblocks.push_back(make_int2(i, c));
             for (long long w0 = 0; w0 < p->N; w0 += kMagBlock) {
@@ -198,7 +201,7 @@ extern "C" int axx_mag_batch_create(axx_ctx* ctx, int n_inst, const axx_mag_inst
             return fail(ctx, AXX_ERR_LIMIT, "mag batch: substrate tables exceed LDS");
         }
         int per_cu = 0;
-        const mag_batch_fn fn = pick_batch(b->integ, b->F);
+        const mag_batch_fn fn = pick_batch(b->integ, b->F, b->cls);
         HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kMagBlock, b->lds_walk));
         if (per_cu < 1) {
             mag_batch_free(b);
@@ -227,7 +230,7 @@ extern "C" int axx_mag_batch_run(axx_mag_batch* b) {
                            (const WalkArgs*)b->d_args, (const MagArgs*)b->d_margs, b->F, (const int4*)b->d_tiles);
         HIP_TRY(ctx, hipGetLastError());
         MagBatchArgs B{b->d_args, b->d_margs, b->d_cls, b->d_tiles, b->n_tiles, b->d_next};
-        hipLaunchKernelGGL(pick_batch(b->integ, b->F), dim3(b->grid), dim3(kMagBlock), b->lds_walk, ctx->stream, B);
+        hipLaunchKernelGGL(pick_batch(b->integ, b->F, b->cls), dim3(b->grid), dim3(kMagBlock), b->lds_walk, ctx->stream, B);
         HIP_TRY(ctx, hipGetLastError());
         hipLaunchKernelGGL(mag_batch_reduce_kernel, dim3(b->n_red, b->F), dim3(kMagRedThreads), 0, ctx->stream,
                            (const MagRed*)b->d_red, (const int2*)b->d_red_blocks);

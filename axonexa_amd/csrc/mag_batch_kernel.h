@@ -2,8 +2,9 @@
 // a grid sized to the chip takes (run, walker range) tiles from an atomic
 // counter, restages the run's substrate tables in LDS when the run changes
 // and walks the tile's walkers with that run's fields.  All runs of a batch
-// have the same field count MB and integrator; geometry class and boundary
-// policy are per run (a wave-uniform switch).
+// have the same field count MB and integrator; geometry class, boundary
+// policy and step generator are per run (a wave-uniform switch), or, when
+// every run shares them with isotropic steps, fixed in the instantiation.
 #pragma once
 #include "mag_kernel.h"
 
@@ -18,20 +19,25 @@ struct MagBatchArgs {
     unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
 };
 
-// Waves per SIMD the register allocation must allow.  The kernel inlines
-// four walk variants (geometry class x boundary policy) with every step
-// generator; left to itself the compiler takes 200-310 VGPRs (one or two
-// waves).  Three waves (168 VGPRs, spills outside the step loop) measured
-// faster for every field count and integrator except Dormand-Prince with one
-// or two fields (rotation, 2 fields: +14 %; 8 fields: +43 %; Dormand-Prince,
-// 8 fields: 2.2x; Dormand-Prince, 2 fields: -9 % at three, so two there).
-template <int MB, int INTEG>
+// CLS: -1 = runs of mixed geometry classes / boundary policies / step
+// generators (a wave-uniform switch over four walk variants, every step
+// generator compiled in); 0..3 = every run of the batch has class CLS
+// (general geometry | rejection << 1) and isotropic steps: one walk variant.
+//
+// Waves per SIMD the register allocation must allow.  Mixed: left to itself
+// the compiler takes 200-310 VGPRs (one or two waves); three waves (168
+// VGPRs, spills outside the step loop) measured faster for every field count
+// and integrator except Dormand-Prince with one or two fields, which keeps
+// its own allocation.  Uniform: four waves (128 VGPRs): rotation, 2 fields
+// +9 %, 8 fields +43 %; RK4 +14 %; Dormand-Prince +30-37 % over the mixed
+// kernel on the same runs.
+template <int MB, int INTEG, int CLS>
 constexpr int mag_batch_waves() {
-    return (INTEG == kMagDopri && MB <= 2) ? 2 : 3;
+    return CLS >= 0 ? 4 : ((INTEG == kMagDopri && MB <= 2) ? 1 : 3);
 }
 
-template <int MB, int INTEG>
-__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_batch_waves<MB, INTEG>())))
+template <int MB, int INTEG, int CLS>
+__global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_batch_waves<MB, INTEG, CLS>())))
 void mag_batch_walk_kernel(MagBatchArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
@@ -48,29 +54,36 @@ void mag_batch_walk_kernel(MagBatchArgs B) {
         const int w0 = __builtin_amdgcn_readfirstlane(tile.y);
         const int w1 = __builtin_amdgcn_readfirstlane(tile.z);
         const WalkArgs A = B.args[i];
-        const int c = __builtin_amdgcn_readfirstlane(B.cls[i]);
+        const int c = CLS >= 0 ? CLS : __builtin_amdgcn_readfirstlane(B.cls[i]);
         if (i != staged) {
             // every lane finished the previous tile (barriers above)
-            T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
+            if constexpr (CLS >= 0)
+                T = stage_tables<(CLS & 1) != 0>(A, lds);
+            else
+                T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
             staged = i;
         }
         const MagArgs G = B.margs[i];
         for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x) {
             MagAcc<MB, INTEG> acc;
             acc.init(G);
-            switch (c) {
-            case 0: walk_lane<MagAcc<MB, INTEG>, false, false, true>(A, T, acc, (uint32_t)w); break;
-            case 1: walk_lane<MagAcc<MB, INTEG>, true, false, true>(A, T, acc, (uint32_t)w); break;
-            case 2: walk_lane<MagAcc<MB, INTEG>, false, true, true>(A, T, acc, (uint32_t)w); break;
-            default: walk_lane<MagAcc<MB, INTEG>, true, true, true>(A, T, acc, (uint32_t)w); break;
+            if constexpr (CLS >= 0) {
+                walk_lane<MagAcc<MB, INTEG>, (CLS & 1) != 0, (CLS & 2) != 0, false>(A, T, acc, (uint32_t)w);
+            } else {
+                switch (c) {
+                case 0: walk_lane<MagAcc<MB, INTEG>, false, false, true>(A, T, acc, (uint32_t)w); break;
+                case 1: walk_lane<MagAcc<MB, INTEG>, true, false, true>(A, T, acc, (uint32_t)w); break;
+                case 2: walk_lane<MagAcc<MB, INTEG>, false, true, true>(A, T, acc, (uint32_t)w); break;
+                default: walk_lane<MagAcc<MB, INTEG>, true, true, true>(A, T, acc, (uint32_t)w); break;
+                }
             }
         }
     }
 }
 
 typedef void (*mag_batch_fn)(MagBatchArgs);
-mag_batch_fn pick_mag_batch_rotate(int mb);
-mag_batch_fn pick_mag_batch_rk4(int mb);
-mag_batch_fn pick_mag_batch_dopri(int mb);
+mag_batch_fn pick_mag_batch_rotate(int mb, int cls);
+mag_batch_fn pick_mag_batch_rk4(int mb, int cls);
+mag_batch_fn pick_mag_batch_dopri(int mb, int cls);
 
 }  // namespace axx
