@@ -40,12 +40,12 @@ struct BatchWalkArgs {
     unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
 };
 
-template <int PT, bool GEN, bool REJ>
+template <int PT, bool GEN, bool REJ, bool ALT>
 __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
     for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x) {
         PhaseAcc<PT, 7> acc;
         acc.prof = (const_fptr)A.prof;
-        walk_lane<PhaseAcc<PT, 7>, GEN, REJ, true>(A, T, acc, (uint32_t)w);
+        walk_lane<PhaseAcc<PT, 7>, GEN, REJ, ALT>(A, T, acc, (uint32_t)w);
     }
 }
 
@@ -53,6 +53,9 @@ __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T
 // within the 40 KB LDS budget): left to itself the compiler takes 154 for the
 // twelve inlined walk variants and only three workgroups fit a CU (19 % slower);
 // the few spills this costs sit in the per-walker setup, not the step loop.
+// ALT: the axis-sign and Gaussian step generators compiled in; a batch whose
+// instances all take isotropic steps runs the instantiation without them (4 %).
+template <bool ALT>
 __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(4))) void batch_walk_kernel(
     BatchWalkArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -77,18 +80,18 @@ __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(
             staged = i;
         }
         switch (c) {
-        case 0: batch_tile<1, false, false>(A, T, w0, w1); break;
-        case 1: batch_tile<1, true, false>(A, T, w0, w1); break;
-        case 2: batch_tile<1, false, true>(A, T, w0, w1); break;
-        case 3: batch_tile<1, true, true>(A, T, w0, w1); break;
-        case 4: batch_tile<4, false, false>(A, T, w0, w1); break;
-        case 5: batch_tile<4, true, false>(A, T, w0, w1); break;
-        case 6: batch_tile<4, false, true>(A, T, w0, w1); break;
-        case 7: batch_tile<4, true, true>(A, T, w0, w1); break;
-        case 8: batch_tile<8, false, false>(A, T, w0, w1); break;
-        case 9: batch_tile<8, true, false>(A, T, w0, w1); break;
-        case 10: batch_tile<8, false, true>(A, T, w0, w1); break;
-        default: batch_tile<8, true, true>(A, T, w0, w1); break;
+        case 0: batch_tile<1, false, false, ALT>(A, T, w0, w1); break;
+        case 1: batch_tile<1, true, false, ALT>(A, T, w0, w1); break;
+        case 2: batch_tile<1, false, true, ALT>(A, T, w0, w1); break;
+        case 3: batch_tile<1, true, true, ALT>(A, T, w0, w1); break;
+        case 4: batch_tile<4, false, false, ALT>(A, T, w0, w1); break;
+        case 5: batch_tile<4, true, false, ALT>(A, T, w0, w1); break;
+        case 6: batch_tile<4, false, true, ALT>(A, T, w0, w1); break;
+        case 7: batch_tile<4, true, true, ALT>(A, T, w0, w1); break;
+        case 8: batch_tile<8, false, false, ALT>(A, T, w0, w1); break;
+        case 9: batch_tile<8, true, false, ALT>(A, T, w0, w1); break;
+        case 10: batch_tile<8, false, true, ALT>(A, T, w0, w1); break;
+        default: batch_tile<8, true, true, ALT>(A, T, w0, w1); break;
         }
     }
 }
@@ -142,6 +145,7 @@ struct axx_batch {
     axx_ctx* ctx = nullptr;
     std::vector<axx_plan*> plans;
     std::vector<int> cls;
+    bool alt = false;   // some instance takes axis-sign or Gaussian steps
     int block = 256, wpt = 1, grid = 0, n_tiles = 0, n_red = 0;
     size_t lds_walk = 0, lds_init = 0;
     std::string signature;
@@ -174,6 +178,11 @@ std::map<std::string, std::pair<int, int>>& tune_cache() {
 std::mutex& tune_mutex() {
     static std::mutex mu;
     return mu;
+}
+
+typedef void (*batch_walk_fn)(BatchWalkArgs);
+batch_walk_fn walk_kernel_of(const axx_batch* b) {
+    return b->alt ? (batch_walk_fn)batch_walk_kernel<true> : (batch_walk_fn)batch_walk_kernel<false>;
 }
 
 int class_of(const axx_plan* p) {
@@ -243,7 +252,7 @@ int build_tiles(axx_batch* b, int block, int wpt) {
     int rc = upload(ctx, &b->d_tiles, h);
     if (rc) return rc;
     int per_cu = 0;
-    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, b->lds_walk));
+    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_kernel_of(b), block, b->lds_walk));
     if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "batch kernel does not fit a CU at block %d", block);
     b->grid = std::max(1, std::min(per_cu * std::max(ctx->n_cu, 1), b->n_tiles));
     b->block = block;
@@ -261,7 +270,7 @@ int launch_walk_part(axx_batch* b, const WalkArgs* args) {
                        (const WalkArgs*)b->d_args, (const int*)b->d_nmom, (const int4*)b->d_tiles);
     HIP_TRY(ctx, hipGetLastError());
     BatchWalkArgs B{args, b->d_cls, b->d_tiles, b->n_tiles, b->d_next};
-    hipLaunchKernelGGL(batch_walk_kernel, dim3(b->grid), dim3(b->block), b->lds_walk, ctx->stream, B);
+    hipLaunchKernelGGL(walk_kernel_of(b), dim3(b->grid), dim3(b->block), b->lds_walk, ctx->stream, B);
     HIP_TRY(ctx, hipGetLastError());
     return AXX_OK;
 }
@@ -406,6 +415,7 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
             b->plans.push_back(p);   // capacity reserved: does not throw
             p_new = nullptr;
             b->cls.push_back(class_of(p));
+            b->alt = b->alt || p->run.step_mode != AXX_STEP_ISOTROPIC;
             WalkArgs A = plan_walk_args(p, 1, p->n_steps);
             args.push_back(A);
             nmom.push_back(3 * p->Pt);
@@ -419,8 +429,8 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
             cnt.push_back(p->d_cnt);
             outs.push_back(p->d_out);
             char buf[96];
-            std::snprintf(buf, sizeof(buf), ";c%dN%lldn%di%dm%d", b->cls.back(), (long long)p->N, p->n_steps,
-                          p->n_img, p->M);
+            std::snprintf(buf, sizeof(buf), ";c%dN%lldn%di%dm%ds%d", b->cls.back(), (long long)p->N, p->n_steps,
+                          p->n_img, p->M, p->run.step_mode);
             b->signature += buf;
         }
         b->n_red = (int)rblocks.size();
