@@ -46,6 +46,7 @@ inline int mag_samples(int integ, int s) {
 template <int MB, int INTEG>
 struct MagAcc {
     static constexpr bool kOneSite = true;
+    static constexpr bool kCacheColl = true;
     const_dptr tab;
     const_iptr off;
     const_iptr nsub;

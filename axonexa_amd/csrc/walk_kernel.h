@@ -181,6 +181,10 @@ constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 template <int P, int AX>
 struct PhaseAcc {
     static constexpr bool kOneSite = false;
+    // keep the last transmission-draw words in registers (four events per
+    // Philox call) unless the moments already fill the register file (96 or
+    // more accumulators): there recomputing the call per event is 1 % faster
+    static constexpr bool kCacheColl = P * ((AX & 1) + ((AX >> 1) & 1) + ((AX >> 2) & 1)) < 96;
     const_fptr prof;
     float m[P][3];
 
@@ -403,7 +407,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 if (permeable && !pl) {     // planes never transmit (plane.cuh:39)
                     // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
                     const uint32_t n = ncoll - 1u;   // this event's ordinal in the walk
-                    if ((n >> 2) != cblk) {
+                    if (!Acc::kCacheColl || (n >> 2) != cblk) {
                         cw = philox10(wid, n >> 2, kStreamColl, A.inst, A.k0, A.k1);
                         cblk = n >> 2;
                     }

@@ -202,17 +202,20 @@ def test_planes_rejection_mode():
     assert sig.collisions > 100
 
 
-def test_wide_profile_set_gamma_packing():
-    # 64 profiles x 3 axes: the 512-thread, 256-VGPR instantiation
+@pytest.mark.parametrize("mode", [abi.BOUNDARY_SPECULAR, abi.BOUNDARY_PERMEABLE])
+def test_wide_profile_set_gamma_packing(mode):
+    # 64 profiles x 3 axes: the 512-thread, 256-VGPR instantiation (which
+    # recomputes the transmission draws per event instead of keeping them)
     from axonexa_amd import workloads as W
     sub = sb.gamma_packing(30, 5.0, 0.4, 0.6, D_in=1.7, D_ex=2.0, seed=4, T2_in=70.0, T2_ex=50.0,
-                           rmin=0.25)
+                           rmin=0.25, kappa=sb.kappa_for_probability(0.3, 1.7, 5e-3))
     fs = W.c3_functors(n_dirs=2, amps=(0.3,))
     acq = wf.build_acq(fs, 600, 5e-3, GAMMA)
     assert acq.n_profiles == 64
-    run = abi.RunSpec(n_walkers=1500, seed=77, relaxation=1)
-    sig, _ = _run_both("wide_profiles", sub, acq, run)
+    run = abi.RunSpec(n_walkers=1500, seed=77, relaxation=1, boundary_mode=mode)
+    sig, _ = _run_both(f"wide_profiles_{mode}", sub, acq, run)
     assert sig.collisions > 1000
+    assert (sig.transmissions > 1000) == (mode == abi.BOUNDARY_PERMEABLE)
 
 
 def test_dense_permeable_packing_many_events_per_step():
