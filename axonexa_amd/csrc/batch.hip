@@ -55,7 +55,9 @@ __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T
 // the few spills this costs sit in the per-walker setup, not the step loop.
 // ALT: the axis-sign and Gaussian step generators compiled in; a batch whose
 // instances all take isotropic steps runs the instantiation without them (4 %).
-template <bool ALT>
+// CLS: -1 = instances of several classes (a wave-uniform switch per tile);
+// 0..11 = every instance has class CLS and isotropic steps: one walk variant.
+template <bool ALT, int CLS>
 __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(4))) void batch_walk_kernel(
     BatchWalkArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -73,11 +75,19 @@ __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(
         const int w0 = __builtin_amdgcn_readfirstlane(tile.y);
         const int w1 = __builtin_amdgcn_readfirstlane(tile.z);
         const WalkArgs A = B.args[i];
-        const int c = __builtin_amdgcn_readfirstlane(B.cls[i]);
+        const int c = CLS >= 0 ? CLS : __builtin_amdgcn_readfirstlane(B.cls[i]);
         if (i != staged) {
             // every lane finished the previous tile (barriers above)
-            T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
+            if constexpr (CLS >= 0)
+                T = stage_tables<(CLS & 1) != 0>(A, lds);
+            else
+                T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
             staged = i;
+        }
+        if constexpr (CLS >= 0) {
+            constexpr int PT = CLS < 4 ? 1 : (CLS < 8 ? 4 : 8);
+            batch_tile<PT, (CLS & 1) != 0, (CLS & 2) != 0, ALT>(A, T, w0, w1);
+            continue;
         }
         switch (c) {
         case 0: batch_tile<1, false, false, ALT>(A, T, w0, w1); break;
@@ -146,6 +156,7 @@ struct axx_batch {
     std::vector<axx_plan*> plans;
     std::vector<int> cls;
     bool alt = false;   // some instance takes axis-sign or Gaussian steps
+    int uniform = -1;   // the class every instance has with isotropic steps, else -1
     int block = 256, wpt = 1, grid = 0, n_tiles = 0, n_red = 0;
     size_t lds_walk = 0, lds_init = 0;
     std::string signature;
@@ -181,8 +192,16 @@ std::mutex& tune_mutex() {
 }
 
 typedef void (*batch_walk_fn)(BatchWalkArgs);
+template <int C>
+batch_walk_fn uniform_kernel(int c) {
+    if constexpr (C < 12)
+        return c == C ? (batch_walk_fn)batch_walk_kernel<false, C> : uniform_kernel<C + 1>(c);
+    else
+        return nullptr;
+}
 batch_walk_fn walk_kernel_of(const axx_batch* b) {
-    return b->alt ? (batch_walk_fn)batch_walk_kernel<true> : (batch_walk_fn)batch_walk_kernel<false>;
+    if (b->uniform >= 0) return uniform_kernel<0>(b->uniform);
+    return b->alt ? (batch_walk_fn)batch_walk_kernel<true, -1> : (batch_walk_fn)batch_walk_kernel<false, -1>;
 }
 
 int class_of(const axx_plan* p) {
@@ -416,6 +435,8 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
             p_new = nullptr;
             b->cls.push_back(class_of(p));
             b->alt = b->alt || p->run.step_mode != AXX_STEP_ISOTROPIC;
+            const int ci = p->run.step_mode == AXX_STEP_ISOTROPIC ? b->cls.back() : -1;
+            b->uniform = i == 0 ? ci : (ci == b->uniform ? ci : -1);
             WalkArgs A = plan_walk_args(p, 1, p->n_steps);
             args.push_back(A);
             nmom.push_back(3 * p->Pt);

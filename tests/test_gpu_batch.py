@@ -137,3 +137,39 @@ def test_batch_rejects_what_it_cannot_run():
     with pytest.raises(eng.AxxError) as e:
         eng.Batch([(w.sub, wide, w.run)])
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("kind", ["cylinders", "spheres_rejection"])
+def test_uniform_batch_runs_its_class_kernel(kind):
+    """Instances that all share one class (profile width, geometry, boundary
+    policy) with isotropic steps run the kernel fixed to that class; each is
+    still bitwise its oracle walk."""
+    from axonexa_amd import substrates as sb
+    dt = 2e-3
+    acq = wf.build_acq([wf.CosGFunc(g * 1e-6, 0.4, 0.1, f, (1, 0, 0)) for f in (1.0, 2.0, 3.0)
+                        for g in (0.1, 0.4)], 300, dt, W.GAMMA)
+
+    class Case:
+        pass
+
+    ws = []
+    for k in range(5):
+        c = Case()
+        if kind == "cylinders":
+            c.sub = sb.hex_lattice(1.5 + 0.5 * k, 4.0 + 1.2 * k, 3, 2, D_in=1.7, D_ex=2.0, T2_in=70.0,
+                                   T2_ex=50.0, kappa=sb.kappa_for_probability(0.1, 1.7, dt))
+            mode, init = abi.BOUNDARY_PERMEABLE, abi.INIT_UNIFORM
+        else:
+            c.sub = sb.single_sphere(12.0 + k, 3.0 + 0.5 * k, D_in=2.0, D_ex=2.0)
+            mode, init = abi.BOUNDARY_REJECTION, abi.INIT_INSIDE_PRIM
+        c.acq = acq
+        c.run = abi.RunSpec(n_walkers=600 + 173 * k, seed=60 + k, instance=k, boundary_mode=mode,
+                            relaxation=1, init_mode=init)
+        ws.append(c)
+    eng = _engine()
+    with eng.Batch([(w.sub, w.acq, w.run) for w in ws]) as b:
+        b.run()
+        for i, w in enumerate(ws):
+            g = _check_instance(b, i, w, f"{kind} {i}")
+            assert g.collisions > 0, i
+    print(f"[batch] uniform {kind}: bitwise", flush=True)
