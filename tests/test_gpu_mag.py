@@ -214,3 +214,28 @@ def test_eight_fields_single_and_batched(integ):
         for i, (s_, f, run) in enumerate(items):
             assert np.array_equal(b.get_state(i), po.walk_mag(s_, f, run)), (integ, i)
     print(f"[mag] eight fields, integrator {integ}: single and batched bitwise", flush=True)
+
+
+def test_batch_with_mixed_step_generators():
+    """A magnetization batch whose runs differ in step generator (isotropic,
+    axis-sign, Gaussian) runs the per-run switching kernel; every run is still
+    bitwise its oracle walk."""
+    eng = _engine()
+    R = 0.0025
+    L = 6 * R
+    sub = sb.single_sphere(L, R, D_in=2.5e-6, D_ex=2.5e-6)
+    items = []
+    for k, smode in enumerate((abi.STEP_ISOTROPIC, abi.STEP_AXES, abi.STEP_GAUSSIAN)):
+        d = 0.003 * (k + 1)
+        n = int((2 * d + 0.002) / 1e-4)
+        f = fl.build_field([fl.QSC_GRE(5e-4, 0.0, d, 0.002), fl.QSC_GRE(5e-4, 3e-5, d, 0.002)], n, 1e-4,
+                           origin=(L / 2,) * 3)
+        run = abi.RunSpec(n_walkers=900 + 300 * k, seed=70 + k, instance=k, init_mode=abi.INIT_INSIDE_PRIM,
+                          step_mode=smode)
+        items.append((sub, f, run))
+    with eng.MagBatch(items) as b:
+        b.run()
+        for i, (s_, f, run) in enumerate(items):
+            assert np.array_equal(b.get_state(i), po.walk_mag(s_, f, run)), i
+            assert np.array_equal(b.results(i).mx, eng.run_mag(s_, f, run).mx), i
+    print("[mag] batch with mixed step generators: bitwise", flush=True)
