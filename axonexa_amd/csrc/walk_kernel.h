@@ -8,7 +8,7 @@
 //
 // Reference: updateWalkersLatticePhase (src/kernels/kernelLattice.cuh:1-104)
 // with boundaryCheckImpermeable / boundaryCheckPermeable
-// (src/kernels/boundaryCheck.cuh:138-217) and the Lattice scans
+// (src/kernels/boundaryCheck.cuh:34-64, :66-113) and the Lattice scans
 // (src/substrate/lattice.cuh:90-247).  Differences by design: Philox
 // counters instead of per-thread XORWOW state, moments per (profile, axis)
 // instead of M global read-modify-writes per step, LDS cell bins instead of
@@ -405,7 +405,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 const bool ins = bins_;
                 bool transmit = false;
                 if (permeable && !pl) {     // planes never transmit (plane.cuh:39)
-                    // transmission with p = 4 kappa / v (boundaryCheck.cuh:199)
+                    // transmission with p = 4 kappa / v (boundaryCheck.cuh:94)
                     const uint32_t n = ncoll - 1u;   // this event's ordinal in the walk
                     if (!Acc::kCacheColl || (n >> 2) != cblk) {
                         cw = philox10(wid, n >> 2, kStreamColl, A.inst, A.k0, A.k1);
@@ -421,7 +421,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 // plane reflection): the next scan skips this surface
                 excl = (pl || ins == transmit) ? bk : -1;
                 if (transmit) {
-                    // continue on the far side with its speed (boundaryCheck.cuh:211-215)
+                    // continue on the far side with its speed (boundaryCheck.cuh:104-111)
                     const int nc = ins ? 0 : pcomp;
                     const float f = (ell - t) / (gs * Lc);
                     const float ph = 1.0f - f;
@@ -434,7 +434,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     ntran++;
                 } else {
                     // specular reflection about the surface normal (p - c) / R, or
-                    // the plane's normal (boundaryCheck.cuh:149-167,
+                    // the plane's normal (boundaryCheck.cuh:59-62, :96-103,
                     // cylinderXY.cuh:125-131, plane.cuh:32)
                     float nx, ny, nz, kd;
                     if (pl) {

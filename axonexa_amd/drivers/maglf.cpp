@@ -1,7 +1,7 @@
 // maglf — low-field saddle-coil gradient echoes in a spherical pore
 // (magnetization path).
 //
-// The experiment of the reference driver examples/magLF_spherNeuman.cu:65-183
+// The experiment of the reference driver examples/magLF_spherNeuman.cu:65-171
 // (units mm, ms, T), written against axonexa_compat.hpp: walkers diffuse
 // (D = 2.5e-6 mm^2/ms, dt = 0.1 us) inside a reflecting sphere of radius
 // 2.5 um starting with M = (1, 0, 0); each acquisition i has two QSC_GRE

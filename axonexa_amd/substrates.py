@@ -147,7 +147,7 @@ def hex_intra_fraction(R: float, spacing: float) -> float:
 def kappa_for_probability(p: float, D: float, dt: float) -> float:
     """Permeability giving transmission probability p per collision for a
     walker of diffusivity D: p = 4*kappa/v, v = sqrt(6 D / dt)
-    (src/kernels/boundaryCheck.cuh:199)."""
+    (src/kernels/boundaryCheck.cuh:94)."""
     return p * math.sqrt(6.0 * D / dt) / 4.0
 
 

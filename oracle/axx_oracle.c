@@ -9,7 +9,7 @@
  * operation (built with -ffp-contract=off), the semantics of the reference's
  * lattice walk:
  *   kernelLattice.cuh:1-104           per-walker loop (init, step, phase, T2)
- *   boundaryCheck.cuh:138-217         multi-reflection / permeable boundary
+ *   boundaryCheck.cuh:34-64, :66-113  multi-reflection / permeable boundary
  *   lattice.cuh:31-247                periodic cell, wrap (lmod), region lookup
  *   cylinderXY.cuh:69-131, Sphere.cuh:89-149   inside / intersect / normal
  *   plane.cuh:18-39                   plane crossing, normal, no region
@@ -26,7 +26,11 @@
  * (Random123) and rocRAND's engine; closed-form ray/circle and ray/sphere
  * roots in fp64; the exact free-diffusion sinc product; exp(-bD); Neuman /
  * GPD restricted limits; invariants (step length, reflection isometry,
- * lattice translation, uniform equilibrium under permeability 4*kappa/v).
+ * lattice translation, uniform equilibrium under permeability 4*kappa/v);
+ * the transmitted fraction of the boundary events from each side equals
+ * 4*kappa/v of that side; the T2 exponent of every walker equals the one
+ * rebuilt from its trajectory with fp64 crossing times; the ORC_PLANT
+ * variants below fail those checks.
  */
 #include <math.h>
 #include <pthread.h>
@@ -36,6 +40,15 @@
 
 #include "../include/axonexa.h"
 #include "axx_oracle.h"
+
+/* ORC_PLANT selects a deliberately wrong variant, built only by the tests
+ * that show their checks catch it: 1 doubles the transmission probability,
+ * 2 charges a whole crossing step's T2 decay to the far side (the
+ * reference's by-value accumtime, defect D2), 3 uses the lattice speed for
+ * the transmission threshold from both sides.  0 is the oracle. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 
 /* ------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Replaces cuRAND
@@ -796,6 +809,9 @@ typedef struct {
     int64_t w_begin, w_end;
     const magjob_t* mag;   /* magnetization of a group of fields instead of phase moments */
     int err;
+    /* boundary events at circles and spheres: [0] from inside the surface,
+     * [1] of those transmitted, [2] from outside, [3] of those transmitted */
+    int64_t ev[4];
 } job_t;
 
 #define FS(f) (((float*)J->st) + (size_t)(f) * J->N)
@@ -1014,7 +1030,14 @@ static void* job_run(void* arg) {
                         draw(wid, n >> 2, STREAM_COLL << 24, inst, key, oc);
                         float U = unif(oc[n & 3u]);
                         float thr = inside ? P->thr_out[pk->orig] : P->thr_in[pk->orig];
+#if ORC_PLANT == 1
+                        thr = thr + thr;                 /* planted: twice 4 kappa / v */
+#elif ORC_PLANT == 3
+                        thr = P->thr_in[pk->orig];       /* planted: lattice speed on both sides */
+#endif
                         transmit = U <= thr;
+                        J->ev[inside ? 0 : 2]++;
+                        J->ev[inside ? 1 : 3] += transmit;
                     }
                     lastplane = pk->type == AXX_PRIM_PLANE ? bk : -1;
                     /* left outward: outside reflection, transmission from
@@ -1025,6 +1048,9 @@ static void* job_run(void* arg) {
                         float rem = ell - t;
                         float f = rem / (gs * P->L[comp]);
                         float ph = 1.0f - f;
+#if ORC_PLANT == 2
+                        ph = 0.0f;   /* planted: the whole step charged to the far side (defect D2) */
+#endif
                         E = fmaf((float)(i - li) + (ph - lf), P->rate[comp], E);
                         li = i;
                         lf = ph;
@@ -1139,7 +1165,8 @@ int orc_axes_mask(const axx_acq* q) {
 }
 
 static int walk_common(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
-                       int step_end, uint32_t* state, int nthreads, double bin_width, const magjob_t* mag) {
+                       int step_end, uint32_t* state, int nthreads, double bin_width, const magjob_t* mag,
+                       int64_t* ev_out) {
     if (!s || !q || !run || !state) return AXX_ERR_INVALID;
     if (run->n_walkers < 1) return AXX_ERR_INVALID;
     if (step_begin < 0 || step_end > q->n_steps || step_begin > step_end) return AXX_ERR_INVALID;
@@ -1171,8 +1198,11 @@ static int walk_common(const axx_substrate* s, const axx_acq* q, const axx_run* 
     }
     if (nthreads > 1)
         for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
-    for (int t = 0; t < nthreads; ++t)
+    for (int t = 0; t < nthreads; ++t) {
         if (jobs[t].err) rc = jobs[t].err;
+        if (ev_out)
+            for (int e = 0; e < 4; ++e) ev_out[e] += jobs[t].ev[e];
+    }
     free(jobs); free(th);
     oprep_free(&P);
     return rc;
@@ -1180,7 +1210,15 @@ static int walk_common(const axx_substrate* s, const axx_acq* q, const axx_run* 
 
 int orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
              int step_end, uint32_t* state, int nthreads, double bin_width) {
-    return walk_common(s, q, run, step_begin, step_end, state, nthreads, bin_width, NULL);
+    return walk_common(s, q, run, step_begin, step_end, state, nthreads, bin_width, NULL, NULL);
+}
+
+/* orc_walk that also adds the walk's boundary events at circles and
+ * spheres to ev[4]: {from inside, transmitted from inside, from outside,
+ * transmitted from outside} (the decision of boundaryCheck.cuh:94-111). */
+int orc_walk_events(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+                    int step_end, uint32_t* state, int nthreads, double bin_width, int64_t ev[4]) {
+    return walk_common(s, q, run, step_begin, step_end, state, nthreads, bin_width, NULL, ev);
 }
 
 /* Magnetization walk of fields [group*AXX_MAG_GROUP, +F) (updateWalkersMag,
@@ -1221,7 +1259,7 @@ int orc_walk_mag(const axx_substrate* s, const axx_field* f, const axx_run* run,
     q.n_steps = f->n_steps; q.dt = f->dt; q.gamma = f->gamma;
     q.n_profiles = 1; q.profiles = prof;
     q.n_waveforms = 1; q.wf_profile = &wfp; q.wf_amplitude = &amp; q.wf_dir = dir;
-    int rc = walk_common(s, &q, run, step_begin, step_end, state, nthreads, bin_width, &G);
+    int rc = walk_common(s, &q, run, step_begin, step_end, state, nthreads, bin_width, &G, NULL);
     free(off); free(iT1); free(iT2); free(prof);
     return rc;
 }
@@ -1247,6 +1285,28 @@ void orc_reduce_mag(const uint32_t* state, int64_t n_walkers, int F, double* out
 /* Signal sums in fp64 (phaseAcquisitionStream.cuh:171-189, misc.cuh:21-39):
  *   phi_j = gamma*dt*amp_j * sum_a dir_ja * (m_{p(j),a} + r0_a * S_p),
  *   S_p = sum_i profile_p(t_i);  weight = exp(-(E + tail*rate)). */
+/* T2 exponent of walker w at the end of an n_steps walk: the stays summed
+ * so far plus the last one, (n - li - lf) steps in the compartment it is in
+ * (kernelLattice.cuh:87-96 with the true split; T2factor = -exponent). */
+static double t2_exponent(const oprep* P, const uint32_t* state, int64_t N, int n, int64_t w) {
+    const float* F = (const float*)state;
+    const int32_t* I = (const int32_t*)state;
+    const int comp = I[AXX_F_COMP * N + w];
+    const int li = I[AXX_F_LI * N + w];
+    return (double)F[AXX_F_E * N + w] + ((double)(n - li) - (double)F[AXX_F_LF * N + w]) * (double)P->rate[comp];
+}
+
+int orc_exponents(const axx_substrate* s, const axx_acq* q, const axx_run* run, const uint32_t* state,
+                  double* out) {
+    if (!s || !q || !run || !state || !out) return AXX_ERR_INVALID;
+    oprep P;
+    int rc = oprep_build(&P, s, q, 0.0, 1.0);
+    if (rc) { oprep_free(&P); return rc; }
+    for (int64_t w = 0; w < run->n_walkers; ++w) out[w] = t2_exponent(&P, state, run->n_walkers, q->n_steps, w);
+    oprep_free(&P);
+    return AXX_OK;
+}
+
 int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
                const uint32_t* state, axx_result* out) {
     if (!s || !q || !run || !state || !out) return AXX_ERR_INVALID;
@@ -1264,10 +1324,7 @@ int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
     int64_t nintra = 0;
     uint64_t coll = 0, tran = 0, capn = 0;
     for (int64_t w = 0; w < N; ++w) {
-        int comp = I[AXX_F_COMP * N + w];
-        int li = I[AXX_F_LI * N + w];
-        double Ef = (double)F[AXX_F_E * N + w] +
-                    ((double)(n - li) - (double)F[AXX_F_LF * N + w]) * (double)P.rate[comp];
+        double Ef = t2_exponent(&P, state, N, n, w);
         double wt = run->relaxation ? exp(-Ef) : 1.0;
         int intra = I[AXX_F_COMP0 * N + w] != 0;
         nintra += intra;

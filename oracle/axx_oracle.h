@@ -23,6 +23,14 @@ int   orc_walk(const axx_substrate* s, const axx_acq* q, const axx_run* run, int
                int step_end, uint32_t* state, int nthreads, double bin_width);
 int   orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
                  const uint32_t* state, axx_result* out);
+/* orc_walk that also counts boundary events at circles and spheres:
+ * ev += {from inside, transmitted of those, from outside, transmitted}. */
+int   orc_walk_events(const axx_substrate* s, const axx_acq* q, const axx_run* run, int step_begin,
+                      int step_end, uint32_t* state, int nthreads, double bin_width, int64_t ev[4]);
+/* Per-walker T2 exponent (the T2factor of kernelLattice.cuh:101-103, negated)
+ * of a finished walk. */
+int   orc_exponents(const axx_substrate* s, const axx_acq* q, const axx_run* run, const uint32_t* state,
+                    double* out);
 /* Magnetization walk of one group of fields (state: AXX_F_MOM + 6 F words
  * per walker) and its sums; the double-precision sine/cosine and square
  * root it uses, elementwise. */

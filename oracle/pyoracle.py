@@ -5,6 +5,7 @@ the CPU-baseline leg of ``bench.py`` — never by the product package.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -28,30 +29,55 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        u32p = C.POINTER(C.c_uint32)
-        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
-        L.orc_unif.argtypes = [C.c_uint32]
-        L.orc_unif.restype = C.c_float
-        L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_neg2log_unif.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
-        L.orc_sincos64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
-        L.orc_sqrt_cr.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
-        L.orc_walk_mag.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Field), C.POINTER(abi.Run),
-                                   C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_double]
-        L.orc_reduce_mag.argtypes = [C.POINTER(C.c_uint32), C.c_int64, C.c_int, C.c_void_p]
-        L.orc_validate.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
-        L.orc_walk.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
-                               C.c_int, C.c_int, u32p, C.c_int, C.c_double]
-        L.orc_reduce.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
-                                 u32p, C.POINTER(abi.Result)]
-        L.orc_image_count.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
-        L.orc_axes_mask.argtypes = [C.POINTER(abi.Acq)]
-        L.orc_hit_test.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
-                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
-                                   C.POINTER(C.c_float)]
-        _LIB = L
+        _LIB = _configure(C.CDLL(path))
     return _LIB
+
+
+@contextlib.contextmanager
+def planted(variant: int, workdir: str):
+    """The oracle rebuilt with the deliberately wrong variant ``variant``
+    (ORC_PLANT in axx_oracle.c) into ``workdir`` and used by every function
+    of this module inside the block: tests show their checks fail on it."""
+    global _LIB
+    so = os.path.join(workdir, f"liboracle_plant{int(variant)}.so")
+    subprocess.run(["gcc", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+                    f"-DORC_PLANT={int(variant)}", "-o", so, os.path.join(_HERE, "axx_oracle.c"),
+                    "-shared", "-lpthread", "-lm"], check=True)
+    saved = lib()
+    _LIB = _configure(C.CDLL(so))
+    try:
+        yield
+    finally:
+        _LIB = saved
+
+
+def _configure(L):
+    u32p = C.POINTER(C.c_uint32)
+    L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    L.orc_unif.argtypes = [C.c_uint32]
+    L.orc_unif.restype = C.c_float
+    L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_neg2log_unif.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.orc_sincos64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+    L.orc_sqrt_cr.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.orc_walk_mag.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Field), C.POINTER(abi.Run),
+                               C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_double]
+    L.orc_reduce_mag.argtypes = [C.POINTER(C.c_uint32), C.c_int64, C.c_int, C.c_void_p]
+    L.orc_validate.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
+    L.orc_walk.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                           C.c_int, C.c_int, u32p, C.c_int, C.c_double]
+    L.orc_reduce.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                             u32p, C.POINTER(abi.Result)]
+    L.orc_image_count.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq)]
+    L.orc_axes_mask.argtypes = [C.POINTER(abi.Acq)]
+    L.orc_hit_test.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                               C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                               C.POINTER(C.c_float)]
+    L.orc_walk_events.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                                  C.c_int, C.c_int, u32p, C.c_int, C.c_double, C.POINTER(C.c_int64)]
+    L.orc_exponents.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
+                                u32p, C.c_void_p]
+    return L
 
 
 class OracleError(RuntimeError):
@@ -203,3 +229,31 @@ def reduce(sub, acq, run, state) -> abi.Signals:
 def run_phase(sub, acq, run, threads=None, bin_width=0.0):
     st = walk(sub, acq, run, threads=threads, bin_width=bin_width)
     return reduce(sub, acq, run, st), st
+
+
+def walk_events(sub, acq, run, step_begin=0, step_end=None, state=None, threads=None, bin_width=0.0):
+    """walk() that also returns the boundary events at circles and spheres:
+    (from inside, transmitted from inside, from outside, transmitted from
+    outside)."""
+    if step_end is None:
+        step_end = acq.n_steps
+    if state is None:
+        state = new_state(run.n_walkers, acq.n_profiles)
+    if threads is None:
+        threads = os.cpu_count() or 1
+    ev = (C.c_int64 * 4)()
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    rc = lib().orc_walk_events(C.byref(s), C.byref(q), C.byref(r), int(step_begin), int(step_end),
+                               state.ctypes.data_as(C.POINTER(C.c_uint32)), int(threads), float(bin_width), ev)
+    _check(rc, "orc_walk_events")
+    return state, tuple(int(v) for v in ev)
+
+
+def exponents(sub, acq, run, state) -> np.ndarray:
+    """Per-walker T2 exponent of a finished walk (signal weight exp(-x))."""
+    out = np.empty(run.n_walkers, np.float64)
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    rc = lib().orc_exponents(C.byref(s), C.byref(q), C.byref(r), state.ctypes.data_as(C.POINTER(C.c_uint32)),
+                             out.ctypes.data)
+    _check(rc, "orc_exponents")
+    return out

@@ -4,7 +4,7 @@ Same seeded inputs on both sides; integer fields (compartment, collision,
 transmission and bounce-cap counts) must be identical, float fields within
 1e-5 relative; signal sums within 1e-12 relative.  Cases cover the
 reference's boundary policies (kernelLattice.cuh:64-79,
-boundaryCheck.cuh:138-217), initial placements (kernelLattice.cuh:29-37),
+boundaryCheck.cuh:34-64, :66-113), initial placements (kernelLattice.cuh:29-37),
 cylinders and spheres, relaxation, chunked (checkpointed) walks.
 """
 import math

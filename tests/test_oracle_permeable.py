@@ -1,0 +1,181 @@
+"""Pins of the oracle's permeable-membrane physics: what the reference's
+boundaryCheckPermeable (src/kernels/boundaryCheck.cuh:66-113) and the T2
+bookkeeping of updateWalkersLatticePhase (src/kernels/kernelLattice.cuh:87-96)
+fix, checked against rules stated independently of the oracle's code:
+
+  * a boundary event transmits with probability 4 kappa / v, v the speed on
+    the side the walker comes from (boundaryCheck.cuh:76,94): counted per side
+    with D_in != D_ex, within 4 binomial sigma;
+  * after a transmission the walker goes on along the same direction with the
+    far side's speed for the rest of dt (boundaryCheck.cuh:83,109): the end
+    point of a one-event step sits where that rule puts it;
+  * the T2 exponent of a step that crosses a membrane is the time before the
+    crossing over T2 of the near side plus the time after over T2 of the far
+    side (kernelLattice.cuh:87-92, with the crossing time the reference meant
+    to pass, defect D2): every walker's exponent equals the sum rebuilt from
+    its trajectory, the crossing found by an fp64 ray-circle intersection;
+  * with T2 equal on both sides every exponent is (n - 1) dt / T2;
+  * from a uniform start, the mean exponent is (n - 1) dt (f/T2_in +
+    (1 - f)/T2_ex), f the intra-axonal area fraction (the uniform density is
+    the equilibrium of the 4 kappa / v rule).
+
+Each check is also run on oracles rebuilt with a planted mistake (ORC_PLANT
+in oracle/axx_oracle.c: twice the probability; the reference's whole-step
+charge to the far side; the lattice speed on both sides) and must fail on
+each of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from axonexa_amd import abi
+from axonexa_amd import substrates as sb
+from axonexa_amd import waveforms as wf
+from oracle import pyoracle as po
+
+GAMMA = 2.675e5
+DT = 1e-2
+D_IN, D_EX = 1.0, 2.5
+
+
+def _v(D):
+    return math.sqrt(6.0 * D / DT)
+
+
+def _acq(n_steps):
+    return wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], n_steps, DT, GAMMA)
+
+
+def check_rate_per_side():
+    """Transmitted fraction of the events from each side vs 4 kappa / v_side."""
+    kappa = sb.kappa_for_probability(0.4, D_IN, DT)
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=D_IN, D_ex=D_EX, kappa=kappa)
+    run = abi.RunSpec(n_walkers=4000, seed=17, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    _, ev = po.walk_events(sub, _acq(1000), run, bin_width=1.0)
+    n_in, t_in, n_out, t_out = ev
+    res = []
+    for n, t, D in ((n_in, t_in, D_IN), (n_out, t_out, D_EX)):
+        p = 4.0 * kappa / _v(D)
+        sd = math.sqrt(p * (1 - p) / max(n, 1))
+        res.append((n > 20000, abs(t / max(n, 1) - p) < 4 * sd, t / max(n, 1), p))
+    ok = all(a and b for a, b, _, _ in res)
+    return ok, res
+
+
+def _one_step_walk(sub, acq, run):
+    """The walk one step per call (chunked walks equal the whole walk bitwise):
+    the state before every step and the final state."""
+    states = [po.walk(sub, acq, run, 0, 1, bin_width=1.0).copy()]
+    for i in range(1, acq.n_steps):
+        states.append(po.walk(sub, acq, run, i, i + 1, state=states[-1].copy(), bin_width=1.0))
+    return states
+
+
+def _first_circle_crossing(s, u, centers, R):
+    """Smallest t > 0 with |s + t u - c| = R over the circles (fp64, xy)."""
+    best = math.inf
+    for c in centers:
+        ox, oy = s[0] - c[0], s[1] - c[1]
+        a = u[0] * u[0] + u[1] * u[1]
+        if a == 0.0:
+            continue
+        b = ox * u[0] + oy * u[1]
+        cc = ox * ox + oy * oy - R * R
+        disc = b * b - a * cc
+        if disc < 0:
+            continue
+        for t in ((-b - math.sqrt(disc)) / a, (-b + math.sqrt(disc)) / a):
+            if t > 1e-9 and t < best:
+                best = t
+    return best
+
+
+def check_split_follows_geometry():
+    """Every walker's exponent rebuilt from its trajectory, and the end point of
+    every one-event transmission step placed by the far-side speed."""
+    cell, R = 8.0, 2.5
+    T2 = {0: 90.0, 1: 20.0}        # compartment 0 = outside, 1 = inside
+    Lc = {0: math.sqrt(6 * D_EX * DT), 1: math.sqrt(6 * D_IN * DT)}
+    kappa = sb.kappa_for_probability(0.5, D_IN, DT)
+    sub = sb.single_cylinder(cell, R, D_in=D_IN, D_ex=D_EX, T2_in=T2[1], T2_ex=T2[0], kappa=kappa)
+    acq = _acq(301)
+    run = abi.RunSpec(n_walkers=600, seed=29, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    states = _one_step_walk(sub, acq, run)
+    views = [abi.state_view(s, run.n_walkers, 1) for s in states]
+    centers = [(cell / 2 + kx * cell, cell / 2 + ky * cell) for kx in (-1, 0, 1) for ky in (-1, 0, 1)]
+    expect = np.zeros(run.n_walkers)
+    skip = np.zeros(run.n_walkers, bool)
+    crossings, worst_len = 0, 0.0
+    for i in range(1, acq.n_steps):
+        a, b = views[i - 1], views[i]
+        dcoll = b["ncoll"].astype(np.int64) - a["ncoll"]
+        dtran = b["ntran"].astype(np.int64) - a["ntran"]
+        d = b["disp"].astype(np.float64) - a["disp"]
+        for w in range(run.n_walkers):
+            c0, c1 = int(a["comp"][w]), int(b["comp"][w])
+            if dtran[w] == 0:
+                expect[w] += DT / T2[c0]
+                continue
+            if dtran[w] != 1 or dcoll[w] != 1:
+                skip[w] = True
+                continue
+            s = a["pos"][w].astype(np.float64)
+            dl = float(np.linalg.norm(d[w]))
+            u = d[w] / dl
+            t = _first_circle_crossing(s, u, centers, R)
+            f_pre = t / Lc[c0]
+            expect[w] += DT * (f_pre / T2[c0] + (1.0 - f_pre) / T2[c1])
+            worst_len = max(worst_len, abs(dl - (t + (1.0 - f_pre) * Lc[c1])))
+            crossings += 1
+    got = po.exponents(sub, acq, run, states[-1])
+    err = np.abs(got - expect)[~skip]
+    ok = (crossings > 300 and np.mean(skip) < 0.2 and float(err.max()) < 1e-6 and worst_len < 2e-5)
+    return ok, (crossings, float(np.mean(skip)), float(err.max()), worst_len)
+
+
+def check_equal_t2_exponent():
+    kappa = sb.kappa_for_probability(0.3, D_IN, DT)
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=D_IN, D_ex=D_EX, T2_in=60.0, T2_ex=60.0, kappa=kappa)
+    acq = _acq(2001)
+    run = abi.RunSpec(n_walkers=2000, seed=5, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    st = po.walk(sub, acq, run, bin_width=1.0)
+    x = po.exponents(sub, acq, run, st)
+    want = (acq.n_steps - 1) * DT / 60.0
+    ntran = int(abi.state_view(st, run.n_walkers, 1)["ntran"].sum())
+    ok = ntran > 5000 and float(np.max(np.abs(x / want - 1.0))) < 1e-5
+    return ok, (ntran, float(np.max(np.abs(x / want - 1.0))))
+
+
+def check_mean_exponent_at_equilibrium():
+    T2_in, T2_ex = 30.0, 90.0
+    kappa = sb.kappa_for_probability(0.4, D_IN, DT)
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=D_IN, D_ex=D_EX, T2_in=T2_in, T2_ex=T2_ex, kappa=kappa)
+    acq = _acq(3001)
+    run = abi.RunSpec(n_walkers=20000, seed=41, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    st = po.walk(sub, acq, run, bin_width=1.0)
+    x = po.exponents(sub, acq, run, st)
+    f = sb.hex_intra_fraction(2.5, 6.0)
+    want = (acq.n_steps - 1) * DT * (f / T2_in + (1 - f) / T2_ex)
+    sd = float(x.std()) / math.sqrt(run.n_walkers)
+    ok = abs(float(x.mean()) - want) < 4 * sd
+    return ok, (float(x.mean()), want, sd)
+
+
+CHECKS = [check_rate_per_side, check_split_follows_geometry, check_equal_t2_exponent,
+          check_mean_exponent_at_equilibrium]
+
+
+@pytest.mark.parametrize("check", CHECKS, ids=lambda c: c.__name__)
+def test_oracle_permeable_physics(check):
+    ok, detail = check()
+    assert ok, detail
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_planted_mistakes_fail_a_check(plant, tmp_path):
+    # 1: twice 4 kappa / v; 2: whole crossing step charged to the far side
+    # (D2); 3: the lattice speed in the threshold from both sides
+    with po.planted(plant, str(tmp_path)):
+        results = {c.__name__: c()[0] for c in CHECKS}
+    assert not all(results.values()), results

@@ -14,7 +14,7 @@ where no golden vectors exist: the reference ships no tests or fixtures).
   * binned and linear-scan candidate searches give identical walkers;
   * chunked (checkpointed) walks and thread counts give identical walkers.
 Reference semantics: src/kernels/kernelLattice.cuh:1-104,
-src/kernels/boundaryCheck.cuh:138-217.
+src/kernels/boundaryCheck.cuh:34-64, :66-113.
 """
 import math
 
