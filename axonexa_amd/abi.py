@@ -43,6 +43,7 @@ INIT_COMPARTMENT = 1
 INIT_INSIDE_PRIM = 2
 
 MAX_PROFILES = 64
+BATCH_MAX_PROFILES = 32     # AXX_BATCH_MAX_PROFILES: profiles per instance of a batched run
 MAX_PRIMS = 32768
 
 F_X, F_Y, F_Z, F_DX, F_DY, F_DZ, F_X0, F_Y0, F_Z0, F_E, F_LF = range(11)

@@ -21,23 +21,33 @@
 #include "init_kernel.h"
 
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 
 namespace axx {
 
-// Instance kernel variants: padded profile width 1, 4 or 8, every gradient
-// axis and step generator, reflecting/permeable or rejection boundaries,
-// cylinders-only or general geometry.  class = 4 * width_index + 2 * rej + gen.
+// Instance kernel variants: padded profile width 1, 4, 8, 16 or 32, every
+// gradient axis and step generator, reflecting/permeable or rejection
+// boundaries, cylinders-only or general geometry.
+// class = 4 * width_index + 2 * rej + gen.  Classes 0-11 (widths 1-8) run in
+// the narrow kernel, 12-19 (widths 16, 32: up to 96 moments per lane) in the
+// wide kernel, which has a register file of its own to keep them in.
 constexpr int kBatchMaxProfiles = AXX_BATCH_MAX_PROFILES;
 constexpr int kBatchBlockMax = 512;
+constexpr int kBatchWidths[] = {1, 4, 8, 16, 32};
+constexpr int kBatchClasses = 4 * (int)(sizeof(kBatchWidths) / sizeof(kBatchWidths[0]));
+constexpr int kNarrowClasses = 12;
+static_assert(kBatchWidths[4] == kBatchMaxProfiles, "widest batch class = the batch profile limit");
+
+constexpr int class_width(int c) { return kBatchWidths[c >> 2]; }
 
 struct BatchWalkArgs {
     const WalkArgs* __restrict__ args;   // [n_inst]
     const int* __restrict__ cls;         // [n_inst]
-    const int4* __restrict__ tiles;      // [n_tiles] (instance, w0, w1, 0)
-    int n_tiles;
-    unsigned int* __restrict__ next;     // tile counter, zeroed before the launch
+    const int4* __restrict__ tiles;      // [tile0, tile0 + n_tiles): (instance, w0, w1, 0)
+    int tile0, n_tiles;
+    unsigned int* __restrict__ next;     // this kernel's tile counter, zeroed before the launch
 };
 
 template <int PT, bool GEN, bool REJ, bool ALT>
@@ -70,7 +80,7 @@ __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(
         const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= B.n_tiles) break;
-        const int4 tile = B.tiles[t];
+        const int4 tile = B.tiles[B.tile0 + t];
         const int i = __builtin_amdgcn_readfirstlane(tile.x);
         const int w0 = __builtin_amdgcn_readfirstlane(tile.y);
         const int w1 = __builtin_amdgcn_readfirstlane(tile.z);
@@ -106,6 +116,44 @@ __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+// Instances of profile width 16 or 32 (classes 12-19): the same persistent
+// tile loop with a wave-uniform class switch.  512-thread bounds leave each
+// lane up to 256 VGPRs, so the 48 or 96 moments stay in registers.
+template <bool ALT>
+__global__ __launch_bounds__(kBatchBlockMax) void batch_wide_kernel(BatchWalkArgs B) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_tile;
+    int staged = -1;
+    LdsTables T{};
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(B.next, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t >= B.n_tiles) break;
+        const int4 tile = B.tiles[B.tile0 + t];
+        const int i = __builtin_amdgcn_readfirstlane(tile.x);
+        const int w0 = __builtin_amdgcn_readfirstlane(tile.y);
+        const int w1 = __builtin_amdgcn_readfirstlane(tile.z);
+        const WalkArgs A = B.args[i];
+        const int c = __builtin_amdgcn_readfirstlane(B.cls[i]);
+        if (i != staged) {
+            T = (c & 1) ? stage_tables<true>(A, lds) : stage_tables<false>(A, lds);
+            staged = i;
+        }
+        switch (c) {
+        case 12: batch_tile<16, false, false, ALT>(A, T, w0, w1); break;
+        case 13: batch_tile<16, true, false, ALT>(A, T, w0, w1); break;
+        case 14: batch_tile<16, false, true, ALT>(A, T, w0, w1); break;
+        case 15: batch_tile<16, true, true, ALT>(A, T, w0, w1); break;
+        case 16: batch_tile<32, false, false, ALT>(A, T, w0, w1); break;
+        case 17: batch_tile<32, true, false, ALT>(A, T, w0, w1); break;
+        case 18: batch_tile<32, false, true, ALT>(A, T, w0, w1); break;
+        default: batch_tile<32, true, true, ALT>(A, T, w0, w1); break;
+        }
+    }
+}
+
 // Placement: one block per tile.
 __global__ __launch_bounds__(256) void batch_init_kernel(const WalkArgs* __restrict__ args,
                                                          const int* __restrict__ n_mom,
@@ -121,11 +169,11 @@ __global__ __launch_bounds__(256) void batch_init_kernel(const WalkArgs* __restr
     for (int w = tile.y + (int)threadIdx.x; w < tile.z; w += (int)blockDim.x) init_lane(A, geom, meta, w, nm);
 }
 
-// Zero the tile counter and every instance's event counters (a kernel, not
-// a memset node, so the graph replays it reliably).
+// Zero the two tile counters and every instance's event counters (a
+// kernel, not a memset node, so the graph replays it reliably).
 __global__ void batch_zero_kernel(unsigned int* next, unsigned long long* const* __restrict__ cnt, int n_inst) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *next = 0u;
+    if (t < 2) next[t] = 0u;
     if (t < 4 * n_inst) cnt[t >> 2][t & 3] = 0ull;
 }
 
@@ -157,7 +205,9 @@ struct axx_batch {
     std::vector<int> cls;
     bool alt = false;   // some instance takes axis-sign or Gaussian steps
     int uniform = -1;   // the class every instance has with isotropic steps, else -1
-    int block = 256, wpt = 1, grid = 0, n_tiles = 0, n_red = 0;
+    int block = 256, wpt = 1, n_red = 0;
+    int grid = 0, n_tiles = 0;          // narrow kernel (classes 0-11): grid, tiles
+    int grid_w = 0, n_tiles_w = 0;      // wide kernel (classes 12-19), tiles after the narrow ones
     size_t lds_walk = 0, lds_init = 0;
     std::string signature;
     // device tables
@@ -194,22 +244,30 @@ std::mutex& tune_mutex() {
 typedef void (*batch_walk_fn)(BatchWalkArgs);
 template <int C>
 batch_walk_fn uniform_kernel(int c) {
-    if constexpr (C < 12)
+    if constexpr (C < kNarrowClasses)
         return c == C ? (batch_walk_fn)batch_walk_kernel<false, C> : uniform_kernel<C + 1>(c);
     else
         return nullptr;
 }
 batch_walk_fn walk_kernel_of(const axx_batch* b) {
-    if (b->uniform >= 0) return uniform_kernel<0>(b->uniform);
+    if (b->uniform >= 0 && b->uniform < kNarrowClasses) return uniform_kernel<0>(b->uniform);
     return b->alt ? (batch_walk_fn)batch_walk_kernel<true, -1> : (batch_walk_fn)batch_walk_kernel<false, -1>;
 }
-
-int class_of(const axx_plan* p) {
-    const int wi = p->Pt <= 1 ? 0 : (p->Pt <= 4 ? 1 : 2);
-    return 4 * wi + (p->run.boundary_mode == AXX_BOUNDARY_REJECTION ? 2 : 0) + (p->gen ? 1 : 0);
+batch_walk_fn wide_kernel_of(const axx_batch* b) {
+    return b->alt ? (batch_walk_fn)batch_wide_kernel<true> : (batch_walk_fn)batch_wide_kernel<false>;
 }
 
-int min_pt_for(int P) { return P <= 1 ? 1 : (P <= 4 ? 4 : 8); }
+// Smallest batch width >= P (the padded profile width of the instance).
+constexpr int width_index_for(int P) {
+    int wi = 0;
+    while (wi < 4 && kBatchWidths[wi] < P) ++wi;
+    return wi;
+}
+int min_pt_for(int P) { return kBatchWidths[width_index_for(P)]; }
+
+int class_of(const axx_plan* p) {
+    return 4 * width_index_for(p->Pt) + (p->run.boundary_mode == AXX_BOUNDARY_REJECTION ? 2 : 0) + (p->gen ? 1 : 0);
+}
 
 void batch_free_graph(axx_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -242,38 +300,63 @@ int upload(axx_ctx* ctx, T** dst, const std::vector<T>& v) {
     return AXX_OK;
 }
 
-// Relative cost of one walker-step of a class (moments dominate for wide
-// profile sets).
-double class_weight(int c) {
-    static const double w[6] = {1.0, 1.2, 1.2, 1.4, 1.45, 1.65};
-    return w[c];
+// Relative cost of one walker-step of a class, for the longest-first tile
+// order: the walk plus 6 P moment flops per step (three axes), general
+// geometry (3D bins, meta words) a little dearer, rejection a little cheaper.
+constexpr double class_weight(int c) {
+    return (1.0 + 6.0 * class_width(c) / 150.0) * ((c & 1) ? 1.15 : 1.0) * ((c & 2) ? 0.9 : 1.0);
 }
+constexpr bool class_weights_ordered() {
+    for (int c = 0; c < kBatchClasses; ++c) {
+        if (!(class_weight(c) > 0.0)) return false;
+        if (c >= 4 && !(class_weight(c) > class_weight(c - 4))) return false;   // wider costs more
+    }
+    return true;
+}
+static_assert(class_weights_ordered(), "every class has a positive weight, increasing with the width");
+static_assert(width_index_for(1) == 0 && width_index_for(2) == 1 && width_index_for(8) == 2 &&
+                  width_index_for(9) == 3 && width_index_for(32) == 4,
+              "profile counts map to their padded batch width");
 
 // Tiles of block * wpt walkers, longest (walker-steps x class weight) first;
-// equal costs keep instance then walker order.
+// equal costs keep instance then walker order.  Tiles of the narrow classes
+// come first, then those of the wide classes; each kernel takes its own part.
 int build_tiles(axx_batch* b, int block, int wpt) {
     axx_ctx* ctx = b->ctx;
     const long long T = (long long)block * wpt;
     struct Tile { double cost; int inst, w0, w1; };
-    std::vector<Tile> tiles;
+    std::vector<Tile> tiles[2];
     for (int i = 0; i < (int)b->plans.size(); ++i) {
         const axx_plan* p = b->plans[i];
+        const int c = b->cls[i];
         for (long long w0 = 0; w0 < p->N; w0 += T) {
             const long long w1 = std::min<long long>(p->N, w0 + T);
-            tiles.push_back({(double)(w1 - w0) * (p->n_steps - 1) * class_weight(b->cls[i]), i, (int)w0,
-                             (int)w1});
+            tiles[c >= kNarrowClasses].push_back(
+                {(double)(w1 - w0) * (p->n_steps - 1) * class_weight(c), i, (int)w0, (int)w1});
         }
     }
-    std::stable_sort(tiles.begin(), tiles.end(), [](const Tile& x, const Tile& y) { return x.cost > y.cost; });
-    std::vector<int4> h(tiles.size());
-    for (size_t k = 0; k < tiles.size(); ++k) h[k] = make_int4(tiles[k].inst, tiles[k].w0, tiles[k].w1, 0);
-    b->n_tiles = (int)h.size();
+    std::vector<int4> h;
+    for (auto& part : tiles) {
+        std::stable_sort(part.begin(), part.end(), [](const Tile& x, const Tile& y) { return x.cost > y.cost; });
+        for (const Tile& t : part) h.push_back(make_int4(t.inst, t.w0, t.w1, 0));
+    }
+    b->n_tiles = (int)tiles[0].size();
+    b->n_tiles_w = (int)tiles[1].size();
     int rc = upload(ctx, &b->d_tiles, h);
     if (rc) return rc;
-    int per_cu = 0;
-    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_kernel_of(b), block, b->lds_walk));
-    if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "batch kernel does not fit a CU at block %d", block);
-    b->grid = std::max(1, std::min(per_cu * std::max(ctx->n_cu, 1), b->n_tiles));
+    b->grid = b->grid_w = 0;
+    if (b->n_tiles > 0) {
+        int per_cu = 0;
+        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_kernel_of(b), block, b->lds_walk));
+        if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "batch kernel does not fit a CU at block %d", block);
+        b->grid = std::max(1, std::min(per_cu * std::max(ctx->n_cu, 1), b->n_tiles));
+    }
+    if (b->n_tiles_w > 0) {
+        int per_cu = 0;
+        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wide_kernel_of(b), block, b->lds_walk));
+        if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "wide batch kernel does not fit a CU at block %d", block);
+        b->grid_w = std::max(1, std::min(per_cu * std::max(ctx->n_cu, 1), b->n_tiles_w));
+    }
     b->block = block;
     b->wpt = wpt;
     return AXX_OK;
@@ -285,12 +368,19 @@ int launch_walk_part(axx_batch* b, const WalkArgs* args) {
     hipLaunchKernelGGL(batch_zero_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, ctx->stream, b->d_next,
                        b->d_cnt, n);
     HIP_TRY(ctx, hipGetLastError());
-    hipLaunchKernelGGL(batch_init_kernel, dim3(b->n_tiles), dim3(256), b->lds_init, ctx->stream,
+    hipLaunchKernelGGL(batch_init_kernel, dim3(b->n_tiles + b->n_tiles_w), dim3(256), b->lds_init, ctx->stream,
                        (const WalkArgs*)b->d_args, (const int*)b->d_nmom, (const int4*)b->d_tiles);
     HIP_TRY(ctx, hipGetLastError());
-    BatchWalkArgs B{args, b->d_cls, b->d_tiles, b->n_tiles, b->d_next};
-    hipLaunchKernelGGL(walk_kernel_of(b), dim3(b->grid), dim3(b->block), b->lds_walk, ctx->stream, B);
-    HIP_TRY(ctx, hipGetLastError());
+    if (b->n_tiles > 0) {
+        BatchWalkArgs B{args, b->d_cls, b->d_tiles, 0, b->n_tiles, b->d_next};
+        hipLaunchKernelGGL(walk_kernel_of(b), dim3(b->grid), dim3(b->block), b->lds_walk, ctx->stream, B);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (b->n_tiles_w > 0) {
+        BatchWalkArgs B{args, b->d_cls, b->d_tiles, b->n_tiles, b->n_tiles_w, b->d_next + 1};
+        hipLaunchKernelGGL(wide_kernel_of(b), dim3(b->grid_w), dim3(b->block), b->lds_walk, ctx->stream, B);
+        HIP_TRY(ctx, hipGetLastError());
+    }
     return AXX_OK;
 }
 
@@ -328,8 +418,22 @@ bool valid_config(int blk, int wpt) {
     return blk >= 64 && blk <= kBatchBlockMax && blk % 64 == 0 && wpt >= 1 && wpt <= 64;
 }
 
-// call with tune_mutex held; entries that are not a valid configuration
-// (a damaged file) are skipped
+// Cache key of a batch signature: its 64-bit FNV-1a hash as 16 hex digits
+// (signatures of large batches run to many kilobytes; the key has a fixed
+// length, so every file line is short and reads back whole).
+std::string cache_key(const std::string& sig) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (unsigned char ch : sig) {
+        h ^= ch;
+        h *= 0x100000001b3ull;
+    }
+    char buf[17];
+    std::snprintf(buf, sizeof(buf), "%016llx", (unsigned long long)h);
+    return buf;
+}
+
+// call with tune_mutex held; lines that are not "key block wpt" with a valid
+// configuration (a damaged file) are skipped
 void load_file_cache() {
     static bool loaded = false;
     if (loaded) return;
@@ -338,25 +442,33 @@ void load_file_cache() {
     if (!path) return;
     FILE* f = std::fopen(path, "r");
     if (!f) return;
-    char key[4096];
-    int blk, wpt;
-    while (std::fscanf(f, "%4095s %d %d", key, &blk, &wpt) == 3)
-        if (valid_config(blk, wpt)) tune_cache()[key] = {blk, wpt};
+    char line[256];
+    while (std::fgets(line, sizeof(line), f)) {
+        char key[32];
+        int blk, wpt;
+        if (std::sscanf(line, "%31s %d %d", key, &blk, &wpt) == 3 && std::strlen(key) == 16 &&
+            valid_config(blk, wpt))
+            tune_cache()[key] = {blk, wpt};
+        // a line longer than the buffer: skip its rest
+        while (!std::strchr(line, '\n') && std::fgets(line, sizeof(line), f)) {
+        }
+    }
     std::fclose(f);
 }
 
-bool cache_lookup(const std::string& key, int& blk, int& wpt) {
+bool cache_lookup(const std::string& sig, int& blk, int& wpt) {
     std::lock_guard<std::mutex> lock(tune_mutex());
     load_file_cache();
-    auto it = tune_cache().find(key);
+    auto it = tune_cache().find(cache_key(sig));
     if (it == tune_cache().end()) return false;
     blk = it->second.first;
     wpt = it->second.second;
     return true;
 }
 
-void cache_store(const std::string& key, int blk, int wpt) {
+void cache_store(const std::string& sig, int blk, int wpt) {
     std::lock_guard<std::mutex> lock(tune_mutex());
+    const std::string key = cache_key(sig);
     tune_cache()[key] = {blk, wpt};
     const char* path = std::getenv("AXX_TUNE_CACHE");
     if (!path) return;
@@ -463,7 +575,7 @@ extern "C" int axx_batch_create(axx_ctx* ctx, int n_inst, const axx_instance* in
             batch_free(b);
             return rc;
         }
-        if (hipMalloc((void**)&b->d_next, sizeof(unsigned int)) != hipSuccess ||
+        if (hipMalloc((void**)&b->d_next, 2 * sizeof(unsigned int)) != hipSuccess ||
             hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
             batch_free(b);
             return fail(ctx, AXX_ERR_DEVICE, "batch: device allocation");
@@ -578,8 +690,8 @@ extern "C" int axx_batch_get_config(const axx_batch* b, int* block, int* wpt, in
         if (!b) return AXX_ERR_INVALID;
         if (block) *block = b->block;
         if (wpt) *wpt = b->wpt;
-        if (grid) *grid = b->grid;
-        if (n_tiles) *n_tiles = b->n_tiles;
+        if (grid) *grid = b->grid + b->grid_w;
+        if (n_tiles) *n_tiles = b->n_tiles + b->n_tiles_w;
         return AXX_OK;
     } catch (...) {
         return api_exception();

@@ -638,7 +638,7 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     A.step_mode = p->run.step_mode;
     A.init_img = p->init_img;
     A.i0 = i0; A.i1 = i1;
-    A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : 256;
+    A.cap = p->run.bounce_cap > 0 ? p->run.bounce_cap : AXX_DEFAULT_BOUNCE_CAP;
     A.err = p->d_err;
     return A;
 }

@@ -138,10 +138,15 @@ def c3_functors(n_dirs: int = 30, amps=(0.1, 0.2, 0.3), total: float = 40.0):
     return fs
 
 
+C3_GAP = 0.08   # um, minimum surface-to-surface distance of the gamma packing
+
+
 def c3_gamma(n_walkers: int = 4000000, n_steps: int = 80000, dt: float = 5e-4,
              seed: int = 4) -> Workload:
+    # surfaces at least one extra-axonal step length (0.0775 um) apart, so no
+    # walker is caught in the cusp between two nearly touching cylinders
     sub = sb.gamma_packing(400, 5.0, 0.4, 0.65, D_in=1.7, D_ex=2.0, seed=seed, T2_in=70.0,
-                           T2_ex=50.0, rmin=0.25)
+                           T2_ex=50.0, rmin=0.25, gap=C3_GAP)
     acq = wf.build_acq(c3_functors(), n_steps, dt, GAMMA)
     run = abi.RunSpec(n_walkers=n_walkers, seed=1234, relaxation=1)
     return Workload("c3_gamma", sub, acq, run,
@@ -150,8 +155,8 @@ def c3_gamma(n_walkers: int = 4000000, n_steps: int = 80000, dt: float = 5e-4,
 
 def _c4_waveforms(rng: np.random.Generator, n_steps: int, dt: float):
     """4..32 waveforms: PGSE (one timing, random directions, linear
-    amplitudes) or cosine OGSE (2, 4 or 8 frequencies x amplitudes along a
-    random axis)."""
+    amplitudes) or cosine OGSE (2, 4, 8, 16 or 32 frequencies, at most one
+    per waveform, x amplitudes along a random axis)."""
     T = (n_steps - 1) * dt
     M = int(rng.integers(4, 33))
     gmax = rng.uniform(0.05, 0.3) * T_PER_M
@@ -164,7 +169,7 @@ def _c4_waveforms(rng: np.random.Generator, n_steps: int, dt: float):
         for j in range(M):
             fs.append(wf.RectGrad(gmax * (j + 1) / M, delta, gap, dirs[j]))
     else:
-        nf = int(rng.choice([2, 4, 8]))
+        nf = min(int(rng.choice([2, 4, 8, 16, 32])), M)
         lobe = rng.uniform(0.3, 0.48) * T
         gap = rng.uniform(0.0, 1.0) * (T - 2 * lobe - 2 * dt)
         axis = np.zeros(3)

@@ -147,9 +147,7 @@ def main(argv=None) -> int:
         kw["n_steps"] = args.n_steps
     wl = make(**kw)
     n = wl.acq.n_steps
-    plan = engine.Plan(wl.sub, wl.acq, wl.run)
-    if args.block:
-        plan.set_config(args.block)
+    plan = engine.Plan(wl.sub, wl.acq, wl.run, block=args.block, lds_budget=args.lds_budget)
     cfg = plan.config()
 
     for _ in range(args.warmup):

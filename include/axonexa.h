@@ -58,6 +58,12 @@ extern "C" {
 
 /* Limits compiled into the kernels. */
 #define AXX_MAX_PROFILES   64   /* distinct waveform time profiles per run      */
+/* Boundary events allowed in one step before it is cut short (counted in
+ * axx_result.cap_hits).  The reference loops without bound
+ * (boundaryCheck.cuh:45,79); a walker sliding along the inside of a surface
+ * at a grazing angle a needs about L / (2 R a) reflections for a step of
+ * length L, so the cap only ends such grazing runs. */
+#define AXX_DEFAULT_BOUNCE_CAP 4096
 #define AXX_MAX_PRIMS      32768
 
 /*
@@ -115,7 +121,7 @@ typedef struct axx_run {
     int init_arg;
     int boundary_mode;       /* AXX_BOUNDARY_*                                   */
     int relaxation;          /* 1: weight by exp(-sum dt/T2) (USE_RELAXATION)    */
-    int bounce_cap;          /* max boundary events per step (0 -> 256)          */
+    int bounce_cap;          /* max boundary events per step (0 -> AXX_DEFAULT_BOUNCE_CAP) */
     int step_mode;           /* AXX_STEP_*                                       */
 } axx_run;
 
@@ -261,7 +267,7 @@ void axx_plan_destroy(axx_plan* plan);
  * boundary policy and step generator.  Instances keep their own seed / instance id / walker
  * offset, so replicas stay independent.
  */
-#define AXX_BATCH_MAX_PROFILES 8
+#define AXX_BATCH_MAX_PROFILES 32
 typedef struct axx_instance {
     const axx_substrate* substrate;
     const axx_acq* acq;

@@ -873,7 +873,7 @@ static void* job_run(void* arg) {
     const axx_run* R = J->run;
     const int Pn = J->Pn;
     const int mode = R->boundary_mode;
-    const int cap = R->bounce_cap > 0 ? R->bounce_cap : 256;
+    const int cap = R->bounce_cap > 0 ? R->bounce_cap : AXX_DEFAULT_BOUNCE_CAP;
     const uint32_t key[2] = {(uint32_t)R->seed, (uint32_t)(R->seed >> 32)};
     const uint32_t inst = R->instance;
     float m[AXX_MAX_PROFILES][3];

@@ -321,3 +321,23 @@ def test_gaussian_steps_need_a_wide_cell():
     acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0)], 10, 1.0, GAMMA)      # L = 3.5 um
     with pytest.raises(po.OracleError):
         po.walk(sb.empty_box(30.0, 2.0), acq, abi.RunSpec(n_walkers=4, step_mode=abi.STEP_GAUSSIAN))
+
+
+def test_grazing_slides_along_a_surface_finish_their_step():
+    """Two walkers of the gamma-packing benchmark run (4e6 walkers x 80,000
+    steps) start a step on the inside of a cylinder, within fp32 rounding of
+    it, with a nearly tangent direction and slide along it in short chords.
+    Walker 1428273 needs 364 reflections in step 27699 and finishes it under
+    the default cap (4096; the earlier 256 cut it short).  Walker 3587312
+    (step 62203) also drifts outwards by rounding, 2e-5 um per 100
+    reflections, and is the one step of that run cut short by the cap."""
+    from axonexa_amd import workloads as W
+    w = W.c3_gamma(n_walkers=1)
+    for wid, step, events, capped in ((1428273, 27699, 364, 0), (3587312, 62203, 4096, 1)):
+        run = abi.RunSpec(n_walkers=1, seed=1234, relaxation=1, walker_offset=wid)
+        st = po.walk(w.sub, w.acq, run, 0, step, bin_width=1.5, threads=1)
+        n0 = int(abi.state_view(st, 1, w.acq.n_profiles)["ncoll"][0])
+        st = po.walk(w.sub, w.acq, run, step, step + 1, state=st, bin_width=1.5, threads=1)
+        v = abi.state_view(st, 1, w.acq.n_profiles)
+        assert (int(v["ncoll"][0]) - n0, int(v["ncap"][0])) == (events, capped), wid
+        assert int(v["comp"][0]) == int(v["comp0"][0]) > 0       # still inside its cylinder

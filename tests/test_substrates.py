@@ -35,6 +35,11 @@ def test_gamma_packing_reaches_target_fraction_without_overlap():
     assert s.n_prims == 400
     assert abs(sb.area_fraction(s) - 0.65) < 1e-9
     assert _min_image_min_gap(s) >= -1e-9
+    # the benchmark packing keeps its surfaces a step length apart
+    from axonexa_amd import workloads as W
+    c3 = W.c3_gamma(n_walkers=16, n_steps=2).sub
+    assert abs(sb.area_fraction(c3) - 0.65) < 1e-9
+    assert _min_image_min_gap(c3) >= W.C3_GAP - 1e-9
     # radii follow Gamma(k=5, theta=0.4) (truncation at 0.25 um is negligible)
     assert stats.kstest(s.radii, "gamma", args=(5.0, 0.0, 0.4)).pvalue > 1e-3
 
