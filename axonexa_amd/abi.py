@@ -88,7 +88,8 @@ class Run(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block", C.c_int), ("bin_width", C.c_double), ("lds_budget", C.c_int)]
+    _fields_ = [("block", C.c_int), ("bin_width", C.c_double), ("lds_budget", C.c_int),
+                ("whole_cell", C.c_int)]
 
 
 class Instance(C.Structure):

@@ -90,6 +90,8 @@ struct HostPrep {
     float ib[3] = {1, 1, 1};
     float ib4[2] = {4, 4};
     double bin_width = 0;
+    double cell[3] = {0, 0, 0};  // the cell positions wrap in (a unit cell of a repeating substrate)
+    int units = 1;               // unit cells in the substrate's cell
     double Lmax = 0;
     float plane_eps = 0;
     bool gen = false;
@@ -179,13 +181,111 @@ static int validate(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q) {
     return AXX_OK;
 }
 
-static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], double reach) {
-    const double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
+// The primitives the walk sees and the cell they repeat in.  A substrate
+// whose primitives map onto themselves under a shift of a/mx along x and b/my
+// along y (a hexagonal lattice built as 20 x 25 copies of its two-cylinder
+// unit cell, examples/hexPhase.cu:108-120) is walked in its unit cell
+// (a/mx) x (b/my) x c: the unit cell's periodic images are the whole
+// substrate, so the walk is the same physics with tables 1/(mx my) the size.
+// Each class of translated primitives is represented by its lowest-numbered
+// member, whose compartment id (index + 1) every walker inside any member
+// carries.  No counterpart in the reference, which scans the whole basis
+// (lattice.cuh:184-247).
+struct GeoView {
+    double a = 0, b = 0, c = 0;      // cell the positions wrap in
+    int mx = 1, my = 1;              // unit cells along x and y of the whole cell
+    std::vector<int> orig;           // substrate index of each primitive of the view
+    std::vector<double> center;      // its centre in the view's cell, 3 per primitive
+};
+
+// |d| up to a multiple of L
+static double pdist(double d, double L) { return std::fabs(d - L * std::nearbyint(d / L)); }
+
+static bool same_kind(const axx_substrate* s, int i, int j) {
+    const double ki = s->kappa ? s->kappa[i] : 0.0, kj = s->kappa ? s->kappa[j] : 0.0;
+    return s->type[i] == s->type[j] && s->radius[i] == s->radius[j] && s->D[i] == s->D[j] &&
+           (s->T2[i] == s->T2[j] || (std::isinf(s->T2[i]) && std::isinf(s->T2[j]))) && ki == kj;
+}
+
+// Does the shift by sh along axis ax map every primitive onto one of the same kind?
+static bool shift_maps_onto_itself(const axx_substrate* s, int ax, double sh, double tol) {
+    const double L[3] = {s->a, s->b, s->c};
+    for (int k = 0; k < s->n_prims; ++k) {
+        bool found = false;
+        for (int j = 0; j < s->n_prims && !found; ++j) {
+            if (!same_kind(s, k, j)) continue;
+            bool ok = true;
+            for (int a = 0; a < 3 && ok; ++a)
+                ok = pdist(s->center[3 * k + a] + (a == ax ? sh : 0.0) - s->center[3 * j + a], L[a]) <= tol;
+            found = ok;
+        }
+        if (!found) return false;
+    }
+    return true;
+}
+
+// Largest m dividing the primitive count whose shift L/m along ax is a symmetry.
+static int period_count(const axx_substrate* s, int ax, double tol) {
+    const double L = ax == 0 ? s->a : s->b;
+    for (int m = s->n_prims; m >= 2; --m)
+        if (s->n_prims % m == 0 && shift_maps_onto_itself(s, ax, L / m, tol)) return m;
+    return 1;
+}
+
+static void whole_cell_view(const axx_substrate* s, GeoView& V) {
+    V.a = s->a; V.b = s->b; V.c = s->c;
+    V.mx = V.my = 1;
+    V.orig.resize(s->n_prims);
+    V.center.assign(s->center, s->center + 3 * (size_t)s->n_prims);
+    for (int k = 0; k < s->n_prims; ++k) V.orig[k] = k;
+}
+
+// The unit-cell view when the substrate repeats (no planes) and the unit cell
+// is at least twice every radius and twice the reach of a step pair (reach),
+// else the whole cell.
+static void unit_cell_view(const axx_substrate* s, bool allow, double reach, GeoView& V) {
+    whole_cell_view(s, V);
+    const int n = s->n_prims;
+    if (!allow || n < 2) return;
+    for (int k = 0; k < n; ++k)
+        if (s->type[k] == AXX_PRIM_PLANE) return;
+    const double tol = 1e-9 * std::max(std::max(s->a, s->b), s->c);
+    const int mx = period_count(s, 0, tol), my = period_count(s, 1, tol);
+    if (mx * my == 1) return;
+    const double W = s->a / mx, H = s->b / my;
+    if (!(2.0 * reach < std::min(W, H))) return;
+    std::vector<int> reps;
+    std::vector<double> rc;
+    for (int k = 0; k < n; ++k) {
+        if (2.0 * s->radius[k] >= std::min(W, H)) return;
+        double rx = s->center[3 * k] - W * std::floor(s->center[3 * k] / W);
+        double ry = s->center[3 * k + 1] - H * std::floor(s->center[3 * k + 1] / H);
+        if (W - rx <= tol) rx -= W;
+        if (H - ry <= tol) ry -= H;
+        const double rz = s->center[3 * k + 2];
+        bool member = false;
+        for (size_t r = 0; r < reps.size() && !member; ++r)
+            member = same_kind(s, k, reps[r]) && std::fabs(rx - rc[3 * r]) <= tol &&
+                     std::fabs(ry - rc[3 * r + 1]) <= tol && pdist(rz - rc[3 * r + 2], s->c) <= tol;
+        if (member) continue;
+        reps.push_back(k);
+        rc.push_back(rx); rc.push_back(ry); rc.push_back(rz);
+    }
+    if ((long long)reps.size() * mx * my != n) return;
+    V.a = W; V.b = H;
+    V.mx = mx; V.my = my;
+    V.orig = reps;
+    V.center = rc;
+}
+
+static bool reach_hits_cell(const axx_substrate* s, const GeoView& V, int kk, const double sh[3], double reach) {
+    const int k = V.orig[kk];
+    const double lo[3] = {0, 0, 0}, hi[3] = {V.a, V.b, V.c};
     if (s->type[k] == AXX_PRIM_PLANE) {
         double n[3], pmin, pmax;
         unit_normal(s, k, n);
         double d = 0;
-        for (int a = 0; a < 3; ++a) d += n[a] * (s->center[3 * k + a] + sh[a]);
+        for (int a = 0; a < 3; ++a) d += n[a] * (V.center[3 * kk + a] + sh[a]);
         box_proj(n, lo, hi, pmin, pmax);
         return d >= pmin - reach && d <= pmax + reach;
     }
@@ -193,7 +293,7 @@ static bool reach_hits_cell(const axx_substrate* s, int k, const double sh[3], d
     const double R = s->radius[k] + reach;
     double d2 = 0;
     for (int a = 0; a < dims; ++a) {
-        const double c = s->center[3 * k + a] + sh[a];
+        const double c = V.center[3 * kk + a] + sh[a];
         double d = 0;
         if (c < lo[a]) d = lo[a] - c;
         else if (c > hi[a]) d = c - hi[a];
@@ -239,7 +339,7 @@ static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
 
 static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_t lds_budget,
-                   double bin_width_req, int step_mode, HostPrep& H) {
+                   double bin_width_req, int step_mode, bool allow_unit_cell, HostPrep& H) {
     const int n = s->n_prims;
     const double dt = q->dt;
     H.L.resize(n + 1);
@@ -277,22 +377,27 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
     // of it, so its reach is longer
     const double reach = 2.0 * step_reach_factor(step_mode) * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
                          (any_plane ? 4.0 * H.plane_eps : 0.0);
-    for (int k = 0; k < n; ++k) {
+    GeoView V;
+    unit_cell_view(s, allow_unit_cell, reach, V);
+    H.cell[0] = V.a; H.cell[1] = V.b; H.cell[2] = V.c;
+    H.units = V.mx * V.my;
+    for (int kk = 0; kk < (int)V.orig.size(); ++kk) {
+        const int k = V.orig[kk];
+        const double* ck = &V.center[3 * (size_t)kk];
         const int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
         const size_t first = H.geom.size();
         for (int kz = -zr; kz <= zr; ++kz)
             for (int ky = -1; ky <= 1; ++ky)
                 for (int kx = -1; kx <= 1; ++kx) {
-                    const double sh[3] = {kx * s->a, ky * s->b, kz * s->c};
-                    if (!reach_hits_cell(s, k, sh, reach)) continue;
+                    const double sh[3] = {kx * V.a, ky * V.b, kz * V.c};
+                    if (!reach_hits_cell(s, V, kk, sh, reach)) continue;
                     if (s->type[k] == AXX_PRIM_PLANE) {
                         // shifts along the plane give the same plane: one image
                         // per distinct float offset
                         double nn[3];
                         unit_normal(s, k, nn);
-                        const float dd = (float)(nn[0] * (s->center[3 * k] + sh[0]) +
-                                                 nn[1] * (s->center[3 * k + 1] + sh[1]) +
-                                                 nn[2] * (s->center[3 * k + 2] + sh[2]));
+                        const float dd = (float)(nn[0] * (ck[0] + sh[0]) + nn[1] * (ck[1] + sh[1]) +
+                                                 nn[2] * (ck[2] + sh[2]));
                         bool dup = false;
                         for (size_t q2 = first; q2 < H.geom.size(); ++q2) dup |= H.geom[q2].w == dd;
                         if (dup) continue;
@@ -302,9 +407,8 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
                         continue;
                     }
                     const float Rf = (float)s->radius[k];
-                    H.geom.push_back(make_float4((float)(s->center[3 * k] + sh[0]),
-                                                 (float)(s->center[3 * k + 1] + sh[1]),
-                                                 (float)(s->center[3 * k + 2] + sh[2]), Rf));
+                    H.geom.push_back(make_float4((float)(ck[0] + sh[0]), (float)(ck[1] + sh[1]),
+                                                 (float)(ck[2] + sh[2]), Rf));
                     H.meta.push_back((k + 1) | (s->type[k] == AXX_PRIM_SPHERE ? kSphereBit : 0));
                     H.orig.push_back(k);
                 }
@@ -321,7 +425,7 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
     // bin width: at least 4 step lengths; widen until the tables fit in LDS.
     // Each candidate carries a 4 x 4 mask of the sub-bins (xy) whose
     // rectangle its surface comes within reach of.
-    const double Ls[3] = {s->a, s->b, s->c};
+    const double Ls[3] = {V.a, V.b, V.c};
     double w = bin_width_req > 0 ? bin_width_req : std::max(4.0 * Lmax, 1e-9);
     for (int attempt = 0; attempt < 64; ++attempt) {
         int nb[3];
@@ -424,6 +528,29 @@ static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej, bool alt) {
     }
 }
 
+// Default threads per block: of 256, 512 and 1024 (within the kernel's
+// bound) the one that keeps the most waves resident per CU for this plan's
+// kernel, register use and LDS tables, the smaller on a tie (more, smaller
+// workgroups also spread small walker counts over every CU).  The 1000-
+// cylinder hexagonal workload in its unit cell: 256 (6 waves per SIMD),
+// 2.18e11 walker-steps/s against 1.97e11 at 1024 (4 waves).
+static int default_block(const axx_plan* p) {
+    const walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION,
+                                 p->run.step_mode != AXX_STEP_ISOTROPIC);
+    int best = 256, best_waves = -1;
+    for (int blk : {256, 512, 1024}) {
+        if (blk > walk_max_block(p->Pt)) continue;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, blk, p->lds_bytes) != hipSuccess)
+            continue;
+        if (per_cu * blk / 64 > best_waves) {
+            best = blk;
+            best_waves = per_cu * blk / 64;
+        }
+    }
+    return best;
+}
+
 static int axes_mask(const axx_acq* q) {
     int mask = 0;
     for (int j = 0; j < q->n_waveforms; ++j)
@@ -501,7 +628,12 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
         if (need >= s->a || need >= s->b || need >= s->c)
             return fail(ctx, AXX_ERR_GEOMETRY, "Gaussian steps need a cell wider than %.3g", need);
     }
-    rc = prepare(ctx, s, q, budget, tune ? tune->bin_width : 0.0, r->step_mode, H);
+    // the unit cell of a repeating substrate unless the tuning asks for the
+    // whole cell or the placement names one particular primitive
+    const bool allow_unit = !(tune && tune->whole_cell) &&
+                            (r->init_mode == AXX_INIT_UNIFORM ||
+                             (r->init_mode == AXX_INIT_COMPARTMENT && r->init_arg == 0));
+    rc = prepare(ctx, s, q, budget, tune ? tune->bin_width : 0.0, r->step_mode, allow_unit, H);
     if (rc) return rc;
     p->ctx = ctx;
     p->run = *r;
@@ -516,9 +648,12 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     p->ax = axes_mask(q);
     p->gen = H.gen;
     p->plane_eps = H.plane_eps;
-    p->cell[0] = (float)s->a; p->cell[1] = (float)s->b; p->cell[2] = (float)s->c;
+    p->cell[0] = (float)H.cell[0]; p->cell[1] = (float)H.cell[1]; p->cell[2] = (float)H.cell[2];
+    p->whole[0] = (float)s->a; p->whole[1] = (float)s->b;
+    p->units = H.units;
     {
-        const double hc = 0.5 * std::min(s->a, s->b), hs = 0.5 * std::min(std::min(s->a, s->b), s->c);
+        const double hc = 0.5 * std::min(H.cell[0], H.cell[1]);
+        const double hs = 0.5 * std::min(std::min(H.cell[0], H.cell[1]), H.cell[2]);
         p->H2c = (float)(hc * hc);
         p->H2s = (float)(hs * hs);
     }
@@ -582,10 +717,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if (tune && tune->block) {
         p->block = tune->block;
     } else {
-        // largest block that still gives every CU at least two workgroups
-        // (small walker counts would otherwise leave CUs idle)
-        const long long per = std::max<long long>(64, p->N / (2LL * std::max(ctx->n_cu, 1)));
-        p->block = (int)std::min<long long>(1024, per / 64 * 64);
+        p->block = default_block(p);
     }
     p->wpt = 1;
     HIP_TRY(ctx, hipDeviceSynchronize());
@@ -621,6 +753,7 @@ extern "C" int axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const ax
 WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     WalkArgs A{};
     A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
+    A.ca = p->whole[0]; A.cb = p->whole[1];
     A.H2c = p->H2c; A.H2s = p->H2s; A.plane_eps = p->plane_eps;
     A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
     A.ibz = p->ib[2];
@@ -881,6 +1014,8 @@ extern "C" int axx_plan_set_config(axx_plan* p, int block, int wpt) {
 }
 
 extern "C" int axx_plan_profile_count(const axx_plan* p) { return p ? p->P : AXX_ERR_INVALID; }
+
+extern "C" int axx_plan_unit_cells(const axx_plan* p) { return p ? p->units : AXX_ERR_INVALID; }
 
 // Unwrapped positions of the traced walkers into out[(j * n_steps + i) * 3].
 static __global__ void trace_gather_kernel(const uint32_t* __restrict__ state, long long N,

@@ -111,6 +111,14 @@ __device__ __forceinline__ float wrap1(float v, float a) {
     return v;
 }
 
+// A placement coordinate v of the substrate's cell taken into the unit cell
+// of width w of a repeating substrate: v - floor(v / w) w in one rounding,
+// then wrapped as a walk position (every operation correctly rounded).
+__device__ __forceinline__ float unit_wrap(float v, float w) {
+    const float k = floorf(v / w);
+    return wrap1(fmaf(-k, w, v), w);
+}
+
 // Crossing of the segment s + t*u, t in [0, ell], with a circle (cylinder,
 // xy only) or sphere surface centred at o-offsets (ox, oy, oz) = s - centre.
 // The walker's side comes from its tracked compartment; the start point is

@@ -30,21 +30,30 @@ __device__ __forceinline__ void init_lane(const WalkArgs& A, const float4* geom,
     bool done = false;
     float x = 0.0f, y = 0.0f, z = 0.0f;
     int comp = 0;
+    // placement in the substrate's cell (x0, y0, z0); the walk position is
+    // its image in the unit cell when the walk runs in one (ca != a, cb != b)
+    const bool unit = A.ca != A.a || A.cb != A.b;
+    float x0 = 0.0f, y0 = 0.0f, z0 = 0.0f;
     if (A.init_mode == 0) {
         u32x4 o = philox10(wid, 0u, kStreamInit, A.inst, A.k0, A.k1);
-        x = unif(o.x) * A.a;
-        y = unif(o.y) * A.b;
-        z = unif(o.z) * A.c;
+        x0 = unif(o.x) * A.ca;
+        y0 = unif(o.y) * A.cb;
+        z0 = unif(o.z) * A.c;
+        x = unit ? unit_wrap(x0, A.a) : x0;
+        y = unit ? unit_wrap(y0, A.b) : y0;
+        z = z0;
         comp = region_of(geom, meta, A.n_img, x, y, z);
         done = true;
     } else {
         for (uint32_t it = 0; it < (1u << 20) && !done; ++it) {
             u32x4 o = philox10(wid, it, kStreamInit, A.inst, A.k0, A.k1);
             if (A.init_mode == 1) {
-                float px = unif(o.x) * A.a, py = unif(o.y) * A.b, pz = unif(o.z) * A.c;
+                float qx = unif(o.x) * A.ca, qy = unif(o.y) * A.cb, pz = unif(o.z) * A.c;
+                float px = unit ? unit_wrap(qx, A.a) : qx, py = unit ? unit_wrap(qy, A.b) : qy;
                 int cm = region_of(geom, meta, A.n_img, px, py, pz);
                 if (cm != A.init_arg) continue;
                 x = px; y = py; z = pz; comp = cm;
+                x0 = qx; y0 = qy; z0 = pz;
                 done = true;
             } else {
                 float4 gk = geom[A.init_img];
@@ -56,6 +65,7 @@ __device__ __forceinline__ void init_lane(const WalkArgs& A, const float4* geom,
                 if (!inside_at(gk, sk, px, py, pz)) continue;
                 px = wrap1(px, A.a); py = wrap1(py, A.b); pz = wrap1(pz, A.c);
                 x = px; y = py; z = pz;
+                x0 = px; y0 = py; z0 = pz;
                 comp = region_of(geom, meta, A.n_img, px, py, pz);
                 done = true;
             }
@@ -68,9 +78,9 @@ __device__ __forceinline__ void init_lane(const WalkArgs& A, const float4* geom,
     fs[(size_t)F_DX * N + w] = 0.0f;
     fs[(size_t)F_DY * N + w] = 0.0f;
     fs[(size_t)F_DZ * N + w] = 0.0f;
-    fs[(size_t)F_X0 * N + w] = x;
-    fs[(size_t)F_Y0 * N + w] = y;
-    fs[(size_t)F_Z0 * N + w] = z;
+    fs[(size_t)F_X0 * N + w] = x0;
+    fs[(size_t)F_Y0 * N + w] = y0;
+    fs[(size_t)F_Z0 * N + w] = z0;
     fs[(size_t)F_E * N + w] = 0.0f;
     fs[(size_t)F_LF * N + w] = 0.0f;
     is[(size_t)F_COMP * N + w] = comp;

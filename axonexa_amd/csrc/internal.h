@@ -68,7 +68,9 @@ struct axx_plan {
     double* d_out = nullptr;
     unsigned long long* d_cnt = nullptr;
     int n_chunks = 0, chunk = 0;
-    float cell[3] = {0, 0, 0};
+    float cell[3] = {0, 0, 0};      // cell the positions wrap in (unit cell of a repeating substrate)
+    float whole[2] = {0, 0};        // the substrate's cell along x and y (placement)
+    int units = 1;                  // unit cells in the substrate's cell
     float H2c = 0, H2s = 0, plane_eps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

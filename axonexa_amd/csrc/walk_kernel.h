@@ -22,7 +22,9 @@
 namespace axx {
 
 struct WalkArgs {
-    float a, b, c;
+    float a, b, c;                          // cell the positions wrap in
+    float ca, cb;                           // the substrate's cell along x, y (placement); != a, b
+                                            // when the walk runs in the unit cell of a repeating substrate
     float H2c, H2s;                         // squared half of the smallest cell side (xy; xyz)
     float plane_eps;                        // planes count as hit within this distance
     int nbx, nby, nbz;

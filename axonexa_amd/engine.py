@@ -68,6 +68,7 @@ def lib():
                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.axx_plan_set_config.argtypes = [vp, C.c_int, C.c_int]
     L.axx_plan_profile_count.argtypes = [vp]
+    L.axx_plan_unit_cells.argtypes = [vp]
     L.axx_plan_destroy.argtypes = [vp]
     IP = C.POINTER(abi.Instance)
     L.axx_batch_create.argtypes = [vp, C.c_int, IP, C.POINTER(abi.Tuning), C.POINTER(vp)]
@@ -166,11 +167,11 @@ class Plan:
 
     def __init__(self, sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
                  ctx: Optional[Context] = None, block: int = 0, bin_width: float = 0.0,
-                 lds_budget: int = 0):
+                 lds_budget: int = 0, whole_cell: bool = False):
         self.ctx = ctx or default_context()
         self.sub, self.acq, self.run = sub, acq, run
         self._s, self._q, self._r = sub.struct(), acq.struct(), run.struct()
-        self._t = abi.Tuning(int(block), float(bin_width), int(lds_budget))
+        self._t = abi.Tuning(int(block), float(bin_width), int(lds_budget), int(bool(whole_cell)))
         self._h = C.c_void_p()
         rc = lib().axx_plan_create_ex(self.ctx.handle, C.byref(self._s), C.byref(self._q),
                                       C.byref(self._r), C.byref(self._t), C.byref(self._h))
@@ -216,7 +217,8 @@ class Plan:
         bw = C.c_double()
         self.ctx.check(lib().axx_plan_get_config(self._h, C.byref(b), C.byref(w), C.byref(bw),
                                                  C.byref(l)), "axx_plan_get_config")
-        return {"block": b.value, "wpt": w.value, "bin_width": bw.value, "lds_bytes": l.value}
+        return {"block": b.value, "wpt": w.value, "bin_width": bw.value, "lds_bytes": l.value,
+                "unit_cells": int(lib().axx_plan_unit_cells(self._h))}
 
     def set_config(self, block: int, wpt: int = 1):
         self.ctx.check(lib().axx_plan_set_config(self._h, int(block), int(wpt)), "axx_plan_set_config")
@@ -426,7 +428,7 @@ class Batch:
             arr[k].acq = C.pointer(q)
             arr[k].run = C.pointer(r)
         self._arr = arr
-        self._t = abi.Tuning(int(block), 0.0, int(lds_budget))
+        self._t = abi.Tuning(int(block), 0.0, int(lds_budget), 0)
         self._h = C.c_void_p()
         rc = lib().axx_batch_create(self.ctx.handle, len(self._structs), arr, C.byref(self._t),
                                     C.byref(self._h))

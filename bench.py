@@ -20,6 +20,7 @@ import dataclasses
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -30,9 +31,21 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (spec sheet)
 PEAK_HBM_GBS = 8000.0
 
 
+def _cpu_counts():
+    """(threads used, nproc, hardware_concurrency): the CPU baseline runs one
+    thread per processing unit `nproc` grants this process (it honours the
+    affinity mask and OMP_NUM_THREADS, the box's CPU share);
+    hardware_concurrency = the online CPUs (what std::thread reports)."""
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.split()[0])
+    except (OSError, ValueError, IndexError, subprocess.CalledProcessError):
+        nproc = len(os.sched_getaffinity(0))
+    hw = os.cpu_count() or 1
+    return max(1, nproc), nproc, hw
+
+
 def _cpu_threads() -> int:
-    n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return _cpu_counts()[0]
 
 
 def _cpu_model() -> str:
@@ -238,10 +251,13 @@ def main(argv=None) -> int:
             gst = sp.get_state()
         bit = np.all(gst.reshape(-1, nw) == ost.reshape(-1, nw), axis=0)
         dE = np.abs(gsig.E_re - osig.E_re) / np.maximum(osig.sigma_re, 1e-300)
+        _, nproc, hw = _cpu_counts()
         out["cpu_baseline"] = {
             "value": cpu_rate,
             "unit": "walker-steps/s",
             "cores": threads,
+            "nproc": nproc,
+            "hardware_concurrency": hw,
             "kind": "port",
             "sample": f"{nw} walkers x {n} steps of the same workload (walker ids 0..{nw - 1}), "
                       f"oracle with the GPU's cell bins, {threads} threads on {_cpu_model()}",
@@ -249,12 +265,13 @@ def main(argv=None) -> int:
         }
         out["vs_cpu"] = value / cpu_rate
         # the reference-faithful variant: every step scans every primitive
-        # image (lattice.cuh:184-199), on a shorter sample
+        # image of the whole cell (lattice.cuh:184-199), on a shorter sample
         lw = min(256, wl.run.n_walkers)
         ln = min(n, 20000)
         lin = make(**dict(kw, n_walkers=lw, n_steps=ln))
         tl = time.perf_counter()
-        po.walk(lin.sub, lin.acq, lin.run, threads=threads, bin_width=0.0)
+        with po.whole_cell():
+            po.walk(lin.sub, lin.acq, lin.run, threads=threads, bin_width=0.0)
         tlin = time.perf_counter() - tl
         out["cpu_baseline_linear_scan"] = {
             "value": lin.walker_steps / tlin,
@@ -262,7 +279,7 @@ def main(argv=None) -> int:
             "cores": threads,
             "kind": "port",
             "sample": f"{lw} walkers x {ln} steps of the same workload, oracle scanning all primitive images "
-                      f"every step, {threads} threads",
+                      f"of the whole cell every step, {threads} threads",
             "seconds": tlin,
         }
         out["signal_error_vs_oracle"] = {

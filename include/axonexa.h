@@ -235,6 +235,12 @@ typedef struct axx_tuning {
     int    block;       /* threads per block: multiple of 64 in [64, 1024]; 0 = 1024 */
     double bin_width;   /* cell-bin edge length; 0 = automatic               */
     int    lds_budget;  /* LDS bytes the substrate tables may use; 0 = 150 KiB */
+    int    whole_cell;  /* 1: walk in the substrate's cell as given; 0: a substrate
+                         * whose primitives repeat along x and y is walked in its
+                         * unit cell (placement AXX_INIT_UNIFORM or compartment 0)
+                         * -- the same physics; positions wrap in the unit cell,
+                         * a walker inside a primitive carries the compartment of
+                         * the lowest-numbered primitive of its translation class */
 } axx_tuning;
 int  axx_plan_create_ex(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
                         const axx_run* run, const axx_tuning* tune, axx_plan** out);
@@ -254,6 +260,10 @@ int  axx_plan_get_config(const axx_plan* plan, int* block, int* wpt,
                          double* bin_width, int* lds_bytes);
 int  axx_plan_set_config(axx_plan* plan, int block, int wpt);
 int  axx_plan_profile_count(const axx_plan* plan);
+/* Unit cells of the substrate's cell the plan walks in one of (1: the whole
+ * cell; see axx_tuning.whole_cell).  No reference counterpart: the reference
+ * scans the whole basis (lattice.cuh:184-247). */
+int  axx_plan_unit_cells(const axx_plan* plan);
 void axx_plan_destroy(axx_plan* plan);
 
 /*

@@ -369,7 +369,9 @@ typedef struct {
 } oprim;
 
 typedef struct {
-    float a, b, c;
+    float a, b, c;   /* cell the positions wrap in (unit cell of a repeating substrate) */
+    float ca, cb;    /* the substrate's cell along x and y (placement) */
+    int units;       /* unit cells in the substrate's cell */
     int n_img;
     oprim* img;
     int n_comp;
@@ -404,10 +406,11 @@ static void unit_normal(const axx_substrate* s, int k, double n[3]) {
     for (int a = 0; a < 3; ++a) n[a] = s->normal[3 * k + a] / l;
 }
 
-static int prim_reach_hits_box(const axx_substrate* s, int k, double sx, double sy, double sz,
-                               double lo[3], double hi[3], double reach) {
-    /* Does primitive k shifted by (sx,sy,sz), grown by `reach`, touch the box? */
-    double c[3] = {s->center[3 * k] + sx, s->center[3 * k + 1] + sy, s->center[3 * k + 2] + sz};
+static int prim_reach_hits_box(const axx_substrate* s, int k, const double ck[3], double sx, double sy,
+                               double sz, double lo[3], double hi[3], double reach) {
+    /* Does primitive k at centre ck shifted by (sx,sy,sz), grown by `reach`,
+     * touch the box? */
+    double c[3] = {ck[0] + sx, ck[1] + sy, ck[2] + sz};
     if (s->type[k] == AXX_PRIM_PLANE) {
         double n[3], pmin, pmax;
         unit_normal(s, k, n);
@@ -507,18 +510,120 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
  * 4.71 such units. */
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
 
+/* Walk a repeating substrate in its unit cell unless this is cleared (tests
+ * and the reference-faithful CPU timing walk the whole cell). */
+static int g_unit_cells = 1;
+void orc_set_unit_cells(int on) { g_unit_cells = on; }
+
+/* The primitives the walk sees and the cell they repeat in (restates the
+ * product's host preparation): a substrate whose primitives map onto
+ * themselves under a shift of a/mx along x and b/my along y is walked in its
+ * unit cell (a/mx) x (b/my) x c; each translation class is represented by its
+ * lowest-numbered member (compartment id = its index + 1), at its centre
+ * reduced into the unit cell.  Otherwise the whole cell. */
+typedef struct {
+    double a, b, c;
+    int mx, my, n;
+    int* orig;        /* [n] substrate index */
+    double* center;   /* [3 n] */
+} oview;
+
+static double pdist(double d, double L) { return fabs(d - L * nearbyint(d / L)); }
+
+static int same_kind(const axx_substrate* s, int i, int j) {
+    double ki = s->kappa ? s->kappa[i] : 0.0, kj = s->kappa ? s->kappa[j] : 0.0;
+    return s->type[i] == s->type[j] && s->radius[i] == s->radius[j] && s->D[i] == s->D[j] &&
+           (s->T2[i] == s->T2[j] || (isinf(s->T2[i]) && isinf(s->T2[j]))) && ki == kj;
+}
+
+static int shift_is_symmetry(const axx_substrate* s, int ax, double sh, double tol) {
+    double L[3] = {s->a, s->b, s->c};
+    for (int k = 0; k < s->n_prims; ++k) {
+        int found = 0;
+        for (int j = 0; j < s->n_prims && !found; ++j) {
+            if (!same_kind(s, k, j)) continue;
+            int ok = 1;
+            for (int a = 0; a < 3 && ok; ++a)
+                ok = pdist(s->center[3 * k + a] + (a == ax ? sh : 0.0) - s->center[3 * j + a], L[a]) <= tol;
+            found = ok;
+        }
+        if (!found) return 0;
+    }
+    return 1;
+}
+
+static int periods(const axx_substrate* s, int ax, double tol) {
+    double L = ax == 0 ? s->a : s->b;
+    for (int m = s->n_prims; m >= 2; --m)
+        if (s->n_prims % m == 0 && shift_is_symmetry(s, ax, L / m, tol)) return m;
+    return 1;
+}
+
+static void oview_whole(const axx_substrate* s, oview* V) {
+    V->a = s->a; V->b = s->b; V->c = s->c;
+    V->mx = V->my = 1;
+    V->n = s->n_prims;
+    V->orig = malloc(sizeof(int) * (size_t)(s->n_prims + 1));
+    V->center = malloc(sizeof(double) * 3 * (size_t)(s->n_prims + 1));
+    for (int k = 0; k < s->n_prims; ++k) {
+        V->orig[k] = k;
+        for (int a = 0; a < 3; ++a) V->center[3 * k + a] = s->center[3 * k + a];
+    }
+}
+
+static void oview_build(const axx_substrate* s, int allow, double reach, oview* V) {
+    oview_whole(s, V);
+    int n = s->n_prims;
+    if (!allow || n < 2) return;
+    for (int k = 0; k < n; ++k)
+        if (s->type[k] == AXX_PRIM_PLANE) return;
+    double tol = 1e-9 * fmax(fmax(s->a, s->b), s->c);
+    int mx = periods(s, 0, tol), my = periods(s, 1, tol);
+    if (mx * my == 1) return;
+    double W = s->a / mx, H = s->b / my;
+    if (!(2.0 * reach < fmin(W, H))) return;
+    int* reps = malloc(sizeof(int) * (size_t)n);
+    double* rc = malloc(sizeof(double) * 3 * (size_t)n);
+    int nr = 0;
+    for (int k = 0; k < n; ++k) {
+        if (2.0 * s->radius[k] >= fmin(W, H)) { free(reps); free(rc); return; }
+        double rx = s->center[3 * k] - W * floor(s->center[3 * k] / W);
+        double ry = s->center[3 * k + 1] - H * floor(s->center[3 * k + 1] / H);
+        if (W - rx <= tol) rx -= W;
+        if (H - ry <= tol) ry -= H;
+        double rz = s->center[3 * k + 2];
+        int member = 0;
+        for (int r = 0; r < nr && !member; ++r)
+            member = same_kind(s, k, reps[r]) && fabs(rx - rc[3 * r]) <= tol && fabs(ry - rc[3 * r + 1]) <= tol &&
+                     pdist(rz - rc[3 * r + 2], s->c) <= tol;
+        if (member) continue;
+        reps[nr] = k;
+        rc[3 * nr] = rx; rc[3 * nr + 1] = ry; rc[3 * nr + 2] = rz;
+        ++nr;
+    }
+    if ((long long)nr * mx * my != n) { free(reps); free(rc); return; }
+    free(V->orig); free(V->center);
+    V->a = W; V->b = H;
+    V->mx = mx; V->my = my;
+    V->n = nr;
+    V->orig = reps;
+    V->center = rc;
+}
+
+static int oprep_build_view(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width,
+                            double reach_factor, int allow_unit_cell);
+
 static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width,
                        double reach_factor) {
+    return oprep_build_view(P, s, q, bin_width, reach_factor, 0);
+}
+
+static int oprep_build_view(oprep* P, const axx_substrate* s, const axx_acq* q, double bin_width,
+                            double reach_factor, int allow_unit_cell) {
     memset(P, 0, sizeof(*P));
     int rc = orc_validate(s, q);
     if (rc) return rc;
-    P->a = (float)s->a; P->b = (float)s->b; P->c = (float)s->c;
-    {
-        double hc = 0.5 * fmin(s->a, s->b), hs = 0.5 * fmin(fmin(s->a, s->b), s->c);
-        P->H2c = (float)(hc * hc);
-        P->H2s = (float)(hs * hs);
-        P->plane_eps = (float)(64.0 * ldexp(1.0, -23) * fmax(fmax(s->a, s->b), s->c));
-    }
+    P->plane_eps = (float)(64.0 * ldexp(1.0, -23) * fmax(fmax(s->a, s->b), s->c));
     int n = s->n_prims;
     P->n_comp = n + 1;
     P->L = malloc(sizeof(float) * (n + 1));
@@ -558,24 +663,35 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
      * counts as hit within plane_eps of it, so its reach is longer */
     double reach = 2.0 * reach_factor * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) +
                    (any_plane ? 4.0 * P->plane_eps : 0.0);
-    double lo[3] = {0, 0, 0}, hi[3] = {s->a, s->b, s->c};
-    int cap = n * 27 + 1;
+    oview V;
+    oview_build(s, allow_unit_cell, reach, &V);
+    P->a = (float)V.a; P->b = (float)V.b; P->c = (float)V.c;
+    P->ca = (float)s->a; P->cb = (float)s->b;
+    P->units = V.mx * V.my;
+    {
+        double hc = 0.5 * fmin(V.a, V.b), hs = 0.5 * fmin(fmin(V.a, V.b), V.c);
+        P->H2c = (float)(hc * hc);
+        P->H2s = (float)(hs * hs);
+    }
+    double lo[3] = {0, 0, 0}, hi[3] = {V.a, V.b, V.c};
+    int cap = V.n * 27 + 1;
     P->img = malloc(sizeof(oprim) * cap);
     int m = 0;
-    for (int k = 0; k < n; ++k) {
+    for (int kk = 0; kk < V.n; ++kk) {
+        int k = V.orig[kk];
+        const double* ck = V.center + 3 * kk;
         int zr = s->type[k] != AXX_PRIM_CYLINDER_XY ? 1 : 0;
         int first = m;
         for (int kz = -zr; kz <= zr; ++kz)
             for (int ky = -1; ky <= 1; ++ky)
                 for (int kx = -1; kx <= 1; ++kx) {
-                    double sx = kx * s->a, sy = ky * s->b, sz = kz * s->c;
-                    if (!prim_reach_hits_box(s, k, sx, sy, sz, lo, hi, reach)) continue;
+                    double sx = kx * V.a, sy = ky * V.b, sz = kz * V.c;
+                    if (!prim_reach_hits_box(s, k, ck, sx, sy, sz, lo, hi, reach)) continue;
                     if (s->type[k] == AXX_PRIM_PLANE) {
                         /* shifts along the plane give the same plane */
                         double nn[3];
                         unit_normal(s, k, nn);
-                        float dd = (float)(nn[0] * (s->center[3 * k] + sx) + nn[1] * (s->center[3 * k + 1] + sy) +
-                                           nn[2] * (s->center[3 * k + 2] + sz));
+                        float dd = (float)(nn[0] * (ck[0] + sx) + nn[1] * (ck[1] + sy) + nn[2] * (ck[2] + sz));
                         int dup = 0;
                         for (int q = first; q < m; ++q) dup |= P->img[q].d == dd;
                         if (dup) continue;
@@ -589,9 +705,9 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                         continue;
                     }
                     oprim* p = &P->img[m++];
-                    p->cx = (float)(s->center[3 * k] + sx);
-                    p->cy = (float)(s->center[3 * k + 1] + sy);
-                    p->cz = (float)(s->center[3 * k + 2] + sz);
+                    p->cx = (float)(ck[0] + sx);
+                    p->cy = (float)(ck[1] + sy);
+                    p->cz = (float)(ck[2] + sz);
                     float Rf = (float)s->radius[k];
                     p->R2 = Rf * Rf;
                     p->R = Rf;
@@ -602,11 +718,12 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                 }
     }
     P->n_img = m;
+    free(V.orig); free(V.center);
     /* cell bins (binned variant): surface within reach of the bin */
     if (bin_width > 0) {
         int any_sphere = 0;
         for (int k = 0; k < n; ++k) any_sphere |= (s->type[k] != AXX_PRIM_CYLINDER_XY);
-        double Ls[3] = {s->a, s->b, s->c};
+        double Ls[3] = {V.a, V.b, V.c};
         for (int a = 0; a < 3; ++a) {
             int nb = (int)floor(Ls[a] / bin_width);
             if (nb < 1) nb = 1;
@@ -640,9 +757,9 @@ static int oprep_build(oprep* P, const axx_substrate* s, const axx_acq* q, doubl
                 for (int bz = lo_b[2]; bz <= hi_b[2]; ++bz)
                     for (int by = lo_b[1]; by <= hi_b[1]; ++by)
                         for (int bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
-                            double blo[3] = {bx * s->a / P->nb[0], by * s->b / P->nb[1], bz * s->c / P->nb[2]};
-                            double bhi[3] = {(bx + 1) * s->a / P->nb[0], (by + 1) * s->b / P->nb[1],
-                                             (bz + 1) * s->c / P->nb[2]};
+                            double blo[3] = {bx * Ls[0] / P->nb[0], by * Ls[1] / P->nb[1], bz * Ls[2] / P->nb[2]};
+                            double bhi[3] = {(bx + 1) * Ls[0] / P->nb[0], (by + 1) * Ls[1] / P->nb[1],
+                                             (bz + 1) * Ls[2] / P->nb[2]};
                             if (!prim_surface_near_box(pi, blo, bhi, reach)) continue;
                             int b = (bz * P->nb[1] + by) * P->nb[0] + bx;
                             if (pass == 0) P->bin_off[b + 1]++;
@@ -771,6 +888,13 @@ static inline float wrap1(float v, float a) {
     return v;
 }
 
+/* A placement coordinate of the substrate's cell taken into the unit cell of
+ * width w: v - floor(v / w) w in one rounding, then wrapped as a position. */
+static inline float unit_wrap(float v, float w) {
+    float k = floorf(v / w);
+    return wrap1(fmaf(-k, w, v), w);
+}
+
 static inline void cand_range(const oprep* P, int use_bins, float x, float y, float z,
                               const int** list, int* cnt, int* dense) {
     if (!use_bins) {
@@ -818,16 +942,22 @@ typedef struct {
 #define IS(f) (((int32_t*)J->st) + (size_t)(f) * J->N)
 
 static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t key[2],
-                       float* x, float* y, float* z, int* comp) {
+                       float* x, float* y, float* z, float* x0, float* y0, float* z0, int* comp) {
     const oprep* P = J->P;
     const axx_run* R = J->run;
     uint32_t inst = R->instance;
     uint32_t o[4];
+    /* placement in the substrate's cell (x0, y0, z0); the walk position is
+     * its image in the unit cell when the walk runs in one */
+    const int unit = P->ca != P->a || P->cb != P->b;
     if (R->init_mode == AXX_INIT_UNIFORM) {
         draw(wid, 0, STREAM_INIT << 24, inst, key, o);
-        *x = unif(o[0]) * P->a;
-        *y = unif(o[1]) * P->b;
-        *z = unif(o[2]) * P->c;
+        *x0 = unif(o[0]) * P->ca;
+        *y0 = unif(o[1]) * P->cb;
+        *z0 = unif(o[2]) * P->c;
+        *x = unit ? unit_wrap(*x0, P->a) : *x0;
+        *y = unit ? unit_wrap(*y0, P->b) : *y0;
+        *z = *z0;
         *comp = region_of(P, *x, *y, *z);
         return 0;
     }
@@ -838,12 +968,14 @@ static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t k
         draw(wid, it, STREAM_INIT << 24, inst, key, o);
         float px, py, pz;
         if (R->init_mode == AXX_INIT_COMPARTMENT) {
-            px = unif(o[0]) * P->a;
-            py = unif(o[1]) * P->b;
+            float qx = unif(o[0]) * P->ca, qy = unif(o[1]) * P->cb;
             pz = unif(o[2]) * P->c;
+            px = unit ? unit_wrap(qx, P->a) : qx;
+            py = unit ? unit_wrap(qy, P->b) : qy;
             int cm = region_of(P, px, py, pz);
             if (cm != R->init_arg) continue;
             *x = px; *y = py; *z = pz; *comp = cm;
+            *x0 = qx; *y0 = qy; *z0 = pz;
             return 0;
         } else { /* inside primitive k: its bounding cube (cylinderXY.cuh:176-183) */
             int k = R->init_arg;
@@ -858,6 +990,7 @@ static int init_walker(const job_t* J, int64_t w, uint32_t wid, const uint32_t k
             if (!inside_at(pk, px, py, pz)) continue;
             px = wrap1(px, P->a); py = wrap1(py, P->b); pz = wrap1(pz, P->c);
             *x = px; *y = py; *z = pz;
+            *x0 = px; *y0 = py; *z0 = pz;
             *comp = region_of(P, px, py, pz);
             return 0;
         }
@@ -886,20 +1019,21 @@ static void* job_run(void* arg) {
         int comp, li;
         uint32_t ncoll, ntran, ncap;
         if (J->i0 == 0) {
-            int rc = init_walker(J, w, wid, key, &x, &y, &z, &comp);
+            float px0, py0, pz0;
+            int rc = init_walker(J, w, wid, key, &x, &y, &z, &px0, &py0, &pz0, &comp);
             if (rc) { J->err = rc; return NULL; }
             dx = dy = dz = 0.0f;
             E = 0.0f; lf = 0.0f; li = 1;
             ncoll = ntran = ncap = 0;
             memset(m, 0, sizeof(m));
-            FS(AXX_F_X0)[w] = x; FS(AXX_F_Y0)[w] = y; FS(AXX_F_Z0)[w] = z;
+            FS(AXX_F_X0)[w] = px0; FS(AXX_F_Y0)[w] = py0; FS(AXX_F_Z0)[w] = pz0;
             IS(AXX_F_COMP0)[w] = comp;
             if (G) {
                 /* M = m0, then the update at t = 0 at the start point */
                 for (int j = 0; j < G->F; ++j) {
                     Mg[j].x = G->f->m0[0]; Mg[j].y = G->f->m0[1]; Mg[j].z = G->f->m0[2];
                 }
-                if (J->i1 > 0) mag_update(G, Mg, 0, x, y, z, 0.0f, 0.0f, 0.0f, comp);
+                if (J->i1 > 0) mag_update(G, Mg, 0, px0, py0, pz0, 0.0f, 0.0f, 0.0f, comp);
             }
         } else {
             x = FS(AXX_F_X)[w]; y = FS(AXX_F_Y)[w]; z = FS(AXX_F_Z)[w];
@@ -1173,7 +1307,9 @@ static int walk_common(const axx_substrate* s, const axx_acq* q, const axx_run* 
     if (run->boundary_mode < 0 || run->boundary_mode > 2) return AXX_ERR_INVALID;
     if (run->step_mode < 0 || run->step_mode > 2) return AXX_ERR_INVALID;
     oprep P;
-    int rc = oprep_build(&P, s, q, bin_width, step_reach_factor(run->step_mode));
+    const int allow_unit = g_unit_cells && (run->init_mode == AXX_INIT_UNIFORM ||
+                                            (run->init_mode == AXX_INIT_COMPARTMENT && run->init_arg == 0));
+    int rc = oprep_build_view(&P, s, q, bin_width, step_reach_factor(run->step_mode), allow_unit);
     if (!rc && run->step_mode == AXX_STEP_GAUSSIAN) {
         double need = 2.0 * step_reach_factor(AXX_STEP_GAUSSIAN) * (double)P.Lmax;
         if (need >= s->a || need >= s->b || need >= s->c) rc = AXX_ERR_GEOMETRY;
@@ -1365,6 +1501,18 @@ int orc_reduce(const axx_substrate* s, const axx_acq* q, const axx_run* run,
     free(S); free(acc);
     oprep_free(&P);
     return AXX_OK;
+}
+
+/* Unit cells of the substrate's cell that a walk of `run` runs in one of. */
+int orc_unit_cells(const axx_substrate* s, const axx_acq* q, const axx_run* run) {
+    if (!s || !q || !run) return AXX_ERR_INVALID;
+    oprep P;
+    const int allow_unit = g_unit_cells && (run->init_mode == AXX_INIT_UNIFORM ||
+                                            (run->init_mode == AXX_INIT_COMPARTMENT && run->init_arg == 0));
+    int rc = oprep_build_view(&P, s, q, 0.0, step_reach_factor(run->step_mode), allow_unit);
+    int u = rc ? rc : P.units;
+    oprep_free(&P);
+    return u;
 }
 
 /* Expanded primitive table (for tests of the image construction). */

@@ -40,6 +40,10 @@ void  orc_reduce_mag(const uint32_t* state, int64_t n_walkers, int F, double* ou
 void  orc_sincos64(const double* x, double* s, double* c, int64_t n);
 void  orc_sqrt_cr(const double* x, double* y, int64_t n);
 int   orc_image_count(const axx_substrate* s, const axx_acq* q);
+/* 1 (default): a substrate whose primitives repeat along x and y is walked in
+ * its unit cell, as the product does; 0: always in the whole cell. */
+void  orc_set_unit_cells(int on);
+int   orc_unit_cells(const axx_substrate* s, const axx_acq* q, const axx_run* run);
 int   orc_hit_test(int type, float cx, float cy, float cz, float R, int inside, const float s3[3],
                    const float u3[3], float ell, float* t);
 #ifdef __cplusplus

@@ -77,6 +77,8 @@ def _configure(L):
                                   C.c_int, C.c_int, u32p, C.c_int, C.c_double, C.POINTER(C.c_int64)]
     L.orc_exponents.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                                 u32p, C.c_void_p]
+    L.orc_set_unit_cells.argtypes = [C.c_int]
+    L.orc_unit_cells.argtypes = [C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run)]
     return L
 
 
@@ -169,6 +171,23 @@ def run_mag(sub, field, run, threads=None) -> abi.MagSignals:
     s = np.concatenate(sums)
     return abi.MagSignals(mx=s[:, 0].copy(), my=s[:, 1].copy(), mz=s[:, 2].copy(), n_walkers=run.n_walkers,
                           collisions=coll[0], transmissions=coll[1], cap_hits=coll[2])
+
+
+@contextlib.contextmanager
+def whole_cell():
+    """Inside the block the oracle walks every substrate in its whole cell
+    (no unit-cell reduction of repeating substrates)."""
+    lib().orc_set_unit_cells(0)
+    try:
+        yield
+    finally:
+        lib().orc_set_unit_cells(1)
+
+
+def unit_cells(sub, acq, run) -> int:
+    """Unit cells of the substrate's cell the oracle walks in one of."""
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    return lib().orc_unit_cells(C.byref(s), C.byref(q), C.byref(r))
 
 
 def validate(sub: abi.SubstrateSpec, acq: abi.AcqSpec) -> int:
