@@ -22,6 +22,14 @@
 #include "internal.h"
 #include "init_kernel.h"
 
+// Scratch of the allocation-free entry point axx_run_phase_noalloc: every
+// device buffer of such a run (substrate tables, profile table, walker state,
+// reduction partials) is a piece of this array.  Each GPU the library runs on
+// holds its own copy until the process ends; host code takes its address on
+// the current device from hipGetSymbolAddress (axx_create), never from the
+// name.  Runs that use it on one GPU at the same time race.
+static __device__ __align__(256) unsigned char scratch[AXX_NOALLOC_SCRATCH_BYTES];
+
 extern "C" const char* axx_version(void) { return "axonexa_amd 0.1.0 (gfx950)"; }
 
 extern "C" const char* axx_strerror(int code) {
@@ -60,6 +68,10 @@ extern "C" int axx_create(axx_ctx** out, int device) {
             c->n_cu = prop.multiProcessorCount;
             c->lds_max = prop.sharedMemPerBlock;
         }
+        // this device's copy of the scratch array (allocation-free runs)
+        void* sp = nullptr;
+        c->scratch = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess
+                         ? static_cast<unsigned char*>(sp) : nullptr;
         *out = c;
         return AXX_OK;
     } catch (...) {
@@ -560,16 +572,55 @@ static int axes_mask(const axx_acq* q) {
     return widen[mask];
 }
 
+// Device memory of a plan: hipMalloc, or (allocation-free runs) the next
+// 256-byte aligned piece of the scratch array.
+static int dev_alloc(axx_ctx* ctx, DevArena* ar, void** out, size_t bytes) {
+    if (!ar) {
+        HIP_TRY(ctx, hipMalloc(out, bytes));
+        return AXX_OK;
+    }
+    const size_t off = (ar->used + 255) & ~(size_t)255;
+    if (off > ar->size || bytes > ar->size - off)
+        return fail(ctx, AXX_ERR_LIMIT, "run needs more than the %zu bytes of scratch of an allocation-free run",
+                    ar->size);
+    *out = ar->base + off;
+    ar->used = off + bytes;
+    return AXX_OK;
+}
+
 template <typename T>
-static int dupload(axx_ctx* ctx, T** dst, const T* src, size_t n) {
+static int dupload(axx_ctx* ctx, DevArena* ar, T** dst, const T* src, size_t n) {
     if (n == 0) n = 1;
-    HIP_TRY(ctx, hipMalloc((void**)dst, n * sizeof(T)));
+    int rc = dev_alloc(ctx, ar, (void**)dst, n * sizeof(T));
+    if (rc) return rc;
     if (src) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return AXX_OK;
+}
+
+static __global__ void zero_words(uint32_t* p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0u;
+}
+
+// Zero n words: hipMemset for a plan's own buffers, a fill kernel on the
+// context stream for scratch (allocation-free runs).
+static int zero_dev(axx_ctx* ctx, DevArena* ar, void* d, size_t n_words) {
+    if (!ar) {
+        HIP_TRY(ctx, hipMemset(d, 0, n_words * sizeof(uint32_t)));
+        return AXX_OK;
+    }
+    const unsigned blocks = (unsigned)std::min<size_t>((n_words + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_words, dim3(std::max(blocks, 1u)), dim3(256), 0, ctx->stream, (uint32_t*)d, n_words);
+    HIP_TRY(ctx, hipGetLastError());
     return AXX_OK;
 }
 
 void plan_free(axx_plan* p) {
     if (!p) return;
+    if (p->arena) {   // buffers are scratch, events were never made
+        delete p;
+        return;
+    }
     void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_bins, p->d_cand, p->d_L, p->d_rate,
                     p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
                     p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
@@ -597,7 +648,7 @@ extern "C" void axx_plan_destroy(axx_plan* plan) {
 }
 
 int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
-                     const axx_tuning* tune, axx_plan* p, int min_pt) {
+                     const axx_tuning* tune, axx_plan* p, int min_pt, DevArena* ar) {
     int rc = validate(ctx, s, q);
     if (rc) return rc;
     if (r->n_walkers < 1) return fail(ctx, AXX_ERR_INVALID, "n_walkers must be >= 1");
@@ -686,34 +737,40 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     p->wf_scale.resize(p->M);
     for (int j = 0; j < p->M; ++j) p->wf_scale[j] = q->gamma * q->dt * q->wf_amplitude[j];
 
-    if ((rc = dupload(ctx, &p->d_geom, H.geom.data(), H.geom.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_cyl, H.cyl.data(), H.cyl.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_inv, H.inv.data(), H.inv.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_L, H.L.data(), H.L.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_rate, H.rate.data(), H.rate.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_thr_out, H.thr_out.data(), H.thr_out.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_thr_in, H.thr_in.data(), H.thr_in.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_prof, prof.data(), prof.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_wfp, p->wf_p.data(), p->wf_p.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_wfdir, p->wf_dir.data(), p->wf_dir.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_wfscale, p->wf_scale.data(), p->wf_scale.size()))) return rc;
-    if ((rc = dupload(ctx, &p->d_S, p->S.data(), p->S.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_geom, H.geom.data(), H.geom.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_cyl, H.cyl.data(), H.cyl.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_inv, H.inv.data(), H.inv.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_L, H.L.data(), H.L.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_rate, H.rate.data(), H.rate.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_thr_out, H.thr_out.data(), H.thr_out.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_thr_in, H.thr_in.data(), H.thr_in.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_prof, prof.data(), prof.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_wfp, p->wf_p.data(), p->wf_p.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_wfdir, p->wf_dir.data(), p->wf_dir.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_wfscale, p->wf_scale.data(), p->wf_scale.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_S, p->S.data(), p->S.size()))) return rc;
     const size_t nst = (size_t)AXX_STATE_FIELDS(p->Pt) * (size_t)p->N;
-    if (hipMalloc((void**)&p->d_state, nst * sizeof(uint32_t)) != hipSuccess)
+    if (ar) {
+        if ((rc = dev_alloc(ctx, ar, (void**)&p->d_state, nst * sizeof(uint32_t)))) return rc;
+    } else if (hipMalloc((void**)&p->d_state, nst * sizeof(uint32_t)) != hipSuccess) {
         return fail(ctx, AXX_ERR_NOMEM, "walker state (%zu bytes)", nst * 4);
-    HIP_TRY(ctx, hipMemset(p->d_state, 0, nst * sizeof(uint32_t)));
-    if ((rc = dupload<int>(ctx, &p->d_err, nullptr, 1))) return rc;
-    HIP_TRY(ctx, hipMemset(p->d_err, 0, sizeof(int)));
+    }
+    if ((rc = zero_dev(ctx, ar, p->d_state, nst))) return rc;
+    if ((rc = dupload<int>(ctx, ar, &p->d_err, nullptr, 1))) return rc;
+    if ((rc = zero_dev(ctx, ar, p->d_err, 1))) return rc;
     p->chunk = kRedThreads * 16;
     p->n_chunks = (int)((p->N + p->chunk - 1) / p->chunk);
-    if ((rc = dupload<double>(ctx, &p->d_partial, nullptr, (size_t)p->M * p->n_chunks * kRedVals))) return rc;
-    if ((rc = dupload<double>(ctx, &p->d_out, nullptr, (size_t)p->M * kRedVals))) return rc;
-    if ((rc = dupload<unsigned long long>(ctx, &p->d_cnt, nullptr, 4))) return rc;
-    HIP_TRY(ctx, hipEventCreate(&p->ev0));
-    HIP_TRY(ctx, hipEventCreate(&p->ev1));
+    if ((rc = dupload<double>(ctx, ar, &p->d_partial, nullptr, (size_t)p->M * p->n_chunks * kRedVals))) return rc;
+    if ((rc = dupload<double>(ctx, ar, &p->d_out, nullptr, (size_t)p->M * kRedVals))) return rc;
+    if ((rc = dupload<unsigned long long>(ctx, ar, &p->d_cnt, nullptr, 4))) return rc;
+    if (!ar) {
+        HIP_TRY(ctx, hipEventCreate(&p->ev0));
+        HIP_TRY(ctx, hipEventCreate(&p->ev1));
+    }
+    p->arena = ar != nullptr;
     if (tune && tune->block) {
         p->block = tune->block;
     } else {
@@ -1072,6 +1129,29 @@ extern "C" int axx_trace(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
         (void)hipFree(d_ids);
         (void)hipFree(d_out);
         axx_plan_destroy(p);
+        return rc;
+    } catch (...) {
+        return api_exception();
+    }
+}
+
+extern "C" int axx_run_phase_noalloc(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q,
+                                     const axx_run* r, axx_result* out) {
+    try {
+        if (!ctx || !s || !q || !r || !out) return fail(ctx, AXX_ERR_INVALID, "null argument");
+        if (!ctx->scratch) return fail(ctx, AXX_ERR_DEVICE, "no scratch address on device %d", ctx->device);
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        DevArena ar{ctx->scratch, (size_t)AXX_NOALLOC_SCRATCH_BYTES, 0};
+        axx_plan pl;
+        pl.ctx = ctx;
+        int rc = plan_create_impl(ctx, s, q, r, nullptr, &pl, 1, &ar);
+        if (!rc) rc = launch_walk(&pl, 0, q->n_steps);
+        if (!rc) {
+            pl.next_step = q->n_steps;
+            rc = axx_plan_reduce(&pl);
+        }
+        if (!rc) rc = axx_plan_results(&pl, out);
+        if (rc) (void)hipStreamSynchronize(ctx->stream);   // nothing of this run left in flight
         return rc;
     } catch (...) {
         return api_exception();

@@ -23,7 +23,14 @@ struct axx_ctx {
     hipStream_t stream = nullptr;
     int n_cu = 0;
     size_t lds_max = 65536;
+    unsigned char* scratch = nullptr;   // this device's scratch array (allocation-free runs)
     std::string err;
+};
+
+// Bump allocator over the scratch array of an allocation-free run.
+struct DevArena {
+    unsigned char* base;
+    size_t size, used;
 };
 
 struct axx_plan {
@@ -46,6 +53,7 @@ struct axx_plan {
     int block = 256, wpt = 1;
     int next_step = 0;
     bool reduced = false;
+    bool arena = false;     // device buffers are pieces of the scratch array (not freed)
     // device buffers
     float4* d_geom = nullptr;
     int* d_meta = nullptr;
@@ -111,7 +119,7 @@ static inline int api_exception() {
 // Plan internals used by the batched runs.  min_pt: smallest padded
 // profile width to allocate moments for (1 for a plain plan).
 int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, const axx_run* r,
-                     const axx_tuning* tune, axx_plan* p, int min_pt);
+                     const axx_tuning* tune, axx_plan* p, int min_pt, DevArena* arena = nullptr);
 void plan_free(axx_plan* p);
 WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1);
 RedArgs plan_red_args(const axx_plan* p);

@@ -53,6 +53,7 @@ def lib():
     S, Q, R, Res = (C.POINTER(abi.Substrate), C.POINTER(abi.Acq), C.POINTER(abi.Run),
                     C.POINTER(abi.Result))
     L.axx_run_phase.argtypes = [vp, S, Q, R, Res]
+    L.axx_run_phase_noalloc.argtypes = [vp, S, Q, R, Res]
     L.axx_trace.argtypes = [vp, S, Q, R, C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_float),
                             C.POINTER(C.c_uint32)]
     L.axx_plan_create.argtypes = [vp, S, Q, R, C.POINTER(vp)]
@@ -239,6 +240,18 @@ class Plan:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def run_phase_noalloc(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,
+                      ctx: Optional[Context] = None) -> abi.Signals:
+    """One-shot run without device allocation (axx_run_phase_noalloc): the
+    run's device buffers are pieces of the library's per-GPU scratch array."""
+    ctx = ctx or default_context()
+    s, q, r = sub.struct(), acq.struct(), run.struct()
+    res, bufs = abi.alloc_result(acq.n_waveforms)
+    ctx.check(lib().axx_run_phase_noalloc(ctx.handle, C.byref(s), C.byref(q), C.byref(r), C.byref(res)),
+              "axx_run_phase_noalloc")
+    return abi.signals_from(res, bufs)
 
 
 def run_phase(sub: abi.SubstrateSpec, acq: abi.AcqSpec, run: abi.RunSpec,

@@ -200,6 +200,23 @@ int axx_run_phase(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
                   const axx_run* run, axx_result* out);
 
 /*
+ * axx_run_phase without device memory allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer of the run (substrate tables, profile table, walker state,
+ * reduction partials) is carved from one static scratch array of
+ * AXX_NOALLOC_SCRATCH_BYTES per GPU, held by the process from the library's
+ * load until it ends; host memory for the preparation of the substrate
+ * tables is still allocated.  A run whose buffers exceed the scratch returns
+ * AXX_ERR_LIMIT (axx_last_error says so).  The scratch keeps the last run's
+ * contents; runs on one GPU that overlap (two threads, or two contexts of
+ * one device) race.  Results equal axx_run_phase's bit for bit.
+ */
+#define AXX_NOALLOC_SCRATCH_BYTES 268435456ull   /* 256 MiB */
+int axx_run_phase_noalloc(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
+                          const axx_run* run, axx_result* out);
+
+/*
  * Trace (parity hook): the walk of axx_run_phase, step by step, with the
  * unwrapped positions (start + displacement, fp32) of k chosen walkers after
  * every step: xyz[(j * n_steps + i) * 3 + a] for walker walker_ids[j]
