@@ -59,40 +59,54 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-PROFILE = os.path.join(ROOT, "profiles", "walk_kernel_c2_hex.txt")
+# rocprofv3 summaries of the walk kernel of the single-run workloads at their
+# full size: (file, walkers, time steps)
+PROFILES = {
+    "c2_hex": ("walk_kernel_c2_hex.txt", 1000000, 250000),
+    "c3_gamma": ("walk_kernel_c3_gamma.txt", 4000000, 80000),
+}
 
 
-def _counted(flop, kernel_ms):
+def _profile(workload: str, walkers: int, n_steps: int):
+    """Path of the workload's profile summary when this run has its size."""
+    if workload not in PROFILES:
+        return None
+    name, w, n = PROFILES[workload]
+    path = os.path.join(ROOT, "profiles", name)
+    return path if (walkers, n_steps) == (w, n) and os.path.exists(path) else None
+
+
+def _counted(flop, kernel_ms, path):
     """Counter-based FP32 throughput of this run's walk kernel."""
     if flop is None:
         return None
     tf = flop / (kernel_ms * 1e-3) / 1e12
     return {"flop_per_launch": flop, "tflops": tf, "frac": tf / PEAK_FP32_TFLOPS,
-            "source": "64 x SQ_INSTS_VALU_FLOPS_FP32 per launch from profiles/walk_kernel_c2_hex.txt, "
+            "source": f"64 x SQ_INSTS_VALU_FLOPS_FP32 per launch from profiles/{os.path.basename(path)}, "
                       "over this run's kernel time"}
 
 
-def _profiled_traffic(walkers: int, n_steps: int):
+def _profiled_traffic(path):
     """HBM bytes per walk launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE (calibrated against a plain copy) for the default workload
-    size; None for other sizes or when the summary is absent."""
-    if walkers != 1000000 or n_steps != 250000 or not os.path.exists(PROFILE):
+    WRITE_SIZE (calibrated against a plain copy) for the workload at its
+    full size; None when there is no summary for this run."""
+    if path is None:
         return None
-    with open(PROFILE) as f:
+    with open(path) as f:
         for line in f:
             if line.strip().startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":", 1)[1])
     return None
 
 
-def _profiled_valu_flops(walkers: int, n_steps: int):
+def _profiled_valu_flops(path):
     """FP32 VALU work per walk launch as the counters count it: 64 lanes x
     SQ_INSTS_VALU_FLOPS_FP32 (ADD + MUL + 2 FMA + transcendental wave
-    instructions) from the rocprofv3 summary of the default workload; None
-    for other sizes or when the summary is absent."""
-    if walkers != 1000000 or n_steps != 250000 or not os.path.exists(PROFILE):
+    instructions) from the workload's rocprofv3 summary; None when there is
+    no summary for this run."""
+    if path is None:
         return None
-    with open(PROFILE) as f:
+    with open(path) as f:
         for line in f:
             parts = line.split()
             if len(parts) >= 2 and parts[0] == "SQ_INSTS_VALU_FLOPS_FP32":
@@ -182,6 +196,7 @@ def main(argv=None) -> int:
     rho = sig.collisions / ws
     F = wl.flops_per_walker_step(rho)
     achieved = F * ws / (avg_ms * 1e-3) / 1e12
+    prof = _profile(args.workload, wl.run.n_walkers, n)
 
     out = {
         "metric": "walker-steps/s",
@@ -207,6 +222,7 @@ def main(argv=None) -> int:
             "block": cfg["block"],
             "bin_width_um": round(cfg["bin_width"], 4),
             "lds_bytes": cfg["lds_bytes"],
+            "unit_cells": cfg.get("unit_cells", 1),
         },
         "roofline": {
             "bound": "valu",
@@ -214,14 +230,13 @@ def main(argv=None) -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": (_profiled_traffic(wl.run.n_walkers, n) if args.workload == "c2_hex"
-                        else None),
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload "
-                              "(profiles/walk_kernel_c2_hex.txt), not measured in this run",
+            "traffic": _profiled_traffic(prof),
+            "traffic_source": (f"rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload "
+                               f"(profiles/{os.path.basename(prof)}), not measured in this run" if prof else None),
             "flops_per_walker_step": F,
             "walker_steps_per_launch": ws,
             "kernel_ms": avg_ms,
-            "valu_fp32_counted": _counted(_profiled_valu_flops(wl.run.n_walkers, n), avg_ms),
+            "valu_fp32_counted": _counted(_profiled_valu_flops(prof), avg_ms, prof),
             "collisions_per_walker_step": rho,
         },
         "signal": {
