@@ -7,7 +7,7 @@
 // GELU (tanh form), the last layer is linear.
 //
 // Layout.  Every feature dimension is padded to a multiple of 128 (the GEMM
-// tile) and batches to a multiple of 512 (tiles x split-K slices of 64);
+// tile) and batches to a multiple of 512 (row tiles x split-K slices of 64);
 // padded weights, biases and activations are zero.  Activations are row
 // major [batch][features] in bf16.  Each layer keeps fp32 master weights
 // Wt[out][in] (+ bias) for Adam and two bf16 copies: Wt[out][in] (forward:
@@ -15,10 +15,12 @@
 // gradient).
 //
 // Kernels.
-//   gemm_nt<EPI>  C[m][n] = sum_k A[m][k] * B[n][k], 128 x 128 x 64 block
-//                 tiles through LDS, 4 waves of 64 x 64, v_mfma_f32_32x32x16_bf16;
-//                 fused epilogues (bias + GELU, bias, GELU backward, split-K
-//                 fp32 partial);
+//   gemm_nt<EPI, Tile>  C[m][n] = sum_k A[m][k] * B[n][k], 256 x 256 block
+//                 tiles (8 waves of 64 x 128, K slabs of 64) or 128 x 128 (4
+//                 waves of 64 x 64, slabs of 32), slabs double-buffered
+//                 through LDS, v_mfma_f32_32x32x16_bf16; fused epilogues
+//                 storing 4 adjacent columns per lane (bias + GELU, bias,
+//                 GELU backward, split-K fp32 partial);
 //   transpose_bf16, mse_grad, bias_grad, adam (sums the split-K partials in
 //   a fixed order: deterministic), gather (minibatch rows by Philox index).
 #include "internal.h"
@@ -28,8 +30,9 @@ namespace axx {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBM = 128, kBN = 128, kBK = 64, kLDK = kBK + 8;
-constexpr int kSplitK = 8;   // weight-gradient GEMMs split the batch reduction
+constexpr int kBM = 128, kBN = 128;   // row and column granule of every GEMM
+constexpr int kRowPad = 512;  // batches are padded to this (row tiles x split-K slices of 64)
+constexpr int kSplitK = 8;    // weight-gradient GEMMs split the batch reduction
 constexpr int kBiasSlices = 32;   // bias gradients: row slices summed in order by Adam
 
 enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_BWD = 2, EPI_PARTIAL = 3 };
@@ -37,100 +40,184 @@ enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_BWD = 2, EPI_PARTIAL = 3 };
 struct EpiArgs {
     const float* __restrict__ bias;    // [N] (EPI_BIAS*)
     __bf16* __restrict__ out_bf;       // bf16 output [M][ldc]
-    __bf16* __restrict__ out_z;        // pre-activation [M][ldc] (EPI_BIAS_GELU)
+    __bf16* __restrict__ out_z;        // pre-activation [M][ldc] (EPI_BIAS_GELU; null: not kept)
     float* __restrict__ out_f;         // fp32 output [M][ldc] (EPI_BIAS, EPI_PARTIAL: + slice * M * ldc)
     const __bf16* __restrict__ zprev;  // pre-activation for GELU backward [M][ldc]
     int ldc;
 };
 
-__device__ __forceinline__ float gelu_f(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * s with s = 1 / (1 + exp(-2u)) (0.5 (1 + tanh u) = s exactly);
+// the derivative is s + x s (1 - s) 2 u'(x) with 1 - tanh^2 u = 4 s (1 - s).
+// exp and the reciprocal are the hardware's (about 1 ulp): the results are
+// rounded to bf16 (8 bits) when stored.
+__device__ __forceinline__ float gelu_s(float x) {
     const float c = 0.7978845608028654f;   // sqrt(2/pi)
     const float u = c * (x + 0.044715f * x * x * x);
-    return 0.5f * x * (1.0f + tanhf(u));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
+
+__device__ __forceinline__ float gelu_f(float x) { return x * gelu_s(x); }
 
 __device__ __forceinline__ float gelu_grad(float x) {
     const float c = 0.7978845608028654f;
-    const float x2 = x * x;
-    const float t = tanhf(c * (x + 0.044715f * x2 * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x2);
+    const float s = gelu_s(x);
+    return s + 2.0f * x * s * (1.0f - s) * c * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
-// C = A . B^T over k in [kbeg, kend); A [M][lda], B [N][ldb] bf16, M % 128,
-// N % 128 and the k range % 64 are zero (checked by the host).  Grid
-// (N / 128, M / 128, slices).
-template <int EPI>
-__global__ __launch_bounds__(256) void gemm_nt(const __bf16* __restrict__ A, int lda,
-                                               const __bf16* __restrict__ B, int ldb, int M, int K_slice,
-                                               EpiArgs E) {
-    __shared__ __align__(16) __bf16 As[kBM][kLDK];
-    __shared__ __align__(16) __bf16 Bs[kBN][kLDK];
-    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-    const int kbeg = blockIdx.z * K_slice, kend = kbeg + K_slice;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint2 pack_bf16x4(float a, float b, float c, float d) {
+    const bf16x4 h = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(uint2, h);
+}
+
+__device__ __forceinline__ float4 unpack_bf16x4(uint2 v) {
+    const bf16x4 h = __builtin_bit_cast(bf16x4, v);
+    return make_float4((float)h[0], (float)h[1], (float)h[2], (float)h[3]);
+}
+
+// Block tile shapes of gemm_nt: WM x WN waves, each MT x NT tiles of
+// 32 x 32, K slabs of BK; WPE = waves per SIMD the register budget allows.
+template <int MT_, int NT_, int WM_, int WN_, int BK_, int WPE_>
+struct GemmTile {
+    static constexpr int MT = MT_, NT = NT_, WM = WM_, WN = WN_, BK = BK_, WPE = WPE_;
+    static constexpr int BM = 32 * MT * WM, BN = 32 * NT * WN, T = 64 * WM * WN;
+    static constexpr int LDK = BK + 8;       // LDS row pitch BK + 8 bf16: conflict-free b128 reads
+    static constexpr int CPR = BK / 8;       // 16-B chunks per slab row
+    static constexpr int RS = T / CPR;       // rows per staging pass
+    static constexpr int CA = BM / RS, CB = BN / RS;   // staged chunks per thread
+    static constexpr int LDS = 2 * (BM + BN) * LDK * 2;
+    static_assert(CA * RS == BM && CB * RS == BN && CA <= 4 && CB <= 4, "staging");
+};
+// 256 x 256, 8 waves of 64 x 128, K slabs of 64 (144 KB LDS: one block per CU)
+using TileBig = GemmTile<2, 4, 4, 2, 64, 2>;
+// 128 x 128, 4 waves of 64 x 64, K slabs of 32 (40 KB LDS: three blocks per CU)
+using TileSmall = GemmTile<2, 2, 2, 2, 32, 3>;
+
+// C = A . B^T over k in [kbeg, kbeg + K_slice); A [M][lda], B [N][ldb] bf16.
+// The K loop runs over BK-wide slabs held in two LDS stages: slab t + 1's
+// global loads (made at slab t - 1) are written to the free stage after slab
+// t's MFMAs, and slab t + 2's loads are issued then, so they are in flight
+// during the next slab's MFMAs; one barrier per slab.  The MFMA takes the B
+// fragment as its first operand, so that a lane's 16 accumulators of a tile
+// are one row m (lane % 32) and columns 8q + 4 (lane / 32) + r: the epilogue
+// stores 4 adjacent columns per instruction.  Grid: (M / BM) * (N / BN)
+// tiles along x, ordered so that the 8 XCDs (dispatch round-robin) each take
+// a contiguous run of row tiles and read each A row slab into one L2;
+// split-K slices along y.  M % BM, N % BN and K_slice % BK are zero (checked
+// by the host).
+template <int EPI, class TL>
+__global__ __launch_bounds__(TL::T) __attribute__((amdgpu_waves_per_eu(TL::WPE))) void gemm_nt(
+    const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B, int ldb, int M, int N, int K_slice,
+    EpiArgs E) {
+    constexpr int MT = TL::MT, NT = TL::NT, BM = TL::BM, BN = TL::BN, BK = TL::BK, RS = TL::RS;
+    __shared__ __align__(16) __bf16 As[2][BM][TL::LDK];
+    __shared__ __align__(16) __bf16 Bs[2][BN][TL::LDK];
+    const int nbn = N / BN, nblk = (M / BM) * nbn;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int n0 = (b % nbn) * BN, m0 = (b / nbn) * BM;
+    const int kbeg = blockIdx.y * K_slice, nk = K_slice / BK;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    f32x16 acc[2][2];
+    const int wm = wave % TL::WM, wn = wave / TL::WM;
+    // global -> register staging: thread t moves 16 B of rows t / CPR + RS i,
+    // bf16 columns 8 (t % CPR) .. + 7 of the slab
+    const int lr = tid / TL::CPR, lc = (tid % TL::CPR) * 8;
+    const __bf16* Ag = A + (size_t)(m0 + lr) * lda + kbeg + lc;
+    const __bf16* Bg = B + (size_t)(n0 + lr) * ldb + kbeg + lc;
+    const size_t arow = (size_t)RS * lda, brow = (size_t)RS * ldb;
+    // staged slab in named registers (an array here would be placed in LDS
+    // or scratch by the compiler's private-array handling)
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define AXX_LA(i) if constexpr (i < TL::CA) ra##i = *reinterpret_cast<const uint4*>(Ag + (i) * arow + ko);
+#define AXX_LB(i) if constexpr (i < TL::CB) rb##i = *reinterpret_cast<const uint4*>(Bg + (i) * brow + ko);
+#define AXX_SA(i) if constexpr (i < TL::CA) *reinterpret_cast<uint4*>(&As[st][lr + RS * (i)][lc]) = ra##i;
+#define AXX_SB(i) if constexpr (i < TL::CB) *reinterpret_cast<uint4*>(&Bs[st][lr + RS * (i)][lc]) = rb##i;
+#define AXX_GEMM_LOAD(t)                                                  \
+    do {                                                                  \
+        const int ko = (t) * BK;                                          \
+        AXX_LA(0) AXX_LA(1) AXX_LA(2) AXX_LA(3)                           \
+        AXX_LB(0) AXX_LB(1) AXX_LB(2) AXX_LB(3)                           \
+    } while (0)
+#define AXX_GEMM_STORE(s)                                                 \
+    do {                                                                  \
+        const int st = (s);                                               \
+        AXX_SA(0) AXX_SA(1) AXX_SA(2) AXX_SA(3)                           \
+        AXX_SB(0) AXX_SB(1) AXX_SB(2) AXX_SB(3)                           \
+    } while (0)
+    f32x16 acc[MT][NT];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NT; ++j)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+    AXX_GEMM_LOAD(0);
+    AXX_GEMM_STORE(0);
+    if (nk > 1) AXX_GEMM_LOAD(1);
+    __syncthreads();
+    for (int t = 0; t < nk; ++t) {
+        const int cur = t & 1;
 #pragma unroll
-        for (int c = tid; c < kBM * kBK / 8; c += 256) {
-            const int r = c >> 3, kc = (c & 7) * 8;
-            *reinterpret_cast<uint4*>(&As[r][kc]) =
-                *reinterpret_cast<const uint4*>(&A[(size_t)(m0 + r) * lda + k0 + kc]);
-            *reinterpret_cast<uint4*>(&Bs[r][kc]) =
-                *reinterpret_cast<const uint4*>(&B[(size_t)(n0 + r) * ldb + k0 + kc]);
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 a[MT], bb[NT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][wm * 32 * MT + i * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                bb[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn * 32 * NT + j * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bb[j], a[i], acc[i][j], 0, 0, 0);
         }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kBK; kk += 16) {
-            bf16x8 a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + i * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + j * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (t + 1 < nk) {
+            AXX_GEMM_STORE(cur ^ 1);
+            if (t + 2 < nk) AXX_GEMM_LOAD(t + 2);
         }
         __syncthreads();
     }
-    // acc[i][j][4 * q + r]: row 8q + 4 (lane / 32) + r, column lane % 32 of the
-    // 32 x 32 tile (i, j)
+#undef AXX_GEMM_LOAD
+#undef AXX_GEMM_STORE
+#undef AXX_LA
+#undef AXX_LB
+#undef AXX_SA
+#undef AXX_SB
+    // acc[i][j][4 q + r]: row m0 + 32 (MT wm + i) + lane % 32, column
+    // n0 + 32 (NT wn + j) + 8 q + 4 (lane / 32) + r
+    const int row0 = m0 + wm * 32 * MT + (lane & 31);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < NT; ++j)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-            float bcol = 0.0f;
-            if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS) bcol = E.bias[col];
+        for (int q = 0; q < 4; ++q) {
+            const int col = n0 + wn * 32 * NT + j * 32 + 8 * q + 4 * (lane >> 5);
+            float4 bc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS) bc = *reinterpret_cast<const float4*>(E.bias + col);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5) + r;
-                    const size_t o = (size_t)row * E.ldc + col;
-                    const float v = acc[i][j][4 * q + r];
-                    if (EPI == EPI_BIAS_GELU) {
-                        const float z = v + bcol;
-                        E.out_z[o] = (__bf16)z;
-                        E.out_bf[o] = (__bf16)gelu_f(z);
-                    } else if (EPI == EPI_BIAS) {
-                        E.out_f[o] = v + bcol;
-                    } else if (EPI == EPI_GELU_BWD) {
-                        E.out_bf[o] = (__bf16)(v * gelu_grad((float)E.zprev[o]));
-                    } else {
-                        E.out_f[(size_t)blockIdx.z * M * E.ldc + o] = v;
-                    }
+            for (int i = 0; i < MT; ++i) {
+                const size_t o = (size_t)(row0 + 32 * i) * E.ldc + col;
+                const f32x16& v = acc[i][j];
+                if (EPI == EPI_BIAS_GELU) {
+                    const float z0 = v[4 * q] + bc.x, z1 = v[4 * q + 1] + bc.y;
+                    const float z2 = v[4 * q + 2] + bc.z, z3 = v[4 * q + 3] + bc.w;
+                    if (E.out_z) *reinterpret_cast<uint2*>(E.out_z + o) = pack_bf16x4(z0, z1, z2, z3);
+                    *reinterpret_cast<uint2*>(E.out_bf + o) = pack_bf16x4(gelu_f(z0), gelu_f(z1), gelu_f(z2), gelu_f(z3));
+                } else if (EPI == EPI_BIAS) {
+                    *reinterpret_cast<float4*>(E.out_f + o) =
+                        make_float4(v[4 * q] + bc.x, v[4 * q + 1] + bc.y, v[4 * q + 2] + bc.z, v[4 * q + 3] + bc.w);
+                } else if (EPI == EPI_GELU_BWD) {
+                    const float4 z = unpack_bf16x4(*reinterpret_cast<const uint2*>(E.zprev + o));
+                    *reinterpret_cast<uint2*>(E.out_bf + o) =
+                        pack_bf16x4(v[4 * q] * gelu_grad(z.x), v[4 * q + 1] * gelu_grad(z.y),
+                                    v[4 * q + 2] * gelu_grad(z.z), v[4 * q + 3] * gelu_grad(z.w));
+                } else {
+                    *reinterpret_cast<float4*>(E.out_f + (size_t)blockIdx.y * M * E.ldc + o) =
+                        make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
                 }
+            }
         }
 }
 
@@ -292,6 +379,8 @@ int pad_to(int v, int m) { return (v + m - 1) / m * m; }
 
 struct Layer {
     int in = 0, out = 0, inp = 0, outp = 0;   // real and padded sizes
+    int kin = 0;   // reduction length of the forward GEMM: inp, or for the first layer in padded to 64
+                   // (the input columns beyond it are zero in both operands)
     bool gelu = true;
     float *w = nullptr, *mw = nullptr, *vw = nullptr, *b = nullptr, *mb = nullptr, *vb = nullptr;
     float *gpart = nullptr, *gb = nullptr;
@@ -348,17 +437,32 @@ int dalloc(axx_ctx* ctx, T** p, size_t n, bool zero = true) {
     return AXX_OK;
 }
 
+template <int EPI>
+void launch_epi(axx_ctx* ctx, bool big, dim3 grid, const __bf16* A, int lda, const __bf16* B, int ldb, int M,
+                int N, int ks, const EpiArgs& E) {
+    if (big)
+        hipLaunchKernelGGL((gemm_nt<EPI, TileBig>), grid, dim3(TileBig::T), 0, ctx->stream, A, lda, B, ldb, M, N, ks, E);
+    else
+        hipLaunchKernelGGL((gemm_nt<EPI, TileSmall>), grid, dim3(TileSmall::T), 0, ctx->stream, A, lda, B, ldb, M, N,
+                           ks, E);
+}
+
+// 256 x 256 tiles where the shape divides and the grid still holds a block
+// per CU, else 128 x 128.
 int launch_gemm(axx_ctx* ctx, int epi, const __bf16* A, int lda, const __bf16* B, int ldb, int M, int N, int K,
                 int slices, const EpiArgs& E) {
-    if (M % kBM || N % kBN || K % (kBK * slices))
+    if (M % TileSmall::BM || N % TileSmall::BN || K % (TileSmall::BK * slices) || (E.ldc & 3))
         return fail(ctx, AXX_ERR_INVALID, "gemm shape %d x %d x %d (slices %d) not tiled", M, N, K, slices);
-    const dim3 grid(N / kBN, M / kBM, slices);
     const int ks = K / slices;
+    const bool big = M % TileBig::BM == 0 && N % TileBig::BN == 0 && ks % TileBig::BK == 0 &&
+                     (long long)(M / TileBig::BM) * (N / TileBig::BN) * slices >= 256;
+    const int bm = big ? TileBig::BM : TileSmall::BM, bn = big ? TileBig::BN : TileSmall::BN;
+    const dim3 grid((unsigned)((M / bm) * (N / bn)), (unsigned)slices);
     switch (epi) {
-    case EPI_BIAS_GELU: hipLaunchKernelGGL(gemm_nt<EPI_BIAS_GELU>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
-    case EPI_BIAS: hipLaunchKernelGGL(gemm_nt<EPI_BIAS>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
-    case EPI_GELU_BWD: hipLaunchKernelGGL(gemm_nt<EPI_GELU_BWD>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
-    default: hipLaunchKernelGGL(gemm_nt<EPI_PARTIAL>, grid, dim3(256), 0, ctx->stream, A, lda, B, ldb, M, ks, E); break;
+    case EPI_BIAS_GELU: launch_epi<EPI_BIAS_GELU>(ctx, big, grid, A, lda, B, ldb, M, N, ks, E); break;
+    case EPI_BIAS: launch_epi<EPI_BIAS>(ctx, big, grid, A, lda, B, ldb, M, N, ks, E); break;
+    case EPI_GELU_BWD: launch_epi<EPI_GELU_BWD>(ctx, big, grid, A, lda, B, ldb, M, N, ks, E); break;
+    default: launch_epi<EPI_PARTIAL>(ctx, big, grid, A, lda, B, ldb, M, N, ks, E); break;
     }
     HIP_TRY(ctx, hipGetLastError());
     return AXX_OK;
@@ -370,8 +474,9 @@ int transpose(axx_ctx* ctx, const __bf16* in, int R, int Cc, int ldi, __bf16* ou
     return AXX_OK;
 }
 
-// Forward over n rows already in H[0] (n % 128 == 0).
-int forward(axx_mlp* m, int n) {
+// Forward over n rows already in H[0] (n % 128 == 0); keep_z: store the
+// hidden layers' pre-activations for a backward pass.
+int forward(axx_mlp* m, int n, bool keep_z) {
     axx_ctx* ctx = m->ctx;
     const int nl = (int)m->L.size();
     for (int l = 0; l < nl; ++l) {
@@ -382,11 +487,11 @@ int forward(axx_mlp* m, int n) {
         int rc;
         if (y.gelu) {
             E.out_bf = m->H[l + 1];
-            E.out_z = m->Z[l];
-            rc = launch_gemm(ctx, EPI_BIAS_GELU, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.inp, 1, E);
+            E.out_z = keep_z ? m->Z[l] : nullptr;
+            rc = launch_gemm(ctx, EPI_BIAS_GELU, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.kin, 1, E);
         } else {
             E.out_f = m->Y;
-            rc = launch_gemm(ctx, EPI_BIAS, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.inp, 1, E);
+            rc = launch_gemm(ctx, EPI_BIAS, m->H[l], y.inp, y.wt_bf, y.inp, n, y.outp, y.kin, 1, E);
         }
         if (rc) return rc;
     }
@@ -505,7 +610,7 @@ extern "C" int axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int m
         m = new (std::nothrow) axx_mlp();
         if (!m) return AXX_ERR_NOMEM;
         m->ctx = ctx;
-        m->max_batch = pad_to(max_batch, kBK * kSplitK);
+        m->max_batch = pad_to(max_batch, kRowPad);
         int maxdim = 0;
         for (int l = 0; l < n_layers; ++l) {
             Layer y;
@@ -514,6 +619,7 @@ extern "C" int axx_mlp_create(axx_ctx* ctx, int n_layers, const int* dims, int m
             // inputs of layer l are the padded outputs of layer l - 1
             y.inp = l == 0 ? pad_to(y.in, kBN) : m->L[l - 1].outp;
             y.outp = pad_to(y.out, kBN);
+            y.kin = l == 0 ? std::min(y.inp, pad_to(y.in, TileBig::BK)) : y.inp;
             y.gelu = l + 1 < n_layers;
             maxdim = std::max(maxdim, std::max(y.inp, y.outp));
             m->L.push_back(y);
@@ -629,7 +735,7 @@ extern "C" int axx_mlp_forward(axx_mlp* m, const float* x, int n, float* y) {
         HIP_TRY(ctx, hipSetDevice(ctx->device));
         const int n_pad = pad_to(n, kBM);
         int rc = load_inputs(m, x, n, n_pad);
-        if (!rc) rc = forward(m, n_pad);
+        if (!rc) rc = forward(m, n_pad, false);
         if (rc) return rc;
         const Layer& last = m->L.back();
         float* tmp = m->xin;   // reuse the staging buffer for the unpadded output
@@ -664,7 +770,7 @@ extern "C" int axx_mlp_time_forward(axx_mlp* m, int n, float* ms) {
         HIP_TRY(ctx, hipEventCreate(&e0));
         HIP_TRY(ctx, hipEventCreate(&e1));
         HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
-        int rc = forward(m, pad_to(n, kBM));
+        int rc = forward(m, pad_to(n, kBM), false);
         HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
         HIP_TRY(ctx, hipEventSynchronize(e1));
         HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
@@ -683,13 +789,13 @@ extern "C" int axx_mlp_loss_grad(axx_mlp* m, const float* x, const float* t, int
         if (!m || !x || !t || n < 1 || n > m->max_batch) return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp grad: n");
         axx_ctx* ctx = m->ctx;
         HIP_TRY(ctx, hipSetDevice(ctx->device));
-        const int n_pad = pad_to(n, kBK * kSplitK);   // M tiles and split-K slices
+        const int n_pad = pad_to(n, kRowPad);   // M tiles and split-K slices
         const Layer& last = m->L.back();
         int rc = load_inputs(m, x, n, n_pad);
         if (rc) return rc;
         HIP_TRY(ctx, hipMemsetAsync(m->T, 0, (size_t)n_pad * last.out * sizeof(float), ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(m->T, t, (size_t)n * last.out * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = forward(m, n_pad))) return rc;
+        if ((rc = forward(m, n_pad, true))) return rc;
         if (n_pad != n) {
             // padded rows must not contribute: their targets equal their outputs
             HIP_TRY(ctx, hipMemcpy2DAsync(m->T + (size_t)n * last.out, last.out * sizeof(float),
@@ -771,8 +877,8 @@ extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_
     try {
         if (!m || steps < 1 || batch < 1 || batch > m->max_batch || !m->dX)
             return fail(m ? m->ctx : nullptr, AXX_ERR_INVALID, "mlp train: arguments or no data");
-        if (batch % (kBK * kSplitK) || batch % kBM)
-            return fail(m->ctx, AXX_ERR_INVALID, "mlp train: batch must be a multiple of %d", kBK * kSplitK);
+        if (batch % kRowPad)
+            return fail(m->ctx, AXX_ERR_INVALID, "mlp train: batch must be a multiple of %d", kRowPad);
         axx_ctx* ctx = m->ctx;
         HIP_TRY(ctx, hipSetDevice(ctx->device));
         const Layer& f = m->L[0];
@@ -785,7 +891,7 @@ extern "C" int axx_mlp_train(axx_mlp* m, int steps, int batch, float lr, uint64_
                                (const float*)m->dT, last.out, m->n_data, batch, (uint32_t)(m->adam_t + 1),
                                (uint32_t)seed, (uint32_t)(seed >> 32), m->H[0], m->T);
             HIP_TRY(ctx, hipGetLastError());
-            int rc = forward(m, batch);
+            int rc = forward(m, batch, true);
             if (!rc) rc = backward(m, batch);
             if (!rc) rc = adam_step(m, lr);
             if (rc) {

@@ -4,10 +4,11 @@
         one device-to-device copy of a known size (calibrates the HBM byte
         counters FETCH_SIZE / WRITE_SIZE against a plain copy kernel)
 
-    python -m axonexa_amd.profiling mlp [--batch 65536] [--train-steps 20]
+    python -m axonexa_amd.profiling mlp [--batch 65536] [--train-steps 20] [--library]
         the surrogate network (268 -> 1024 x 6 -> 30): 5 forward passes at
         the batch, then Adam steps at batch 4096 on random data (a short
-        program for a kernel trace)
+        program for a kernel trace); --library times the same forward pass
+        and one 1024 x 1024 layer through torch's bf16 GEMMs instead
 
     python -m axonexa_amd.profiling summarize <results.db> [--kernel NAME]
         print per-kernel dispatch count, mean duration and the mean of every
@@ -53,6 +54,50 @@ def mlp(batch: int, train_steps: int) -> None:
         m.set_data(x[:8192], t)
         tr = m.train(train_steps, batch=4096, lr=1e-3, seed=1)
         print(f"{train_steps} Adam steps at batch 4096: loss {tr[0]:.4f} -> {tr[-1]:.4f}")
+
+
+def mlp_library(batch: int) -> None:
+    """The same forward pass through torch (its bf16 library GEMMs, bias and
+    GELU as separate kernels), timed with events on torch's stream: what the
+    vendor library reaches on these shapes.  Run in a process of its own:
+    torch's HIP runtime and this package's are separate libraries."""
+    import torch
+
+    dims = [268] + [1024] * 6 + [30]
+    flops = 2.0 * batch * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    ws = [torch.randn(b, a, device="cuda", dtype=torch.bfloat16, generator=g) * a ** -0.5
+          for a, b in zip(dims[:-1], dims[1:])]
+    bs = [torch.zeros(b, device="cuda", dtype=torch.bfloat16) for b in dims[1:]]
+    x = torch.randn(batch, dims[0], device="cuda", dtype=torch.bfloat16, generator=g)
+
+    def run():
+        h = x
+        for i, (w, b) in enumerate(zip(ws, bs)):
+            h = torch.nn.functional.linear(h, w, b)
+            if i + 1 < len(ws):
+                h = torch.nn.functional.gelu(h, approximate="tanh")
+        return h
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            out.append(e0.elapsed_time(e1))
+        return sorted(out)[2]
+
+    ms = timed(run)
+    print(f"torch forward (linear + gelu) batch {batch}: {ms:.3f} ms, {flops / ms / 1e9:.1f} TFLOP/s")
+    a = torch.randn(batch, 1024, device="cuda", dtype=torch.bfloat16, generator=g)
+    w = ws[1].contiguous()
+    ms = timed(lambda: torch.matmul(a, w.t()))
+    print(f"torch bf16 GEMM {batch} x 1024 x 1024: {ms:.3f} ms, {2.0 * batch * 1024 * 1024 / ms / 1e9:.1f} TFLOP/s")
 
 
 def _kernels(con):
@@ -124,6 +169,8 @@ def main(argv=None):
     ml = sub.add_parser("mlp")
     ml.add_argument("--batch", type=int, default=65536)
     ml.add_argument("--train-steps", type=int, default=20)
+    ml.add_argument("--library", action="store_true",
+                    help="time the same forward pass through torch's library GEMMs instead")
     s = sub.add_parser("summarize")
     s.add_argument("db")
     s.add_argument("--kernel", default="")
@@ -132,7 +179,7 @@ def main(argv=None):
     if args.cmd == "copy":
         copy(args.mib)
     elif args.cmd == "mlp":
-        mlp(args.batch, args.train_steps)
+        mlp_library(args.batch) if args.library else mlp(args.batch, args.train_steps)
     else:
         print(summarize(args.db, args.kernel, args.durations))
 

@@ -130,8 +130,9 @@ def main(argv=None) -> int:
                     help="c4_batch: LDS bytes per workgroup for substrate tables (0: library default)")
     ap.add_argument("--mag-fields", type=int, default=2, help="mag_lf: fields per walker")
     ap.add_argument("--mag-integrator", type=int, default=0, help="mag_lf: 0 rotate, 1 RK4, 2 DOPRI")
-    ap.add_argument("--sur-geoms", type=int, default=4096,
-                    help="c5_surrogate: training geometries (8 queries each)")
+    ap.add_argument("--sur-geoms", type=int, default=8448,
+                    help="c5_surrogate: training geometries (8 queries of 30 signals each: "
+                         "8448 -> 2.03e6 signals)")
     ap.add_argument("--sur-steps", type=int, default=4000, help="c5_surrogate: Adam steps")
     ap.add_argument("--no-unbatched", action="store_true",
                     help="c4_batch: skip timing the instances one plan at a time")
@@ -609,7 +610,7 @@ def bench_surrogate(args) -> int:
                         axis=1)
     vh = S.physical(uh)
     gh = S.geometries(uh[:, :3], seed=21, processes=_cpu_threads())
-    truth_walkers = 20000
+    truth_walkers = 100000
     Yh = S.simulate(vh, gh, np.arange(H), proto, truth_walkers, seed=777, first_instance=10 ** 6)
     Xh = S.features(uh, np.stack([S.waveform_samples(vh[q], proto) for q in range(H)]))
     Ph = sur.predict_features(Xh)

@@ -3,8 +3,11 @@ engine, training with the project's MFMA kernels, held-out error against
 Monte Carlo, the Monte Carlo fallback outside the training box, and what the
 network gains over the engine (queries/s at matched noise).
 
-Sizes are reduced (2,048 training queries of a 10 ms protocol) so the test
-runs in about a minute; the full-size figures come from
+Sizes are reduced (16,384 training queries of a 10 ms protocol, 2,000
+walkers each) so the test runs in well under a minute; the held-out truth
+uses 40,000 walkers (noise sigma = 1/sqrt(2N) = 0.0035, mean absolute noise
+0.0028), so the 1 % gate on the mean absolute error is dominated by the
+network's own error.  The full-size figures come from
 ``bench.py --workload c5_surrogate``.
 """
 import math
@@ -31,19 +34,19 @@ def _data(proto, G, variants, seed, walkers, first_instance=0):
 
 def test_surrogate_learns_the_engine_and_falls_back_outside_its_box():
     proto = S.Protocol(TE=10.0, dt=5e-3)
-    u, X, Y = _data(proto, 512, 4, seed=31, walkers=1000)
+    u, X, Y = _data(proto, 4096, 4, seed=31, walkers=2000)
     print(f"[surrogate] {len(X)} training queries simulated", flush=True)
-    uh, Xh, Yh = _data(proto, 64, 1, seed=41, walkers=8000, first_instance=10 ** 6)
+    uh, Xh, Yh = _data(proto, 128, 1, seed=41, walkers=40000, first_instance=10 ** 6)
     sur = S.Surrogate(proto, max_batch=4096)
     try:
         sur.init(0)
-        trace = sur.fit(X, Y.astype(np.float32), steps=600, batch=2048, lr=1e-3)
+        trace = sur.fit(X, Y.astype(np.float32), steps=3000, batch=4096, lr=1e-3)
         P = sur.predict_features(Xh)
         mae = float(np.abs(P - Yh).mean())
         base = float(np.abs(Y.mean(axis=0) - Yh).mean())      # predict the mean signal
         print(f"[surrogate] loss {trace[:10].mean():.4f} -> {trace[-10:].mean():.5f}; held-out MAE "
               f"{mae:.4f} vs mean-predictor {base:.4f}", flush=True)
-        assert mae < 0.03 and mae < 0.25 * base
+        assert mae < 0.01 and mae < 0.25 * base
         # a query outside the training box is answered by Monte Carlo, exactly
         uo = uh[:3].copy()
         uo[1, 2] = 1.1                                          # packing fraction beyond the box
