@@ -194,8 +194,12 @@ struct Json {
     }
 };
 
+// Recursive descent over the whole text; malformed text throws
+// std::runtime_error (never reads past the end).  Nesting deeper than
+// kMaxDepth throws as well, so a hostile file cannot exhaust the stack.
 class JsonParser {
   public:
+    static constexpr int kMaxDepth = 64;
     explicit JsonParser(const std::string& t) : s(t) {}
     Json parse() {
         Json v = value();
@@ -207,6 +211,7 @@ class JsonParser {
   private:
     const std::string& s;
     size_t i = 0;
+    int depth = 0;
     [[noreturn]] void fail(const char* what) const {
         throw std::runtime_error(std::string("substrate JSON: ") + what + " at offset " + std::to_string(i));
     }
@@ -219,7 +224,7 @@ class JsonParser {
         return false;
     }
     std::string string_() {
-        if (s[i] != '"') fail("expected a string");
+        if (peek() != '"') fail("expected a string");
         ++i;
         std::string out;
         while (i < s.size() && s[i] != '"') {
@@ -237,41 +242,25 @@ class JsonParser {
         ++i;
         return out;
     }
+    char peek() const {
+        if (i >= s.size()) fail("unexpected end");
+        return s[i];
+    }
+    struct Nest {
+        JsonParser& p;
+        explicit Nest(JsonParser& q) : p(q) {
+            if (++p.depth > kMaxDepth) p.fail("nesting too deep");
+        }
+        ~Nest() { --p.depth; }
+    };
     Json value() {
         ws();
         if (i >= s.size()) fail("unexpected end");
         Json v;
         const char c = s[i];
-        if (c == '{') {
-            v.kind = Json::OBJ;
-            ++i;
-            ws();
-            if (s[i] == '}') { ++i; return v; }
-            for (;;) {
-                ws();
-                std::string k = string_();
-                ws();
-                if (s[i] != ':') fail("expected ':'");
-                ++i;
-                v.obj.emplace_back(k, value());
-                ws();
-                if (s[i] == ',') { ++i; continue; }
-                if (s[i] == '}') { ++i; return v; }
-                fail("expected ',' or '}'");
-            }
-        }
-        if (c == '[') {
-            v.kind = Json::ARR;
-            ++i;
-            ws();
-            if (s[i] == ']') { ++i; return v; }
-            for (;;) {
-                v.arr.push_back(value());
-                ws();
-                if (s[i] == ',') { ++i; continue; }
-                if (s[i] == ']') { ++i; return v; }
-                fail("expected ',' or ']'");
-            }
+        if (c == '{' || c == '[') {
+            Nest guard(*this);
+            return c == '{' ? object() : array();
         }
         if (c == '"') { v.kind = Json::STR; v.str = string_(); return v; }
         if (lit("null")) return v;
@@ -283,6 +272,39 @@ class JsonParser {
         i = (size_t)(end - s.c_str());
         v.kind = Json::NUM;
         return v;
+    }
+    Json object() {
+        Json v;
+        v.kind = Json::OBJ;
+        ++i;
+        ws();
+        if (peek() == '}') { ++i; return v; }
+        for (;;) {
+            ws();
+            std::string k = string_();
+            ws();
+            if (peek() != ':') fail("expected ':'");
+            ++i;
+            v.obj.emplace_back(k, value());
+            ws();
+            if (peek() == ',') { ++i; continue; }
+            if (peek() == '}') { ++i; return v; }
+            fail("expected ',' or '}'");
+        }
+    }
+    Json array() {
+        Json v;
+        v.kind = Json::ARR;
+        ++i;
+        ws();
+        if (peek() == ']') { ++i; return v; }
+        for (;;) {
+            v.arr.push_back(value());
+            ws();
+            if (peek() == ',') { ++i; continue; }
+            if (peek() == ']') { ++i; return v; }
+            fail("expected ',' or ']'");
+        }
     }
 };
 
@@ -297,6 +319,15 @@ inline double json_real(const Json* v, double dflt) {
     if (!v || v->kind == Json::NUL) return dflt;
     if (v->kind != Json::NUM) throw std::runtime_error("substrate JSON: expected a number");
     return v->num;
+}
+
+// The three numbers of an array member ("cell", "center", "normal").
+inline void json_vec3(const Json* v, double out[3], const std::string& what) {
+    if (!v || v->kind != Json::ARR || v->arr.size() != 3) throw std::runtime_error(what + " must hold 3 numbers");
+    for (int k = 0; k < 3; ++k) {
+        if (v->arr[k].kind != Json::NUM) throw std::runtime_error(what + " must hold 3 numbers");
+        out[k] = v->arr[k].num;
+    }
 }
 
 inline void write_substrate_json(const std::string& path, const Lattice& lat, const std::vector<Prim>& prims,
@@ -332,31 +363,38 @@ inline void read_substrate_json(const std::string& path, Lattice& lat, std::vect
     const Json root = JsonParser(text).parse();
     const Json* fmt = root.get("format");
     if (!fmt || fmt->str != "axonexa-substrate") throw std::runtime_error(path + ": not an axonexa substrate file");
-    const Json* cell = root.get("cell");
-    if (!cell || cell->arr.size() != 3) throw std::runtime_error(path + ": cell must have 3 lengths");
+    double cell[3];
+    json_vec3(root.get("cell"), cell, path + ": cell");
     const Json* L = root.get("lattice");
     const double inf = std::numeric_limits<double>::infinity();
-    lat = Lattice(cell->arr[0].num, cell->arr[1].num, cell->arr[2].num, L ? json_real(L->get("T2"), inf) : inf,
+    lat = Lattice(cell[0], cell[1], cell[2], L ? json_real(L->get("T2"), inf) : inf,
                   L ? json_real(L->get("T1"), 0.0) : 0.0, L ? json_real(L->get("D"), 0.0) : 0.0, 0);
     prims.clear();
     const Json* P = root.get("primitives");
+    if (P && P->kind != Json::ARR) throw std::runtime_error(path + ": primitives must be an array");
     if (P)
         for (const Json& e : P->arr) {
             Prim p;
             const Json* t = e.get("type");
             const std::string ty = t ? t->str : "cylinder_xy";
             p.type = ty == "sphere" ? AXX_PRIM_SPHERE : ty == "plane" ? AXX_PRIM_PLANE : AXX_PRIM_CYLINDER_XY;
-            const Json* c = e.get("center");
-            if (!c || c->arr.size() != 3) throw std::runtime_error(path + ": primitive without a 3-d center");
-            p.cx = c->arr[0].num; p.cy = c->arr[1].num; p.cz = c->arr[2].num;
+            double c[3];
+            json_vec3(e.get("center"), c, path + ": primitive center");
+            p.cx = c[0]; p.cy = c[1]; p.cz = c[2];
             p.R = json_real(e.get("radius"), 0.0);
             p.D = json_real(e.get("D"), 0.0);
             p.T2 = json_real(e.get("T2"), inf);
             p.T1 = json_real(e.get("T1"), 0.0);
             p.kappa = json_real(e.get("kappa"), 0.0);
-            p.region = (int)json_real(e.get("region"), 0.0);
-            const Json* nrm = e.get("normal");
-            if (nrm && nrm->arr.size() == 3) { p.nx = nrm->arr[0].num; p.ny = nrm->arr[1].num; p.nz = nrm->arr[2].num; }
+            const double region = json_real(e.get("region"), 0.0);
+            if (!(region >= 0 && region <= 1e9) || region != std::floor(region))
+                throw std::runtime_error(path + ": region must be a non-negative integer");
+            p.region = (int)region;
+            if (const Json* nrm = e.get("normal")) {
+                double n3[3];
+                json_vec3(nrm, n3, path + ": primitive normal");
+                p.nx = n3[0]; p.ny = n3[1]; p.nz = n3[2];
+            }
             prims.push_back(p);
         }
     lat.setBasisSize((int)prims.size());

@@ -24,10 +24,15 @@ def build() -> str:
 
 
 def lib():
+    """The oracle library: ``oracle/liboracle.so`` (built on first use), or
+    the build that the environment variable AXX_ORACLE_LIB names (the
+    sanitizer build of ``make -C oracle sanitize``)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.environ.get("AXX_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
+            if os.environ.get("AXX_ORACLE_LIB"):
+                raise FileNotFoundError(path)
             build()
         _LIB = _configure(C.CDLL(path))
     return _LIB
