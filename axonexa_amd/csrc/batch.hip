@@ -117,10 +117,13 @@ __global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Instances of profile width 16 or 32 (classes 12-19): the same persistent
-// tile loop with a wave-uniform class switch.  512-thread bounds leave each
-// lane up to 256 VGPRs, so the 48 or 96 moments stay in registers.
+// tile loop with a wave-uniform class switch.  The register budget of three
+// waves per SIMD (168 VGPRs) keeps the 48 or 96 moments in registers with
+// 128 B of per-lane spills; on the batched uneven workload it is 7 % faster
+// than the compiler's own 196 VGPRs at two waves.
 template <bool ALT>
-__global__ __launch_bounds__(kBatchBlockMax) void batch_wide_kernel(BatchWalkArgs B) {
+__global__ __launch_bounds__(kBatchBlockMax) __attribute__((amdgpu_waves_per_eu(3))) void batch_wide_kernel(
+    BatchWalkArgs B) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
     int staged = -1;
