@@ -249,7 +249,8 @@ int  axx_plan_create(axx_ctx* ctx, const axx_substrate* s, const axx_acq* acq,
  * threads and numOfSM from the driver, phaseAcquisitionStream.cuh:127-129).
  * Zero fields select the defaults. */
 typedef struct axx_tuning {
-    int    block;       /* threads per block: multiple of 64 in [64, 1024]; 0 = 1024 */
+    int    block;       /* threads per block: multiple of 64 in [64, 1024]; 0 = the
+                         * size (256, 512 or 1024) of highest occupancy */
     double bin_width;   /* cell-bin edge length; 0 = automatic               */
     int    lds_budget;  /* LDS bytes the substrate tables may use; 0 = 150 KiB */
     int    whole_cell;  /* 1: walk in the substrate's cell as given; 0: a substrate
