@@ -239,3 +239,38 @@ def test_batch_with_mixed_step_generators():
             assert np.array_equal(b.get_state(i), po.walk_mag(s_, f, run)), i
             assert np.array_equal(b.results(i).mx, eng.run_mag(s_, f, run).mx), i
     print("[mag] batch with mixed step generators: bitwise", flush=True)
+
+
+@pytest.mark.parametrize("integ", [abi.MAG_ROTATE, abi.MAG_RK4, abi.MAG_DOPRI])
+@pytest.mark.parametrize("cls", [0, 2, 3])
+def test_uniform_batch_runs_its_class_kernel(integ, cls):
+    """A magnetization batch whose runs share one class (bit 0: spheres or
+    planes, bit 1: rejection) and take isotropic steps runs the kernel
+    compiled for that class alone; classes 0 (cylinders), 2 (cylinders,
+    rejection) and 3 (spheres, rejection) with two fields per run, every
+    walker bitwise its oracle walk."""
+    eng = _engine()
+    R = 0.0025
+    if cls & 1:
+        L = 6 * R
+        sub = sb.single_sphere(L, R, D_in=2.5e-6, D_ex=2.5e-6)
+        origin, init = (L / 2,) * 3, abi.INIT_INSIDE_PRIM
+    else:
+        sub = sb.hex_lattice(R, 0.006, 2, 1, D_in=1e-6, D_ex=2.5e-6, kappa=0.0 if cls & 2 else 1e-2)
+        origin, init = (sub.a / 2, sub.b / 2, 0.0), abi.INIT_UNIFORM
+    boundary = abi.BOUNDARY_REJECTION if cls & 2 else (abi.BOUNDARY_SPECULAR if cls & 1 else abi.BOUNDARY_PERMEABLE)
+    items = []
+    for k in range(2):
+        d = 0.003 * (k + 1)
+        n = int((2 * d + 0.002) / 1e-4)
+        f = fl.build_field([fl.QSC_GRE(5e-4, 0.0, d, 0.002), fl.QSC_GRE(5e-4, 4e-5 * (k + 1), d, 0.002)], n, 1e-4,
+                           integrator=integ, origin=origin,
+                           substeps=None if integ == abi.MAG_ROTATE else np.full(n, 2, np.int32))
+        run = abi.RunSpec(n_walkers=800 + 500 * k, seed=90 + k, instance=k, init_mode=init, boundary_mode=boundary)
+        items.append((sub, f, run))
+    print(f"[mag] uniform batch class {cls}, integrator {integ}", flush=True)
+    with eng.MagBatch(items) as b:
+        b.run()
+        for i, (s_, f, run) in enumerate(items):
+            assert np.array_equal(b.get_state(i), po.walk_mag(s_, f, run)), (cls, integ, i)
+    print(f"[mag] uniform batch class {cls}, integrator {integ}: bitwise", flush=True)
