@@ -11,12 +11,19 @@ plus the fp64 signal reduction, inputs resident on the device.
 
 --workload c3_gamma measures the gamma-packing workload (400 cylinders,
 4e6 walkers x 80,000 steps, 64 profiles x 3 axes); c1_free / c1_cylinder the
-small PGSE workloads.
+small PGSE workloads; c4_batch the batched uneven workload; c5_surrogate the
+learned surrogate (data, training, held-out error, speed); mag_lf /
+mag_stream the magnetization path.
 
-Prints one JSON line.
+Prints one JSON line.  The default run (hexagonal workload) adds a
+"secondary" object after the primary measurement: one or two timed runs of
+each other workload in the same process (about 30 s; --no-secondary skips
+it).
 """
 import argparse
+import contextlib
 import dataclasses
+import io
 import json
 import math
 import os
@@ -143,6 +150,8 @@ def main(argv=None) -> int:
                     help="walkers in the CPU-baseline sample (full step count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: plan default)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="c2_hex: skip the short measurements of the other workloads")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -161,6 +170,90 @@ def main(argv=None) -> int:
     if args.workload == "mag_stream":
         return bench_mag_stream(args)
 
+    out = bench_single(args)
+    if args.workload == "c2_hex" and not args.no_secondary:
+        out["secondary"] = _secondary(args)
+    print(json.dumps(out))
+    return 0
+
+
+def _quiet(fn, args) -> dict:
+    """Run one of the bench_* functions that print a JSON line and return
+    that line as a dict instead."""
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        rc = fn(args)
+    lines = [l for l in buf.getvalue().splitlines() if l.startswith("{")]
+    if rc or not lines:
+        raise RuntimeError(f"{fn.__name__} returned {rc}")
+    return json.loads(lines[-1])
+
+
+def _surrogate_forward(batch: int = 65536) -> dict:
+    """The surrogate network's forward pass at `batch` queries with random
+    weights (its speed does not depend on the weights), median of 3 timed
+    passes with HIP events."""
+    import numpy as np
+
+    from axonexa_amd import engine
+
+    dims = [268] + [1024] * 6 + [30]
+    rng = np.random.default_rng(0)
+    with engine.MLP(dims, max_batch=batch) as m:
+        m.set_params(np.concatenate([np.concatenate([rng.uniform(-1, 1, a * b) / np.sqrt(a), np.zeros(b)])
+                                     for a, b in zip(dims[:-1], dims[1:])]).astype(np.float32))
+        m.forward(rng.uniform(-1, 1, size=(batch, 268)).astype(np.float32))
+        ms = float(np.median([m.time_forward(batch) for _ in range(3)]))
+    flops = 2.0 * batch * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"value": batch / (ms * 1e-3), "unit": "queries/s", "ms_per_step": ms,
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0},
+            "config": f"MLP 268 -> 1024 x 6 -> 30, BF16, batch {batch}, forward only, random weights "
+                      "(training and held-out error: bench.py --workload c5_surrogate)"}
+
+
+def _secondary(args) -> dict:
+    """Short measurements of the other workloads in the same process, after
+    the primary figure: one or two timed runs each, no CPU leg.  Each entry
+    keeps the workload's own metric; the full runs with their CPU baselines
+    are `bench.py --workload <name>`."""
+    keep = ("value", "unit", "ms_per_step", "steps", "warmup")
+    out = {}
+
+    def entry(d):
+        e = {k: d[k] for k in keep if k in d}
+        r = d.get("roofline") or {}
+        e["roofline_frac"] = r.get("frac")
+        e["roofline_bound"] = r.get("bound")
+        e["config"] = (d.get("config") or {}).get("workload")
+        return e
+
+    def ns(**kw):
+        d = dict(vars(args))
+        d.update(no_cpu=True, no_unbatched=True, walkers=0, n_steps=0, block=0, no_secondary=True)
+        d.update(kw)
+        return argparse.Namespace(**d)
+
+    runs = [
+        ("c3_gamma", bench_single, ns(workload="c3_gamma", steps=1, warmup=0)),
+        ("c1_free", bench_single, ns(workload="c1_free", steps=2, warmup=1)),
+        ("c1_cylinder", bench_single, ns(workload="c1_cylinder", steps=2, warmup=1)),
+        ("c4_batch", bench_batch, ns(workload="c4_batch", steps=1, warmup=0)),
+        ("mag_lf", bench_mag, ns(workload="mag_lf", steps=1, warmup=1)),
+    ]
+    for name, fn, a in runs:
+        t0 = time.perf_counter()
+        d = fn(a) if fn is bench_single else _quiet(fn, a)
+        out[name] = dict(entry(d), seconds=time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    out["c5_surrogate_forward"] = dict(_surrogate_forward(), seconds=time.perf_counter() - t0)
+    return out
+
+
+def bench_single(args) -> dict:
+    """One plan of a single-substrate workload (hexagonal lattice, gamma
+    packing, free box, single cylinder): warmup walks, then `steps` timed
+    walks + reductions, with the CPU oracle beside it."""
     import numpy as np
 
     from axonexa_amd import abi, engine
@@ -305,9 +398,8 @@ def main(argv=None) -> int:
             "transmissions_equal": bool(gsig.transmissions == osig.transmissions),
             "max_abs_dE_over_sigma": float(np.max(dE)),
         }
-    print(json.dumps(out))
     plan.close()
-    return 0
+    return out
 
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64 (spec sheet)
