@@ -96,6 +96,7 @@ struct HostPrep {
     std::vector<float> L, rate, thr_out, thr_in;
     std::vector<float4> cyl;
     std::vector<float> inv;      // 1 / R per image (float division), 0 for planes
+    std::vector<float2> thr;     // per image: (thr_out, thr_in) of its primitive, 0 for planes
     std::vector<uint4> bins;
     std::vector<uint16_t> cand;
     int nb[3] = {1, 1, 1};
@@ -337,8 +338,11 @@ static bool surface_near_box(const float4& g, int meta, const double lo[3], cons
     return std::sqrt(dmin2) <= R + reach && std::sqrt(dmax2) >= R - reach;
 }
 
+// LDS tables: bins (16 B), images (16 B), per-image transmission
+// thresholds (8 B), meta words (4 B, general geometry), inverse radii (4 B),
+// candidates (2 B).
 static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
-    return n_bins * 16 + n_img * (gen ? 24 : 20) + n_cand * 2;
+    return n_bins * 16 + n_img * (gen ? 32 : 28) + n_cand * 2;
 }
 
 // Images, compartments and bins.  The float32 image centres are
@@ -433,6 +437,9 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         H.cyl.push_back(make_float4(H.geom[i].x, H.geom[i].y, H.geom[i].w, mf));
         const float Rf = H.geom[i].w;
         H.inv.push_back((H.meta[i] & kPlaneBit) ? 0.0f : 1.0f / Rf);
+        const int ci = H.meta[i] & kCompMask;
+        H.thr.push_back((H.meta[i] & kPlaneBit) || ci < 1 ? make_float2(0.0f, 0.0f)
+                                                           : make_float2(H.thr_out[ci - 1], H.thr_in[ci - 1]));
     }
     // bin width: at least 4 step lengths; widen until the tables fit in LDS.
     // Each candidate carries a 4 x 4 mask of the sub-bins (xy) whose
@@ -621,8 +628,8 @@ void plan_free(axx_plan* p) {
         delete p;
         return;
     }
-    void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_bins, p->d_cand, p->d_L, p->d_rate,
-                    p->d_thr_out, p->d_thr_in, p->d_prof, p->d_state, p->d_err, p->d_wfp,
+    void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_thr, p->d_bins, p->d_cand, p->d_L, p->d_rate,
+                    p->d_prof, p->d_state, p->d_err, p->d_wfp,
                     p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -741,12 +748,11 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if ((rc = dupload(ctx, ar, &p->d_meta, H.meta.data(), H.meta.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_cyl, H.cyl.data(), H.cyl.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_inv, H.inv.data(), H.inv.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_thr, H.thr.data(), H.thr.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_bins, H.bins.data(), H.bins.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_cand, H.cand.data(), H.cand.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_L, H.L.data(), H.L.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_rate, H.rate.data(), H.rate.size()))) return rc;
-    if ((rc = dupload(ctx, ar, &p->d_thr_out, H.thr_out.data(), H.thr_out.size()))) return rc;
-    if ((rc = dupload(ctx, ar, &p->d_thr_in, H.thr_in.data(), H.thr_in.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_prof, prof.data(), prof.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_wfp, p->wf_p.data(), p->wf_p.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_wfdir, p->wf_dir.data(), p->wf_dir.size()))) return rc;
@@ -818,7 +824,8 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     A.n_img = p->n_img; A.n_bins = p->n_bins; A.n_cand = p->n_cand;
     A.g_geom = p->d_geom; A.g_meta = p->d_meta; A.g_cyl = p->d_cyl; A.g_bins = p->d_bins; A.g_cand = p->d_cand;
     A.g_inv = p->d_inv;
-    A.compL = p->d_L; A.rate = p->d_rate; A.thr_out = p->d_thr_out; A.thr_in = p->d_thr_in;
+    A.g_thr = p->d_thr;
+    A.compL = p->d_L; A.rate = p->d_rate;
     A.prof = p->d_prof;
     A.state = p->d_state;
     A.N = p->N;

@@ -59,12 +59,11 @@ struct axx_plan {
     int* d_meta = nullptr;
     float4* d_cyl = nullptr;
     float* d_inv = nullptr;
+    float2* d_thr = nullptr;
     uint4* d_bins = nullptr;
     uint16_t* d_cand = nullptr;
     float* d_L = nullptr;
     float* d_rate = nullptr;
-    float* d_thr_out = nullptr;
-    float* d_thr_in = nullptr;
     float* d_prof = nullptr;
     uint32_t* d_state = nullptr;
     int* d_err = nullptr;

@@ -35,13 +35,13 @@ struct WalkArgs {
     const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit | kPlaneBit
     const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
     const float* __restrict__ g_inv;        // [n_img] 1 / R (float division; 0 for planes)
+    const float2* __restrict__ g_thr;       // [n_img] 4 kappa / v_inside, 4 kappa / v_lattice of the
+                                            // image's primitive (0 for planes)
     const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
     const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
     const float* __restrict__ prof;         // [n_steps][Pt] unit profile samples
     const float* __restrict__ compL;        // [n_comp] step length
     const float* __restrict__ rate;         // [n_comp] dt / T2
-    const float* __restrict__ thr_out;      // [n_prims] 4 kappa / v_inside
-    const float* __restrict__ thr_in;       // [n_prims] 4 kappa / v_lattice
     uint32_t* __restrict__ state;           // SoA, AXX_STATE_FIELDS(P) x N
     long long N;
     uint32_t k0, k1, inst, woff;
@@ -116,10 +116,13 @@ __device__ __forceinline__ bool plane_hit(float4 g, float x0, float y0, float z0
 
 // Substrate tables in LDS: bins (16 B each), then one float4 per image
 // (cylinder-only: cx, cy, R, meta; otherwise cx, cy, cz, R plus a meta
-// array), the inverse radii, then the u16 candidate lists.
+// array), the transmission thresholds of each image's primitive (float2:
+// from inside, from outside), the inverse radii, then the u16 candidate
+// lists.
 struct LdsTables {
     uint4* bins;
     float4* geom;
+    float2* thr;
     int* meta;
     float* inv;
     uint16_t* cand;
@@ -130,7 +133,8 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
     LdsTables T;
     T.bins = reinterpret_cast<uint4*>(lds);
     T.geom = reinterpret_cast<float4*>(T.bins + A.n_bins);
-    T.meta = reinterpret_cast<int*>(T.geom + A.n_img);
+    T.thr = reinterpret_cast<float2*>(T.geom + A.n_img);
+    T.meta = reinterpret_cast<int*>(T.thr + A.n_img);
     T.inv = GEN ? reinterpret_cast<float*>(T.meta + A.n_img) : reinterpret_cast<float*>(T.meta);
     T.cand = reinterpret_cast<uint16_t*>(T.inv + A.n_img);
     for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
@@ -142,6 +146,7 @@ __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned ch
             T.geom[i] = A.g_cyl[i];
         }
         T.inv[i] = A.g_inv[i];
+        T.thr[i] = A.g_thr[i];
     }
     for (int i = threadIdx.x; i < A.n_cand; i += blockDim.x) T.cand[i] = A.g_cand[i];
     __syncthreads();
@@ -415,7 +420,8 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     }
                     const uint32_t sel = n & 3u;
                     const uint32_t wv = sel == 0 ? cw.x : sel == 1 ? cw.y : sel == 2 ? cw.z : cw.w;
-                    const float thr = ins ? A.thr_out[pcomp - 1] : A.thr_in[pcomp - 1];
+                    const float2 th = T.thr[bk];   // this surface's (thr_out, thr_in), from LDS
+                    const float thr = ins ? th.x : th.y;
                     transmit = unif(wv) <= thr;
                 }
                 lastpl = pl ? bk : -1;
