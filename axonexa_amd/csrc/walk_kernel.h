@@ -562,8 +562,12 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     // Re-derive the state addresses after the loop (the empty asm hides
     // their equality with the pre-loop ones, so no 64-bit address stays live
-    // across the step loop).
-    float* fso = fs;
+    // across the step loop).  The base is wave-uniform (one instance per
+    // wave); readfirstlane puts it in SGPRs whatever the compiler holds it in.
+    const uint64_t fsu = reinterpret_cast<uint64_t>(fs);
+    float* fso = reinterpret_cast<float*>(
+        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(fsu >> 32)) << 32) |
+        (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)fsu));
     uint32_t wo2 = wo;
     asm volatile("" : "+s"(fso));
     asm volatile("" : "+v"(wo2));
