@@ -630,7 +630,7 @@ void plan_free(axx_plan* p) {
     }
     void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_thr, p->d_bins, p->d_cand, p->d_L, p->d_rate,
                     p->d_prof, p->d_state, p->d_err, p->d_wfp,
-                    p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt};
+                    p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt, p->d_queue};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -673,7 +673,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     HostPrep H;
     size_t budget = 150 * 1024;
     if (tune && tune->lds_budget > 0) budget = (size_t)tune->lds_budget;
-    budget = std::min<size_t>(budget, ctx->lds_max);
+    budget = std::min<size_t>(budget, ctx->lds_max - kWalkStaticLds);   // the walk kernel's own LDS words
     if (tune && tune->block && (tune->block < 64 || tune->block > 1024 || tune->block % 64))
         return fail(ctx, AXX_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
     if (tune && tune->bin_width < 0) return fail(ctx, AXX_ERR_INVALID, "bin_width < 0");
@@ -772,6 +772,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if ((rc = dupload<double>(ctx, ar, &p->d_partial, nullptr, (size_t)p->M * p->n_chunks * kRedVals))) return rc;
     if ((rc = dupload<double>(ctx, ar, &p->d_out, nullptr, (size_t)p->M * kRedVals))) return rc;
     if ((rc = dupload<unsigned long long>(ctx, ar, &p->d_cnt, nullptr, 4))) return rc;
+    if ((rc = dupload<unsigned int>(ctx, ar, &p->d_queue, nullptr, 1 + (size_t)((p->N + 63) / 64)))) return rc;
     if (!ar) {
         HIP_TRY(ctx, hipEventCreate(&p->ev0));
         HIP_TRY(ctx, hipEventCreate(&p->ev1));
@@ -840,6 +841,28 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     return A;
 }
 
+// Automatic task length: a sixteenth of the walk within [512, 4096] steps.
+// The hexagonal workload (1e6 walkers, 250,000 steps) runs 2.662e11,
+// 2.667e11, 2.668e11 and 2.660e11 walker-steps/s at 1024, 2048, 4096 and
+// 6144 steps per task, 2.642e11 at 16384, 2.375e11 with whole walks; the
+// launch ends within one task of the last group's last chunk, so a sixteenth
+// bounds that tail for short walks.
+constexpr int kAutoChunks = 16;
+constexpr int kMinTaskSteps = 512;
+constexpr int kMaxTaskSteps = 4096;
+constexpr uint32_t kWaitTicks = 2000000000u; // 20 s of 100 MHz real time: longest wait of one task
+
+// Steps per walk task.  A task is one group of 64 walkers over one time
+// chunk; chunks pay off once the groups fill the resident waves at least
+// twice: then every wave stays busy until the last chunk of the last group,
+// instead of the launch ending with a partly filled round of whole walks.
+static int task_steps(const axx_plan* p, long long groups, long long resident_waves, int steps) {
+    if (p->sched > 0) return std::min(steps, (p->sched + 1) & ~1);
+    if (p->sched < 0 || groups < 2 * resident_waves) return steps;
+    const int target = (steps + kAutoChunks - 1) / kAutoChunks;
+    return std::min(steps, std::min(kMaxTaskSteps, std::max(kMinTaskSteps, (target + 1) & ~1)));
+}
+
 static int launch_walk(axx_plan* p, int i0, int i1) {
     axx_ctx* ctx = p->ctx;
     WalkArgs A = plan_walk_args(p, i0, i1);
@@ -855,9 +878,35 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
         A.i0 = 1;
     }
     if (A.i0 < A.i1) {
-        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), p->lds_bytes, ctx->stream, A,
+        int per_cu = 0;
+        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, block, p->lds_bytes));
+        if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "walk kernel does not fit a CU at %d threads", block);
+        const long long wpb = block / 64;
+        const long long resident = (long long)per_cu * ctx->n_cu;
+        const long long groups = (p->N + 63) / 64;
+        const int steps = A.i1 - A.i0;
+        int chunk = task_steps(p, groups, resident * wpb, steps);
+        long long chunks = (steps + chunk - 1) / chunk;
+        if (groups * chunks >= 0x7fffffffll) {   // one chunk per walk keeps the task count below 2^31
+            chunk = steps;
+            chunks = 1;
+        }
+        WalkQueue Q{};
+        Q.q = p->d_queue;
+        Q.n_groups = (uint32_t)groups;
+        Q.n_tasks = (uint32_t)(groups * chunks);
+        Q.chunk = chunk;
+        Q.wait_ticks = kWaitTicks;
+        const unsigned blocks = (unsigned)std::min(resident, (groups + wpb - 1) / wpb);
+        const size_t qwords = chunks > 1 ? (size_t)groups + 1 : 1;
+        hipLaunchKernelGGL(zero_words, dim3((unsigned)std::min<size_t>((qwords + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, (uint32_t*)p->d_queue, qwords);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(block), p->lds_bytes, ctx->stream, A, Q,
                            (const float*)p->d_prof);
         HIP_TRY(ctx, hipGetLastError());
+        p->last_task_steps = chunk;
+        p->last_grid = (int)blocks;
     }
     return AXX_OK;
 }
@@ -959,6 +1008,16 @@ extern "C" int axx_plan_sync(axx_plan* p) {
     }
 }
 
+// Error bits the kernels leave in a plan's error word: 1 = a placement by
+// rejection did not finish, kErrWaitLimit = a walk task gave up waiting for
+// its group's previous chunk.
+static int walk_error(axx_ctx* ctx, int err) {
+    if (err & kErrWaitLimit)
+        return fail(ctx, AXX_ERR_DEVICE, "a walk task waited more than %.0f s for its walker group's previous chunk",
+                    kWaitTicks * 1e-8);
+    return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+}
+
 extern "C" int axx_plan_results(axx_plan* p, axx_result* out) {
     try {
         if (!p || !out) return AXX_ERR_INVALID;
@@ -972,7 +1031,7 @@ extern "C" int axx_plan_results(axx_plan* p, axx_result* out) {
         HIP_TRY(ctx, hipMemcpyAsync(cnt, p->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+        if (err) return walk_error(ctx, err);
         for (int j = 0; j < p->M; ++j) {
             const double* v = &h[(size_t)j * kRedVals];
             if (out->sum_cos) out->sum_cos[j] = v[0];
@@ -1005,7 +1064,7 @@ extern "C" int axx_plan_get_state(axx_plan* p, uint32_t* host) {
         HIP_TRY(ctx, hipMemcpy(host, p->d_state, nf * (size_t)p->N * sizeof(uint32_t), hipMemcpyDeviceToHost));
         int err = 0;
         HIP_TRY(ctx, hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost));
-        if (err) return fail(ctx, AXX_ERR_INIT, "rejection initialisation did not finish for some walkers");
+        if (err) return walk_error(ctx, err);
         return AXX_OK;
     } catch (...) {
         return api_exception();
@@ -1071,6 +1130,29 @@ extern "C" int axx_plan_set_config(axx_plan* p, int block, int wpt) {
         if (wpt != 1) return fail(p->ctx, AXX_ERR_INVALID, "walkers per thread must be 1");
         p->block = block;
         p->wpt = wpt;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
+}
+
+extern "C" int axx_plan_set_schedule(axx_plan* p, int task_steps) {
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        if (task_steps < -1) return fail(p->ctx, AXX_ERR_INVALID, "task_steps must be -1, 0 or positive");
+        p->sched = task_steps;
+        return AXX_OK;
+    } catch (...) {
+        return api_exception();
+    }
+}
+
+extern "C" int axx_plan_get_schedule(const axx_plan* p, int* task_steps, int* last_task_steps, int* last_grid) {
+    try {
+        if (!p) return AXX_ERR_INVALID;
+        if (task_steps) *task_steps = p->sched;
+        if (last_task_steps) *last_task_steps = p->last_task_steps;
+        if (last_grid) *last_grid = p->last_grid;
         return AXX_OK;
     } catch (...) {
         return api_exception();

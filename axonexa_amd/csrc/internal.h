@@ -75,6 +75,10 @@ struct axx_plan {
     double* d_out = nullptr;
     unsigned long long* d_cnt = nullptr;
     int n_chunks = 0, chunk = 0;
+    unsigned int* d_queue = nullptr;   // walk task counter + chunks finished per group of 64 walkers
+    int sched = 0;                     // steps per walk task: 0 automatic, -1 the whole walk, > 0 that many
+    int last_task_steps = 0;           // steps per task of the last walk launch
+    int last_grid = 0;                 // workgroups of the last walk launch
     float cell[3] = {0, 0, 0};      // cell the positions wrap in (unit cell of a repeating substrate)
     float whole[2] = {0, 0};        // the substrate's cell along x and y (placement)
     int units = 1;                  // unit cells in the substrate's cell

@@ -169,7 +169,23 @@ __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
 // is wave-uniform, so these become scalar loads (s_load) into SGPRs.
 typedef const __attribute__((address_space(4))) float* const_fptr;
 
-typedef void (*walk_fn)(WalkArgs, const float*);
+// Work of one walk launch as tasks t = k * n_groups + g: group g is walkers
+// [64 g, 64 g + 64), time chunk k is steps [i0 + k chunk, i0 + (k + 1) chunk)
+// clipped to i1.  Waves take tasks from a counter in that order (every
+// group's chunk k before any group's chunk k + 1), so all groups advance
+// together and the launch ends one chunk after the counter runs out, whatever
+// the walker count; a wave that takes chunk k > 0 of a group first waits for
+// that group's chunk k - 1 (taken by some wave before it) to be published.
+struct WalkQueue {
+    unsigned int* __restrict__ q;   // [0] task counter, [1 + g] chunks of group g finished;
+                                    // zeroed before each launch
+    uint32_t n_groups, n_tasks;
+    int chunk;                      // steps per task (even: step pairs are never split)
+    uint32_t wait_ticks;            // limit of one wait, in 100 MHz real-time ticks
+};
+constexpr int kErrWaitLimit = 2;    // walk error bit: a task gave up waiting for its group
+
+typedef void (*walk_fn)(WalkArgs, WalkQueue, const float*);
 walk_fn pick_walk_p1(int ax, bool gen, bool rej, bool alt);
 walk_fn pick_walk_p2(int ax, bool gen, bool rej, bool alt);
 walk_fn pick_walk_p4(int ax, bool gen, bool rej, bool alt);
@@ -591,16 +607,88 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 // 2.28e11 walker-steps/s).
 constexpr int walk_waves(int P, int AX, bool ALT) { return (P == 8 && AX <= 5 && !ALT) ? 6 : 1; }
 
+// Wait until chunk k - 1 of group g is published (lane 0 polls; bounded by
+// Q.wait_ticks of real time), then acquire: the group's state written by the
+// wave that walked chunk k - 1, on any XCD, is visible to the loads that follow.
+__device__ __forceinline__ bool wait_group(const WalkQueue& Q, uint32_t g, uint32_t k, uint32_t lane) {
+    uint32_t ok = 1u;
+    if (lane == 0u) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(Q.q + 1 + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+            __builtin_amdgcn_s_sleep(4);
+            if (__builtin_amdgcn_s_memrealtime() - t0 > (uint64_t)Q.wait_ticks) {
+                ok = 0u;
+                break;
+            }
+        }
+    }
+    ok = (uint32_t)__builtin_amdgcn_readfirstlane((int)ok);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok != 0u;
+}
+
+// Persistent walk: the grid is at most what stays resident; the substrate
+// tables are staged once per workgroup, then each wave takes tasks (group,
+// chunk) until the counter passes Q.n_tasks.  With one chunk per walk there
+// are no waits and nothing to publish.  The queue fields and each wave's
+// task sit in LDS and are read back around every task (the empty asm with a
+// memory clobber forces the reads), so none of them holds an SGPR through
+// the step loop, whose SGPR budget is already full.
+constexpr int kWalkStaticLds = 64 + 4 * 16;   // static LDS of walk_kernel (bytes)
+
 template <int P, int AX, bool GEN, bool REJ, bool ALT>
 __global__ __launch_bounds__(walk_max_block(P)) __attribute__((amdgpu_waves_per_eu(walk_waves(P, AX, ALT)))) void
-walk_kernel(WalkArgs A, const float* __restrict__ prof_g) {
+walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const LdsTables T = stage_tables<GEN>(A, lds);
-    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= A.N) return;
-    PhaseAcc<P, AX> acc;
-    acc.prof = (const_fptr)prof_g;
-    walk_lane<PhaseAcc<P, AX>, GEN, REJ, ALT>(A, T, acc, (uint32_t)w);
+    __shared__ WalkQueue s_q;
+    __shared__ int s_range[2];
+    __shared__ uint32_t s_task[16];
+    static_assert(sizeof(WalkQueue) + sizeof(s_range) <= 64, "static LDS of walk_kernel");
+    if (threadIdx.x == 0) {
+        s_q = Q;
+        s_range[0] = A.i0;
+        s_range[1] = A.i1;
+    }
+    const LdsTables T = stage_tables<GEN>(A, lds);   // ends with a barrier
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = threadIdx.x >> 6;
+    for (;;) {
+        asm volatile("" ::: "memory");
+        const uint32_t n_groups = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_groups);
+        const uint32_t n_tasks = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_tasks);
+        uint32_t t = 0u;
+        if (lane == 0u) t = atomicAdd(s_q.q, 1u);
+        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+        if (t >= n_tasks) break;
+        const uint32_t k = t / n_groups;
+        const uint32_t g = t - k * n_groups;
+        if (k > 0u && !wait_group(s_q, g, k, lane)) {
+            if (lane == 0u) atomicOr(A.err, kErrWaitLimit);
+            break;
+        }
+        if (lane == 0u) s_task[wv] = t;
+        WalkArgs B = A;
+        const int chunk = __builtin_amdgcn_readfirstlane(s_q.chunk);
+        B.i0 = __builtin_amdgcn_readfirstlane(s_range[0]) + (int)k * chunk;
+        B.i1 = min(__builtin_amdgcn_readfirstlane(s_range[1]), B.i0 + chunk);
+        const uint32_t w = g * 64u + lane;
+        if ((long long)w < A.N) {
+            PhaseAcc<P, AX> acc;
+            acc.prof = (const_fptr)prof_g;
+            walk_lane<PhaseAcc<P, AX>, GEN, REJ, ALT>(B, T, acc, w);
+        }
+        asm volatile("" ::: "memory");
+        const uint32_t ng = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_groups);
+        if ((uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_tasks) > ng) {
+            // release: this wave's state stores reach every XCD before the
+            // group's next chunk can be taken up
+            const uint32_t tt = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_task[wv]);
+            const uint32_t kk = tt / ng;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0u)
+                __hip_atomic_store(s_q.q + 1 + (tt - kk * ng), kk + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 }  // namespace axx

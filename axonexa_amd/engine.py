@@ -68,6 +68,8 @@ def lib():
     L.axx_plan_get_config.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.axx_plan_set_config.argtypes = [vp, C.c_int, C.c_int]
+    L.axx_plan_set_schedule.argtypes = [vp, C.c_int]
+    L.axx_plan_get_schedule.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.axx_plan_profile_count.argtypes = [vp]
     L.axx_plan_unit_cells.argtypes = [vp]
     L.axx_plan_destroy.argtypes = [vp]
@@ -223,6 +225,16 @@ class Plan:
 
     def set_config(self, block: int, wpt: int = 1):
         self.ctx.check(lib().axx_plan_set_config(self._h, int(block), int(wpt)), "axx_plan_set_config")
+
+    def set_schedule(self, task_steps: int = 0):
+        """Steps per walk task: 0 automatic, -1 the whole walk, > 0 that many."""
+        self.ctx.check(lib().axx_plan_set_schedule(self._h, int(task_steps)), "axx_plan_set_schedule")
+
+    def schedule(self):
+        s, ls, g = C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(lib().axx_plan_get_schedule(self._h, C.byref(s), C.byref(ls), C.byref(g)),
+                       "axx_plan_get_schedule")
+        return {"task_steps": s.value, "last_task_steps": ls.value, "last_grid": g.value}
 
     def close(self):
         if self._h:

@@ -150,6 +150,8 @@ def main(argv=None) -> int:
                     help="walkers in the CPU-baseline sample (full step count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: plan default)")
+    ap.add_argument("--task-steps", type=int, default=0,
+                    help="steps per walk task (0: automatic, -1: the whole walk per task)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="c2_hex: skip the short measurements of the other workloads")
     args = ap.parse_args(argv)
@@ -269,6 +271,7 @@ def bench_single(args) -> dict:
     wl = make(**kw)
     n = wl.acq.n_steps
     plan = engine.Plan(wl.sub, wl.acq, wl.run, block=args.block, lds_budget=args.lds_budget)
+    plan.set_schedule(args.task_steps)
     cfg = plan.config()
 
     for _ in range(args.warmup):
@@ -317,6 +320,8 @@ def bench_single(args) -> dict:
             "bin_width_um": round(cfg["bin_width"], 4),
             "lds_bytes": cfg["lds_bytes"],
             "unit_cells": cfg.get("unit_cells", 1),
+            "task_steps": plan.schedule()["last_task_steps"],
+            "grid": plan.schedule()["last_grid"],
         },
         "roofline": {
             "bound": "valu",

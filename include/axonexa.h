@@ -277,6 +277,20 @@ int  axx_plan_time_walk(axx_plan* plan, int step_begin, int step_end, float* ms)
 int  axx_plan_get_config(const axx_plan* plan, int* block, int* wpt,
                          double* bin_width, int* lds_bytes);
 int  axx_plan_set_config(axx_plan* plan, int block, int wpt);
+/* Walk schedule (no reference counterpart: the reference launches one thread
+ * per walker for the whole walk, phaseAcquisitionStream.cuh:163-167,
+ * kernelLattice.cuh:12,46).  A walk
+ * launch is a persistent grid whose waves take tasks -- one group of 64
+ * walkers over one time chunk of task_steps steps -- every group's chunk k
+ * before any group's chunk k + 1, so every wave stays busy to the end of the
+ * walk.  task_steps: 0 = automatic (chunks when the groups fill the resident
+ * waves twice or more, one chunk per walk otherwise), -1 = the whole walk as
+ * one chunk, > 0 = that many steps (rounded up to even).  Results are the same
+ * bits for every schedule.  get: the setting, and the steps per task and
+ * workgroups of the last walk launch. */
+int  axx_plan_set_schedule(axx_plan* plan, int task_steps);
+int  axx_plan_get_schedule(const axx_plan* plan, int* task_steps, int* last_task_steps,
+                           int* last_grid);
 int  axx_plan_profile_count(const axx_plan* plan);
 /* Unit cells of the substrate's cell the plan walks in one of (1: the whole
  * cell; see axx_tuning.whole_cell).  No reference counterpart: the reference

@@ -36,10 +36,12 @@ def _pgse(n_steps, dt, Gs=(0.0, 0.1, 0.3), d=None, gap=None, axis=(1, 0, 0)):
     return wf.build_acq([wf.RectGrad(g * 1e-6, d, gap, axis) for g in Gs], n_steps, dt, GAMMA)
 
 
-def _run_both(name, sub, acq, run, chunks=None):
-    print(f"[parity] {name}: N={run.n_walkers} steps={acq.n_steps}", flush=True)
+def _run_both(name, sub, acq, run, chunks=None, schedule=None):
+    print(f"[parity] {name}: N={run.n_walkers} steps={acq.n_steps} schedule={schedule}", flush=True)
     eng = _engine()
     with eng.Plan(sub, acq, run) as plan:
+        if schedule is not None:
+            plan.set_schedule(schedule)
         if chunks:
             prev = 0
             for c in list(chunks) + [acq.n_steps]:
@@ -50,6 +52,8 @@ def _run_both(name, sub, acq, run, chunks=None):
         plan.reduce()
         gsig = plan.results()
         gst = plan.get_state()
+        if schedule is not None and schedule > 0:
+            assert plan.schedule()["last_task_steps"] == min(schedule + (schedule & 1), acq.n_steps - 1)
     ost = po.walk(sub, acq, run)
     osig = po.reduce(sub, acq, run, ost)
     info = compare_states(gst, ost, run.n_walkers, acq.n_profiles)
@@ -229,6 +233,32 @@ def test_dense_permeable_packing_many_events_per_step():
     run = abi.RunSpec(n_walkers=3000, seed=21, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
     sig, _ = _run_both("dense_permeable", sub, acq, run)
     assert sig.collisions > 100000 and sig.transmissions > 50000
+
+
+@pytest.mark.parametrize("task_steps", [2, 16, 64])
+def test_walk_tasks_of_a_few_steps_equal_oracle(task_steps):
+    # a walk launch split into tasks (64 walkers x task_steps steps): with
+    # 47 groups every wave takes each group's next chunk from another wave,
+    # so the publish / wait hand-over runs on nearly every task; many
+    # boundary events per step and transmission draws in blocks of four
+    dt = 2e-2
+    sub = sb.gamma_packing(30, 5.0, 0.4, 0.6, D_in=1.7, D_ex=2.0, seed=7, T2_in=70.0, T2_ex=50.0,
+                           rmin=0.25, kappa=sb.kappa_for_probability(0.6, 1.7, dt))
+    acq = wf.build_acq([wf.RectGrad(g * 1e-6, 3.0, 2.0, (1, 0, 0)) for g in (0.1, 0.3)], 500, dt, GAMMA)
+    run = abi.RunSpec(n_walkers=3000, seed=21, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both(f"tasks_{task_steps}", sub, acq, run, schedule=task_steps)
+    assert sig.collisions > 100000 and sig.transmissions > 50000
+
+
+def test_walk_tasks_inside_chunked_launches():
+    # launches that begin on an even step: every task then starts inside a
+    # step pair and takes the single-step path first
+    sub = sb.single_cylinder(20.0, 4.0, D_in=2.0, D_ex=2.0,
+                             kappa=sb.kappa_for_probability(0.2, 2.0, 1e-2))
+    acq = _pgse(1000, 1e-2)
+    run = abi.RunSpec(n_walkers=2048, seed=9, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    sig, _ = _run_both("tasks_in_chunks", sub, acq, run, chunks=[1, 2, 3, 251, 252, 600], schedule=9)
+    assert sig.transmissions > 1000
 
 
 def test_permeable_sphere_lattice_many_events():

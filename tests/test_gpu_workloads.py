@@ -103,6 +103,35 @@ def test_c2_sample_full_steps_matches_oracle():
     assert np.allclose(gsig.sum_cos, osig.sum_cos, rtol=1e-12, atol=1e-9)
     assert gsig.transmissions == osig.transmissions and gsig.transmissions > 0
     assert gsig.cap_hits == 0 and osig.cap_hits == 0
+    # the same walk as tasks of 8192 steps (31 chunks handed between waves)
+    with eng.Plan(w.sub, w.acq, w.run) as p:
+        p.set_schedule(8192)
+        p.walk()
+        assert p.schedule()["last_task_steps"] == 8192
+        gst2 = p.get_state()
+    assert np.array_equal(gst2, gst)
+
+
+def test_c2_full_size_tasks_equal_whole_walks():
+    # 800,000 walkers fill the resident waves about twice: the automatic
+    # schedule walks them in time chunks handed between waves; every word of
+    # the state equals the run with one task per walker group
+    w = W.c2_hex(n_walkers=800000, n_steps=4001)
+    print("[gpu] c2_hex 800000 walkers x 4001 steps: automatic schedule vs whole walks", flush=True)
+    eng = _engine()
+    states = []
+    for sched in (0, -1):
+        with eng.Plan(w.sub, w.acq, w.run) as p:
+            p.set_schedule(sched)
+            p.walk().reduce()
+            info = p.schedule()
+            print(f"[gpu] schedule {sched}: {info}", flush=True)
+            if sched == 0:
+                assert info["last_task_steps"] < 4000
+            else:
+                assert info["last_task_steps"] == 4000
+            states.append(p.get_state())
+    assert np.array_equal(states[0], states[1])
 
 
 def test_c3_sample_full_steps_matches_oracle():
