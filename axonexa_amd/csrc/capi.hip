@@ -342,7 +342,7 @@ static bool surface_near_box(const float4& g, int meta, const double lo[3], cons
 // thresholds (8 B), meta words (4 B, general geometry), inverse radii (4 B),
 // candidates (2 B).
 static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
-    return n_bins * 16 + n_img * (gen ? 32 : 28) + n_cand * 2;
+    return kAzimuthEntries * sizeof(float2) + n_bins * 16 + n_img * (gen ? 32 : 28) + n_cand * 2;
 }
 
 // Images, compartments and bins.  The float32 image centres are

@@ -53,12 +53,13 @@ __device__ __forceinline__ float unif(uint32_t w) {
     return fmaf((float)(w >> 9), 0x1p-23f, 0x1p-24f);
 }
 
-// sin and cos of 2*pi*u(w): integer quarter-turn reduction, Taylor
-// polynomials of sin(pi/2 r), cos(pi/2 r) for r in [-1/2, 1/2].
-__device__ __forceinline__ void sincos_turn(uint32_t w, float& s, float& c) {
-    // v = u 2^24 + 2^21; quarter turn q = v >> 22 (0..4); r = u 4 - q in
+// sin and cos of 2 pi v 2^-24 for a 24-bit integer v: integer quarter-turn
+// reduction, Taylor polynomials of sin(pi/2 r), cos(pi/2 r) for r in
+// [-1/2, 1/2].  Used to build the azimuth table below.
+__device__ __forceinline__ void sincos_u24(uint32_t v24, float& s, float& c) {
+    // v = v24 + 2^21; quarter turn q = v >> 22 (0..4); r = 4 v24 2^-24 - q in
     // [-1/2, 1/2) from the low 22 bits of v, exact
-    const uint32_t v = ((w >> 8) | 1u) + (1u << 21);
+    const uint32_t v = v24 + (1u << 21);
     const uint32_t q = v >> 22;
     float r = fmaf((float)(v & 0x3FFFFFu), 0x1p-22f, -0.5f);
     float y = r * r;
@@ -77,6 +78,34 @@ __device__ __forceinline__ void sincos_turn(uint32_t w, float& s, float& c) {
     float b0 = (qq & 1u) ? sp : cp;   // cos magnitude source
     s = (qq >= 2u) ? -a0 : a0;
     c = (qq == 1u || qq == 2u) ? -b0 : b0;
+}
+
+// Azimuth table: entry j = (cos, sin) of 2 pi (j + 1/2) / 256, i.e.
+// sincos_u24((2j + 1) << 15), j = 0..255 (2 KB, built in LDS by each
+// workgroup).
+constexpr int kAzimuthEntries = 256;
+__device__ __forceinline__ float2 azimuth_entry(int j) {
+    float s, c;
+    sincos_u24((2u * (uint32_t)j + 1u) << 15, s, c);
+    return make_float2(c, s);
+}
+
+// sin and cos of 2 pi u(w), u(w) = ((w >> 8) | 1) 2^-24: the table entry of
+// the top 8 bits of the 24-bit value v, turned by d = 2 pi (v 2^-24 - (j +
+// 1/2)/256) = (f - 2^15) K1 with f the low 16 bits of v and K1 = the float
+// of 2 pi times 2^-24 (|d| <= pi/256): cos d = 1 - d^2/2, sin d = d (1 -
+// d^2/6) (truncation below 1e-9), then the angle-sum formulas.  Replaces
+// phi = 2 PI curand_uniform; sin(phi), cos(phi) (kernelLattice.cuh:52-57).
+__device__ __forceinline__ void sincos_tab(const float2* tab, uint32_t w, float& s, float& c) {
+    const uint32_t v = (w >> 8) | 1u;
+    const float2 e = tab[v >> 16];
+    // (f - 2^15) K1 in one rounding: 2^15 K1 is exact
+    const float d = fmaf((float)(v & 0xFFFFu), 0x1.921fb6p-22f, -0x1.921fb6p-7f);
+    const float d2 = d * d;
+    const float cd = fmaf(d2, -0.5f, 1.0f);
+    const float sd = d * fmaf(d2, -0.16666667f, 1.0f);
+    c = fmaf(-e.y, sd, e.x * cd);
+    s = fmaf(e.x, sd, e.y * cd);
 }
 
 // -2 ln u(w) for the Box-Muller transform of Gaussian steps: u = k 2^-24

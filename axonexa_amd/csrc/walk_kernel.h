@@ -114,12 +114,13 @@ __device__ __forceinline__ bool plane_hit(float4 g, float x0, float y0, float z0
     return true;
 }
 
-// Substrate tables in LDS: bins (16 B each), then one float4 per image
+// Substrate tables in LDS: the azimuth table (2 KB), bins (16 B each), then one float4 per image
 // (cylinder-only: cx, cy, R, meta; otherwise cx, cy, cz, R plus a meta
 // array), the transmission thresholds of each image's primitive (float2:
 // from inside, from outside), the inverse radii, then the u16 candidate
 // lists.
 struct LdsTables {
+    float2* sc;
     uint4* bins;
     float4* geom;
     float2* thr;
@@ -131,12 +132,14 @@ struct LdsTables {
 template <bool GEN>
 __device__ __forceinline__ LdsTables stage_tables(const WalkArgs& A, unsigned char* lds) {
     LdsTables T;
-    T.bins = reinterpret_cast<uint4*>(lds);
+    T.sc = reinterpret_cast<float2*>(lds);
+    T.bins = reinterpret_cast<uint4*>(T.sc + kAzimuthEntries);
     T.geom = reinterpret_cast<float4*>(T.bins + A.n_bins);
     T.thr = reinterpret_cast<float2*>(T.geom + A.n_img);
     T.meta = reinterpret_cast<int*>(T.thr + A.n_img);
     T.inv = GEN ? reinterpret_cast<float*>(T.meta + A.n_img) : reinterpret_cast<float*>(T.meta);
     T.cand = reinterpret_cast<uint16_t*>(T.inv + A.n_img);
+    for (int i = threadIdx.x; i < kAzimuthEntries; i += blockDim.x) T.sc[i] = azimuth_entry(i);
     for (int i = threadIdx.x; i < A.n_bins; i += blockDim.x) T.bins[i] = A.g_bins[i];
     for (int i = threadIdx.x; i < A.n_img; i += blockDim.x) {
         if (GEN) {
@@ -316,7 +319,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
             const float ct = fmaf((float)(wb >> 9), 0x1p-22f, 0x1p-23f - 1.0f);
             const float sth = sqrt_unit(fmaf(-ct, ct, 1.0f));
             float sphi, cphi;
-            sincos_turn(wa, sphi, cphi);
+            sincos_tab(T.sc, wa, sphi, cphi);
             ux = sth * cphi;
             uy = sth * sphi;
             uz = ct;
@@ -333,8 +336,8 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
             const float r1 = sqrt_unit(neg2log_unif(wa));
             const float r2 = sqrt_unit(neg2log_unif(wc));
             float s1, c1, s2, c2;
-            sincos_turn(wb, s1, c1);
-            sincos_turn(wd, s2, c2);
+            sincos_tab(T.sc, wb, s1, c1);
+            sincos_tab(T.sc, wd, s2, c2);
             const float n1 = r1 * c1, n2 = r1 * s1, n3 = r2 * c2;
             const float nn = sqrtf(fmaf(n3, n3, fmaf(n2, n2, n1 * n1)));
             const float inv = 1.0f / nn;

@@ -92,10 +92,10 @@ static inline void draw(uint32_t wid, uint32_t idx, uint32_t stream_sub, uint32_
 }
 
 /* ------------------------------------------------------------------ */
-/* sin/cos of 2*pi*u from the Philox word directly.  Quarter-turn range
+/* sin/cos of 2 pi v 2^-24 for a 24-bit integer v.  Quarter-turn range
  * reduction in integers, then Taylor polynomials of sin(pi/2 r) and
- * cos(pi/2 r) on r in [-1/2, 1/2] (truncation < 2e-9).  Replaces
- * phi = 2*PI*curand_uniform; sin(phi), cos(phi) (kernelLattice.cuh:52-57). */
+ * cos(pi/2 r) on r in [-1/2, 1/2] (truncation < 2e-9).  Builds the azimuth
+ * table below. */
 #define S1  1.5707963267948966f
 #define S3 -0.6459640975062462f
 #define S5  0.07969262624616704f
@@ -107,8 +107,7 @@ static inline void draw(uint32_t wid, uint32_t idx, uint32_t stream_sub, uint32_
 #define C8  0.00091926027483942659f
 #define C10 -0.000025202042373060605f
 
-void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
-    uint32_t v = (w >> 8) | 1u;                  /* 24-bit odd: 4u = v * 2^-22 */
+void orc_sincos_u24(uint32_t v, float* s_out, float* c_out) {
     uint32_t q = (v + (1u << 21)) >> 22;         /* nearest quarter 0..4       */
     int32_t ri = (int32_t)v - (int32_t)(q << 22);
     float r = (float)ri * 0x1p-22f;              /* exact, in [-1/2, 1/2]      */
@@ -129,6 +128,39 @@ void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
     case 2: *c_out = -cp; *s_out = -sp; break;
     default: *c_out = sp; *s_out = -cp; break;
     }
+}
+
+/* Azimuth table: entry j = (cos, sin) of 2 pi (j + 1/2) / 256 from the
+ * polynomial above at v = (2j + 1) 2^15; filled when the library loads. */
+static float az_cos[256], az_sin[256];
+__attribute__((constructor)) static void az_init(void) {
+    for (uint32_t j = 0; j < 256; ++j) orc_sincos_u24((2u * j + 1u) << 15, &az_sin[j], &az_cos[j]);
+}
+
+void orc_azimuth_table(float* c, float* s) {
+    for (int j = 0; j < 256; ++j) { c[j] = az_cos[j]; s[j] = az_sin[j]; }
+}
+
+/* sin/cos of 2 pi u(w), u(w) = ((w >> 8) | 1) 2^-24, from the Philox word:
+ * the table entry of the top 8 bits j of the 24-bit v, turned by
+ * d = 2 pi (u - (j + 1/2)/256) = (f - 2^15) K1 (f = low 16 bits of v,
+ * K1 = float(2 pi) 2^-24, |d| <= pi/256) with cos d = 1 - d^2/2 and
+ * sin d = d (1 - d^2/6) (truncation < 1e-9) and the angle-sum formulas.
+ * Replaces phi = 2*PI*curand_uniform; sin(phi), cos(phi)
+ * (kernelLattice.cuh:52-57). */
+void orc_sincos_turn(uint32_t w, float* s_out, float* c_out) {
+    uint32_t v = (w >> 8) | 1u;
+    uint32_t j = v >> 16;
+    float d = fmaf((float)(v & 0xFFFFu), 0x1.921fb6p-22f, -0x1.921fb6p-7f);
+    float d2 = d * d;
+    float cd = fmaf(d2, -0.5f, 1.0f);
+    float sd = d * fmaf(d2, -0.16666667f, 1.0f);
+    *c_out = fmaf(-az_sin[j], sd, az_cos[j] * cd);
+    *s_out = fmaf(az_cos[j], sd, az_sin[j] * cd);
+}
+
+void orc_sincos_turn_n(const uint32_t* w, float* s, float* c, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) orc_sincos_turn(w[k], &s[k], &c[k]);
 }
 
 /* -2 ln u(w) for Box-Muller (Gaussian steps, kernelWC.cuh:56-61): u = k 2^-24

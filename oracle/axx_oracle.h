@@ -15,6 +15,9 @@ float orc_unif(uint32_t w);
 /* -2 ln u(w[k]) of the Gaussian step generator, k < n */
 void orc_neg2log_unif(const uint32_t* w, float* out, int64_t n);
 void  orc_sincos_turn(uint32_t w, float* s, float* c);
+void  orc_sincos_turn_n(const uint32_t* w, float* s, float* c, int64_t n);
+void  orc_sincos_u24(uint32_t v, float* s, float* c);
+void  orc_azimuth_table(float* c, float* s);
 int   orc_validate(const axx_substrate* s, const axx_acq* q);
 int   orc_axes_mask(const axx_acq* q);
 /* bin_width <= 0: reference-faithful linear scan over all primitives

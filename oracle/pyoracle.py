@@ -62,6 +62,9 @@ def _configure(L):
     L.orc_unif.argtypes = [C.c_uint32]
     L.orc_unif.restype = C.c_float
     L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_sincos_turn_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+    L.orc_sincos_u24.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_azimuth_table.argtypes = [C.c_void_p, C.c_void_p]
     L.orc_neg2log_unif.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.orc_sincos64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
     L.orc_sqrt_cr.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
@@ -120,6 +123,29 @@ def neg2log_unif(w) -> np.ndarray:
     out = np.empty(w.shape, np.float32)
     lib().orc_neg2log_unif(w.ctypes.data, out.ctypes.data, w.size)
     return out
+
+
+def sincos_turn_n(w):
+    """sin, cos of 2 pi u(w) for an array of Philox words (float32 arrays)."""
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    s = np.empty(w.size, np.float32)
+    c = np.empty(w.size, np.float32)
+    lib().orc_sincos_turn_n(w.ctypes.data, s.ctypes.data, c.ctypes.data, w.size)
+    return s, c
+
+
+def sincos_u24(v: int):
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincos_u24(int(v) & 0xFFFFFF, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def azimuth_table():
+    """The 256 (cos, sin) entries the azimuth of a step is turned from."""
+    c = np.empty(256, np.float32)
+    s = np.empty(256, np.float32)
+    lib().orc_azimuth_table(c.ctypes.data, s.ctypes.data)
+    return c, s
 
 
 def sincos64(x):

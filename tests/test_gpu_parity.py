@@ -261,6 +261,27 @@ def test_walk_tasks_inside_chunked_launches():
     assert sig.transmissions > 1000
 
 
+@pytest.mark.parametrize("wid,cap", [(6, 0), (3, 0), (1, 0), (5, 0), (6, 50)])
+def test_grazing_slides_equal_oracle(wid, cap):
+    # a walker set within fp32 rounding of the inside of a cylinder with a
+    # tangent step direction: 37-69 grazing reflections in one step, or the
+    # bounce cap (tests/test_oracle_physics.py builds the state)
+    from tests.test_oracle_physics import grazing_case, grazing_state
+    sub, acq = grazing_case()
+    run, st = grazing_state(sub, acq, wid, cap)
+    print(f"[parity] grazing walker {wid} cap {cap}", flush=True)
+    ost = po.walk(sub, acq, run, 1, 2, state=st.copy(), threads=1)
+    with _engine().Plan(sub, acq, run) as plan:
+        plan.walk(0, 1)
+        assert np.array_equal(plan.get_state(), po.walk(sub, acq, run, 0, 1, threads=1))
+        plan.set_state(st)
+        plan.walk(1, 2)
+        gst = plan.get_state()
+    assert np.array_equal(gst, ost)
+    v = abi.state_view(gst, 1, acq.n_profiles)
+    assert int(v["ncoll"][0]) >= 37 and int(v["ncap"][0]) == (1 if cap or wid in (1, 5) else 0)
+
+
 def test_permeable_sphere_lattice_many_events():
     dt = 2e-2
     cs = [[2.5 + 5 * i, 2.5 + 5 * j, 2.5 + 5 * k] for i in range(2) for j in range(2) for k in range(2)]

@@ -323,21 +323,45 @@ def test_gaussian_steps_need_a_wide_cell():
         po.walk(sb.empty_box(30.0, 2.0), acq, abi.RunSpec(n_walkers=4, step_mode=abi.STEP_GAUSSIAN))
 
 
+def grazing_state(sub, acq, wid, bounce_cap=0):
+    """A walker placed in cylinder 0 of `sub` (centre (5, 5), R = 2) at the
+    point where the step-1 direction of its own Philox words is tangent to
+    the surface, the position rounded to fp32 (so within rounding of the
+    surface, on either side): step 1 slides along the inside in grazing
+    chords.  Returns (run, state before step 1)."""
+    run = abi.RunSpec(n_walkers=1, seed=1234, walker_offset=wid, init_mode=abi.INIT_INSIDE_PRIM,
+                      init_arg=0, bounce_cap=bounce_cap)
+    st = po.walk(sub, acq, run, 0, 1, threads=1)
+    o = po.philox([wid, 0, 0, 0], [1234, 0])
+    s, c = po.sincos_turn(int(o[0]))
+    n = np.array([-s, c], np.float64)
+    n /= np.linalg.norm(n)
+    p = np.array([5.0, 5.0]) + 2.0 * n
+    v = st.view(np.float32)
+    v[abi.F_X], v[abi.F_Y] = np.float32(p[0]), np.float32(p[1])
+    return run, st
+
+
+def grazing_case():
+    sub = sb.single_cylinder(10.0, 2.0, D_in=1.7, D_ex=2.0)
+    acq = wf.build_acq([wf.RectGrad(0.0, 1.0, 1.0, (1, 0, 0))], 3, 5e-4, GAMMA)
+    return sub, acq
+
+
 def test_grazing_slides_along_a_surface_finish_their_step():
-    """Two walkers of the gamma-packing benchmark run (4e6 walkers x 80,000
-    steps) start a step on the inside of a cylinder, within fp32 rounding of
-    it, with a nearly tangent direction and slide along it in short chords.
-    Walker 1428273 needs 364 reflections in step 27699 and finishes it under
-    the default cap (4096; the earlier 256 cut it short).  Walker 3587312
-    (step 62203) also drifts outwards by rounding, 2e-5 um per 100
-    reflections, and is the one step of that run cut short by the cap."""
-    from axonexa_amd import workloads as W
-    w = W.c3_gamma(n_walkers=1)
-    for wid, step, events, capped in ((1428273, 27699, 364, 0), (3587312, 62203, 4096, 1)):
-        run = abi.RunSpec(n_walkers=1, seed=1234, relaxation=1, walker_offset=wid)
-        st = po.walk(w.sub, w.acq, run, 0, step, bin_width=1.5, threads=1)
-        n0 = int(abi.state_view(st, 1, w.acq.n_profiles)["ncoll"][0])
-        st = po.walk(w.sub, w.acq, run, step, step + 1, state=st, bin_width=1.5, threads=1)
-        v = abi.state_view(st, 1, w.acq.n_profiles)
-        assert (int(v["ncoll"][0]) - n0, int(v["ncap"][0])) == (events, capped), wid
-        assert int(v["comp"][0]) == int(v["comp0"][0]) > 0       # still inside its cylinder
+    """Walkers that start a step within fp32 rounding of the inside of a
+    cylinder with a tangent direction slide along it in short chords
+    (step 0.0714 um, R = 2 um).  Walker 6 needs 69 reflections and ends
+    7e-6 um outside the surface by rounding while still tracked inside;
+    walkers 1, 2 and 5 slide on until the default cap (4096) ends the step,
+    which is counted; a cap of 50 cuts walker 6 at 50."""
+    sub, acq = grazing_case()
+    for wid, cap, events, capped in ((6, 0, 69, 0), (3, 0, 37, 0), (1, 0, 4096, 1), (2, 0, 4096, 1),
+                                     (5, 0, 4096, 1), (6, 50, 50, 1)):
+        run, st = grazing_state(sub, acq, wid, cap)
+        st = po.walk(sub, acq, run, 1, 2, state=st, threads=1)
+        v = abi.state_view(st, 1, acq.n_profiles)
+        assert (int(v["ncoll"][0]), int(v["ncap"][0])) == (events, capped), wid
+        assert int(v["comp"][0]) == int(v["comp0"][0]) == 1       # still tracked inside its cylinder
+        r = math.hypot(float(v["pos"][0][0]) - 5.0, float(v["pos"][0][1]) - 5.0)
+        assert r < 2.0 + 1e-5

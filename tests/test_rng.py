@@ -50,21 +50,46 @@ def test_uniform_exact_and_open_interval():
     assert po.unif(0xFFFFFFFF) == 1.0 - 2.0 ** -24
 
 
-def test_sincos_turn_accuracy_and_quadrants():
-    rng = np.random.default_rng(0)
-    ws = list(rng.integers(0, 2 ** 32, 4000, dtype=np.uint64))
-    # quadrant boundaries and extremes
-    ws += [0, 0xFFFFFFFF, 1 << 30, (1 << 30) - 1, 1 << 31, 3 << 30, (3 << 30) + 255]
-    worst = 0.0
-    for w in ws:
-        w = int(w)
-        s, c = po.sincos_turn(w)
-        u = ((w >> 8) | 1) * 2.0 ** -24
-        phi = 2.0 * math.pi * u
-        worst = max(worst, abs(s - math.sin(phi)), abs(c - math.cos(phi)))
-        assert abs(s * s + c * c - 1.0) < 5e-7
-    # float32 rounding of values of magnitude <= 1 plus polynomial truncation
-    assert worst < 2.5e-7
+def test_sincos_turn_whole_grid_against_libm():
+    """sin, cos of 2 pi u(w) from the azimuth table, at every one of the
+    2^23 values u = k 2^-24 (k odd) against libm in fp64: the table's fp32
+    rounding, the turn by d (|d| <= pi/256, truncation < 1e-9) and the
+    angle-sum products stay within 2.5e-7; the unit norm within 5e-7."""
+    k = np.arange(1, 1 << 24, 2, dtype=np.uint64)
+    s, c = po.sincos_turn_n((k << 8).astype(np.uint32))
+    phi = 2.0 * np.pi * k.astype(np.float64) * 2.0 ** -24
+    es = np.abs(s.astype(np.float64) - np.sin(phi))
+    ec = np.abs(c.astype(np.float64) - np.cos(phi))
+    assert max(es.max(), ec.max()) < 2.5e-7, (es.max(), ec.max())
+    n2 = s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2
+    assert np.abs(n2 - 1.0).max() < 5e-7
+    # the low 8 bits of the word are not used
+    s2, c2 = po.sincos_turn_n((k << 8 | 0x7F).astype(np.uint32)[::4096])
+    assert np.array_equal(s2, s[::4096]) and np.array_equal(c2, c[::4096])
+
+
+def test_azimuth_table_nodes():
+    """Entry j is the quarter-turn polynomial at 2 pi (j + 1/2)/256, within
+    2e-7 of libm; a table centred on j/256 instead (a plausible off-by-half)
+    would leave the turned values 0.012 off."""
+    c, s = po.azimuth_table()
+    j = np.arange(256)
+    ang = 2.0 * np.pi * (j + 0.5) / 256.0
+    assert np.abs(c - np.cos(ang)).max() < 2e-7 and np.abs(s - np.sin(ang)).max() < 2e-7
+    for jj in (0, 1, 63, 64, 127, 128, 200, 255):
+        ps, pc = po.sincos_u24((2 * jj + 1) << 15)
+        assert (np.float32(pc), np.float32(ps)) == (c[jj], s[jj])
+    # the polynomial itself over the quarter-turn boundaries
+    for v in (0, 1, (1 << 22) - 1, 1 << 22, 1 << 23, 3 << 22, (1 << 24) - 1):
+        ps, pc = po.sincos_u24(v)
+        a = 2.0 * math.pi * v * 2.0 ** -24
+        assert abs(ps - math.sin(a)) < 2e-7 and abs(pc - math.cos(a)) < 2e-7
+    # off-by-half planted in a float64 restatement (entries at j/256, turn
+    # still measured from the bin centre): the bound above rejects it
+    kk = np.arange(1, 1 << 24, 2 * 997, dtype=np.int64)
+    d = 2.0 * np.pi * ((kk & 0xFFFF) - 32768) * 2.0 ** -24
+    a_wrong = 2.0 * np.pi * (kk >> 16) / 256.0 + d
+    assert np.abs(np.cos(a_wrong) - np.cos(2.0 * np.pi * kk * 2.0 ** -24)).max() > 1e-2
 
 
 def test_direction_distribution_is_isotropic():
