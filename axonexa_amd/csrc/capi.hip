@@ -859,8 +859,44 @@ constexpr uint32_t kWaitTicks = 2000000000u; // 20 s of 100 MHz real time: longe
 static int task_steps(const axx_plan* p, long long groups, long long resident_waves, int steps) {
     if (p->sched > 0) return std::min(steps, (p->sched + 1) & ~1);
     if (p->sched < 0 || groups < 2 * resident_waves) return steps;
+    if (p->auto_task > 0) return std::min(steps, (p->auto_task + 1) & ~1);
     const int target = (steps + kAutoChunks - 1) / kAutoChunks;
     return std::min(steps, std::min(kMaxTaskSteps, std::max(kMinTaskSteps, (target + 1) & ~1)));
+}
+
+// Grid and task queue of a persistent walk launch of kernel fn (threads
+// per block `block`, the plan's LDS) over steps [i0, i1): at most
+// the resident workgroups, no more waves than walker groups; zeroes the
+// queue words the launch uses on the context stream.
+int plan_tasks(axx_plan* p, const void* fn, int block, int i0, int i1, WalkQueue* Q, unsigned* blocks) {
+    axx_ctx* ctx = p->ctx;
+    int per_cu = 0;
+    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, p->lds_bytes));
+    if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "walk kernel does not fit a CU at %d threads", block);
+    const long long wpb = block / 64;
+    const long long resident = (long long)per_cu * ctx->n_cu;
+    const long long groups = (p->N + 63) / 64;
+    const int steps = std::max(i1 - std::max(i0, 1), 1);   // chunks count from step 1 (step 0: chunk 0)
+    int chunk = task_steps(p, groups, resident * wpb, steps);
+    long long chunks = (steps + chunk - 1) / chunk;
+    if (groups * chunks >= 0x7fffffffll) {   // one chunk per walk keeps the task count below 2^31
+        chunk = steps;
+        chunks = 1;
+    }
+    *Q = WalkQueue{};
+    Q->q = p->d_queue;
+    Q->n_groups = (uint32_t)groups;
+    Q->n_tasks = (uint32_t)(groups * chunks);
+    Q->chunk = chunk;
+    Q->wait_ticks = kWaitTicks;
+    *blocks = (unsigned)std::min(resident, (groups + wpb - 1) / wpb);
+    const size_t qwords = chunks > 1 ? (size_t)groups + 1 : 1;
+    hipLaunchKernelGGL(zero_words, dim3((unsigned)std::min<size_t>((qwords + 255) / 256, 4096)), dim3(256), 0,
+                       ctx->stream, (uint32_t*)p->d_queue, qwords);
+    HIP_TRY(ctx, hipGetLastError());
+    p->last_task_steps = chunk;
+    p->last_grid = (int)*blocks;
+    return AXX_OK;
 }
 
 static int launch_walk(axx_plan* p, int i0, int i1) {
@@ -878,35 +914,12 @@ static int launch_walk(axx_plan* p, int i0, int i1) {
         A.i0 = 1;
     }
     if (A.i0 < A.i1) {
-        int per_cu = 0;
-        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, block, p->lds_bytes));
-        if (per_cu < 1) return fail(ctx, AXX_ERR_LIMIT, "walk kernel does not fit a CU at %d threads", block);
-        const long long wpb = block / 64;
-        const long long resident = (long long)per_cu * ctx->n_cu;
-        const long long groups = (p->N + 63) / 64;
-        const int steps = A.i1 - A.i0;
-        int chunk = task_steps(p, groups, resident * wpb, steps);
-        long long chunks = (steps + chunk - 1) / chunk;
-        if (groups * chunks >= 0x7fffffffll) {   // one chunk per walk keeps the task count below 2^31
-            chunk = steps;
-            chunks = 1;
-        }
-        WalkQueue Q{};
-        Q.q = p->d_queue;
-        Q.n_groups = (uint32_t)groups;
-        Q.n_tasks = (uint32_t)(groups * chunks);
-        Q.chunk = chunk;
-        Q.wait_ticks = kWaitTicks;
-        const unsigned blocks = (unsigned)std::min(resident, (groups + wpb - 1) / wpb);
-        const size_t qwords = chunks > 1 ? (size_t)groups + 1 : 1;
-        hipLaunchKernelGGL(zero_words, dim3((unsigned)std::min<size_t>((qwords + 255) / 256, 4096)), dim3(256), 0,
-                           ctx->stream, (uint32_t*)p->d_queue, qwords);
+        WalkQueue Q;
+        unsigned blocks = 0;
+        int rc = plan_tasks(p, (const void*)fn, block, A.i0, A.i1, &Q, &blocks);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(block), p->lds_bytes, ctx->stream, A, Q, (const float*)p->d_prof);
         HIP_TRY(ctx, hipGetLastError());
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(block), p->lds_bytes, ctx->stream, A, Q,
-                           (const float*)p->d_prof);
-        HIP_TRY(ctx, hipGetLastError());
-        p->last_task_steps = chunk;
-        p->last_grid = (int)blocks;
     }
     return AXX_OK;
 }

@@ -77,6 +77,7 @@ struct axx_plan {
     int n_chunks = 0, chunk = 0;
     unsigned int* d_queue = nullptr;   // walk task counter + chunks finished per group of 64 walkers
     int sched = 0;                     // steps per walk task: 0 automatic, -1 the whole walk, > 0 that many
+    int auto_task = 0;                 // automatic steps per task once chunks pay off (0: the walk's rule)
     int last_task_steps = 0;           // steps per task of the last walk launch
     int last_grid = 0;                 // workgroups of the last walk launch
     float cell[3] = {0, 0, 0};      // cell the positions wrap in (unit cell of a repeating substrate)
@@ -125,4 +126,5 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
                      const axx_tuning* tune, axx_plan* p, int min_pt, DevArena* arena = nullptr);
 void plan_free(axx_plan* p);
 WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1);
+int plan_tasks(axx_plan* p, const void* fn, int block, int i0, int i1, WalkQueue* Q, unsigned* blocks);
 RedArgs plan_red_args(const axx_plan* p);

@@ -11,6 +11,7 @@ constexpr int kMagRedThreads = 256;
 // LDS for the substrate tables of one 256-thread workgroup: four workgroups
 // (one wave per SIMD each) share a CU's 160 KB.
 constexpr int kMagLdsBudget = 40 * 1024;
+constexpr int kMagTaskSteps = 128;   // automatic steps per walk task of a magnetization run
 
 // Sums of Mx, My, Mz of field j over the walkers of chunk c, in a fixed order
 // (strided per thread, then a fixed tree), into partial[(j * n_chunks + c) * 3];

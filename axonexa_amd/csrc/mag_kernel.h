@@ -159,17 +159,16 @@ constexpr int mag_waves() {
 // runs take the instantiation without them: 1-3 % faster).
 template <int MB, int INTEG, bool GEN, bool REJ, bool ALT>
 __global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_waves<MB, INTEG>()))) void
-mag_walk_kernel(WalkArgs A, MagArgs G) {
+mag_walk_kernel(WalkArgs A, MagArgs G, WalkQueue Q) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const LdsTables T = stage_tables<GEN>(A, lds);
-    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= A.N) return;
-    MagAcc<MB, INTEG> acc;
-    acc.init(G);
-    walk_lane<MagAcc<MB, INTEG>, GEN, REJ, ALT>(A, T, acc, (uint32_t)w);
+    walk_tasks<MagAcc<MB, INTEG>, GEN, REJ, ALT>(A, Q, lds, [&]() {
+        MagAcc<MB, INTEG> acc;
+        acc.init(G);
+        return acc;
+    });
 }
 
-typedef void (*mag_fn)(WalkArgs, MagArgs);
+typedef void (*mag_fn)(WalkArgs, MagArgs, WalkQueue);
 mag_fn pick_mag_rotate(int mb, bool gen, bool rej, bool alt);
 mag_fn pick_mag_rk4(int mb, bool gen, bool rej, bool alt);
 mag_fn pick_mag_dopri(int mb, bool gen, bool rej, bool alt);

@@ -173,8 +173,8 @@ __device__ __forceinline__ float4 image_at(const LdsTables& T, int k, int& mt) {
 typedef const __attribute__((address_space(4))) float* const_fptr;
 
 // Work of one walk launch as tasks t = k * n_groups + g: group g is walkers
-// [64 g, 64 g + 64), time chunk k is steps [i0 + k chunk, i0 + (k + 1) chunk)
-// clipped to i1.  Waves take tasks from a counter in that order (every
+// [64 g, 64 g + 64), time chunk k is steps [b + k chunk, b + (k + 1) chunk)
+// with b = max(i0, 1), clipped to [i0, i1).  Waves take tasks from a counter in that order (every
 // group's chunk k before any group's chunk k + 1), so all groups advance
 // together and the launch ends one chunk after the counter runs out, whatever
 // the walker count; a wave that takes chunk k > 0 of a group first waits for
@@ -632,21 +632,21 @@ __device__ __forceinline__ bool wait_group(const WalkQueue& Q, uint32_t g, uint3
 
 // Persistent walk: the grid is at most what stays resident; the substrate
 // tables are staged once per workgroup, then each wave takes tasks (group,
-// chunk) until the counter passes Q.n_tasks.  With one chunk per walk there
-// are no waits and nothing to publish.  The queue fields and each wave's
-// task sit in LDS and are read back around every task (the empty asm with a
-// memory clobber forces the reads), so none of them holds an SGPR through
-// the step loop, whose SGPR budget is already full.
-constexpr int kWalkStaticLds = 64 + 4 * 16;   // static LDS of walk_kernel (bytes)
+// chunk) until the counter passes Q.n_tasks; make_acc() gives a task's
+// accumulator.  With one chunk per walk there are no waits and nothing to
+// publish.  The queue fields and each wave's task sit in LDS and are read
+// back around every task (the empty asm with a memory clobber forces the
+// reads), so none of them holds an SGPR through the step loop, whose SGPR
+// budget is already full.
+constexpr int kWalkStaticLds = 64 + 4 * 16;   // static LDS of walk_tasks (bytes)
 
-template <int P, int AX, bool GEN, bool REJ, bool ALT>
-__global__ __launch_bounds__(walk_max_block(P)) __attribute__((amdgpu_waves_per_eu(walk_waves(P, AX, ALT)))) void
-walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
-    extern __shared__ __align__(16) unsigned char lds[];
+template <class Acc, bool GEN, bool REJ, bool ALT, class MakeAcc>
+__device__ __forceinline__ void walk_tasks(const WalkArgs& A, const WalkQueue& Q, unsigned char* lds,
+                                           MakeAcc make_acc) {
     __shared__ WalkQueue s_q;
     __shared__ int s_range[2];
     __shared__ uint32_t s_task[16];
-    static_assert(sizeof(WalkQueue) + sizeof(s_range) <= 64, "static LDS of walk_kernel");
+    static_assert(sizeof(WalkQueue) + sizeof(s_range) <= 64, "static LDS of walk_tasks");
     if (threadIdx.x == 0) {
         s_q = Q;
         s_range[0] = A.i0;
@@ -670,15 +670,19 @@ walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
             break;
         }
         if (lane == 0u) s_task[wv] = t;
+        // chunk k: steps [b + k chunk, b + (k + 1) chunk) with b = max(i0, 1)
+        // (step pairs start at odd steps), chunk 0 from i0 (step 0 is the
+        // accumulator's t = 0 term)
         WalkArgs B = A;
         const int chunk = __builtin_amdgcn_readfirstlane(s_q.chunk);
-        B.i0 = __builtin_amdgcn_readfirstlane(s_range[0]) + (int)k * chunk;
-        B.i1 = min(__builtin_amdgcn_readfirstlane(s_range[1]), B.i0 + chunk);
+        const int r0 = __builtin_amdgcn_readfirstlane(s_range[0]);
+        const int b = max(r0, 1);
+        B.i0 = k == 0u ? r0 : b + (int)k * chunk;
+        B.i1 = min(__builtin_amdgcn_readfirstlane(s_range[1]), b + (int)(k + 1u) * chunk);
         const uint32_t w = g * 64u + lane;
         if ((long long)w < A.N) {
-            PhaseAcc<P, AX> acc;
-            acc.prof = (const_fptr)prof_g;
-            walk_lane<PhaseAcc<P, AX>, GEN, REJ, ALT>(B, T, acc, w);
+            Acc acc = make_acc();
+            walk_lane<Acc, GEN, REJ, ALT>(B, T, acc, w);
         }
         asm volatile("" ::: "memory");
         const uint32_t ng = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_groups);
@@ -692,6 +696,17 @@ walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
                 __hip_atomic_store(s_q.q + 1 + (tt - kk * ng), kk + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+template <int P, int AX, bool GEN, bool REJ, bool ALT>
+__global__ __launch_bounds__(walk_max_block(P)) __attribute__((amdgpu_waves_per_eu(walk_waves(P, AX, ALT)))) void
+walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    walk_tasks<PhaseAcc<P, AX>, GEN, REJ, ALT>(A, Q, lds, [&]() {
+        PhaseAcc<P, AX> acc;
+        acc.prof = (const_fptr)prof_g;
+        return acc;
+    });
 }
 
 }  // namespace axx

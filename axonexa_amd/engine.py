@@ -103,6 +103,7 @@ def lib():
     L.axx_mag_create.argtypes = [vp, S, Fp, R, C.POINTER(vp)]
     L.axx_mag_walk.argtypes = [vp, C.c_int, C.c_int]
     L.axx_mag_time_walk.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.axx_mag_set_schedule.argtypes = [vp, C.c_int]
     L.axx_mag_results.argtypes = [vp, MR]
     L.axx_mag_get_state.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32)]
     L.axx_mag_destroy.argtypes = [vp]
@@ -316,6 +317,10 @@ class Mag:
             step_end = self.field.n_steps
         self.ctx.check(lib().axx_mag_walk(self._h, int(step_begin), int(step_end)), "axx_mag_walk")
         return self
+
+    def set_schedule(self, task_steps: int = 0):
+        """Steps per walk task of every group: 0 automatic, -1 the whole walk, > 0 that many."""
+        self.ctx.check(lib().axx_mag_set_schedule(self._h, int(task_steps)), "axx_mag_set_schedule")
 
     def time_walk(self, step_begin: int, step_end: int) -> float:
         ms = C.c_float()

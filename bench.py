@@ -424,6 +424,7 @@ def bench_mag(args) -> int:
     wl = W.mag_lf(**kw)
     n = wl.field.n_steps
     m = engine.Mag(wl.sub, wl.field, wl.run)
+    m.set_schedule(getattr(args, "task_steps", 0))
     for _ in range(args.warmup):
         m.walk(0, n)
         m.results()

@@ -425,6 +425,9 @@ int  axx_mag_create(axx_ctx* ctx, const axx_substrate* s, const axx_field* f, co
  * M = m0 first (same chunking rules as axx_plan_walk). */
 int  axx_mag_walk(axx_mag* m, int step_begin, int step_end);
 int  axx_mag_time_walk(axx_mag* m, int step_begin, int step_end, float* ms);
+/* Walk schedule of every group's launches: as axx_plan_set_schedule (no
+ * reference counterpart). */
+int  axx_mag_set_schedule(axx_mag* m, int task_steps);
 int  axx_mag_results(axx_mag* m, axx_mag_result* out);   /* synchronises */
 /* Walker state of one group: AXX_F_MOM + 6 * F words per walker (F fields
  * in the group); M of field j, component c is the double whose low word is

@@ -21,11 +21,13 @@ def _engine():
     return engine
 
 
-def _check(label, sub, field, run, chunks=None):
+def _check(label, sub, field, run, chunks=None, schedule=None):
     eng = _engine()
     print(f"[mag] {label}: N={run.n_walkers} steps={field.n_steps} fields={field.n_fields} "
-          f"integ={field.integrator}", flush=True)
+          f"integ={field.integrator} schedule={schedule}", flush=True)
     with eng.Mag(sub, field, run) as m:
+        if schedule is not None:
+            m.set_schedule(schedule)
         if chunks:
             b = 0
             for e in chunks + [field.n_steps]:
@@ -86,6 +88,22 @@ def test_dopri_single_pore_rejection_rf(smode):
     run = abi.RunSpec(n_walkers=2048, seed=7, boundary_mode=abi.BOUNDARY_REJECTION,
                       init_mode=abi.INIT_INSIDE_PRIM, step_mode=smode)
     _check(f"dopri_pore_{smode}", sub, f, run)
+
+
+@pytest.mark.parametrize("integ,task_steps", [(abi.MAG_ROTATE, 3), (abi.MAG_RK4, 16), (abi.MAG_DOPRI, 7)])
+def test_walk_tasks_of_a_few_steps(integ, task_steps):
+    # every group's walk handed between waves every few steps (step 0 in
+    # the first task), with chunked launches, permeable spheres and
+    # relaxation: the fp64 magnetization of every walker bitwise
+    dt = 1e-4
+    sub = sb.single_sphere(0.007, 0.0025, D_in=1e-6, D_ex=2e-6, T2_in=60.0, T2_ex=80.0,
+                           kappa=sb.kappa_for_probability(0.05, 1e-6, dt))
+    fs = [fl.COS_GRE(2e-4, (g * 3e-4, 0.0, g * 1e-4), 0.01, 0.003, 2) for g in range(3)]
+    f = fl.build_field(fs, 150, dt, integrator=integ, origin=(0.0035, 0.0035, 0.0035),
+                       T1=np.array([100.0, 70.0]))
+    run = abi.RunSpec(n_walkers=2000, seed=16, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    got = _check(f"tasks_{integ}_{task_steps}", sub, f, run, chunks=[1, 60], schedule=task_steps)
+    assert got.transmissions > 0
 
 
 def test_eleven_fields_two_groups_variable_substeps():
