@@ -47,7 +47,7 @@ BATCH_MAX_PROFILES = 32     # AXX_BATCH_MAX_PROFILES: profiles per instance of a
 MAX_PRIMS = 32768
 
 F_X, F_Y, F_Z, F_DX, F_DY, F_DZ, F_X0, F_Y0, F_Z0, F_E, F_LF = range(11)
-F_COMP, F_COMP0, F_LI, F_NCOLL, F_NTRAN, F_NCAP, F_STEP, F_MOM = range(11, 19)
+F_COMP, F_COMP0, F_LI, F_NCOLL, F_NTRAN, F_NCAP, F_STEP, F_BUD, F_MOM = range(11, 20)
 
 
 def state_fields(n_profiles: int) -> int:
@@ -293,7 +293,7 @@ def state_view(state: np.ndarray, n_walkers: int, n_profiles: int):
         "pos": f[F_X:F_Z + 1].T, "disp": f[F_DX:F_DZ + 1].T, "r0": f[F_X0:F_Z0 + 1].T,
         "E": f[F_E], "lf": f[F_LF], "comp": i[F_COMP], "comp0": i[F_COMP0],
         "li": i[F_LI], "ncoll": st[F_NCOLL], "ntran": st[F_NTRAN], "ncap": st[F_NCAP],
-        "step": i[F_STEP],
+        "step": i[F_STEP], "bud": f[F_BUD],
         "mom": f[F_MOM:].reshape(n_profiles, 3, n_walkers).transpose(2, 0, 1),
     }
 

@@ -338,17 +338,25 @@ static bool surface_near_box(const float4& g, int meta, const double lo[3], cons
     return std::sqrt(dmin2) <= R + reach && std::sqrt(dmax2) >= R - reach;
 }
 
-// LDS tables: bins (16 B), images (16 B), per-image transmission
-// thresholds (8 B), meta words (4 B, general geometry), inverse radii (4 B),
-// candidates (2 B).
+// LDS tables: the azimuth table (2 KB), bins (16 B), images (16 B),
+// per-image transmission hazards (8 B), meta words (4 B, general geometry),
+// inverse radii (4 B), candidates (2 B).
 static size_t lds_size(size_t n_img, size_t n_bins, size_t n_cand, bool gen) {
     return kAzimuthEntries * sizeof(float2) + n_bins * 16 + n_img * (gen ? 32 : 28) + n_cand * 2;
 }
 
 // Images, compartments and bins.  The float32 image centres are
 // (float)(centre + shift) computed in double; compartments use
-// L = (float)sqrt(6 D dt), rate = (float)(dt / T2), thresholds
-// (float)(4 kappa / v).
+// L = (float)sqrt(6 D dt), rate = (float)(dt / T2), transmission hazards
+// (float)(-2 ln(1 - 4 kappa / v)).
+// Hazard of a boundary event that transmits with probability p = 4 kappa / v
+// (boundaryCheck.cuh:94): h = -2 ln(1 - p), so that a budget drawn as
+// 2 Exp(1) = -2 ln u runs out at this event with probability p; p >= 1
+// always transmits.
+static float transmission_hazard(double p) {
+    return p >= 1.0 ? INFINITY : (float)(-2.0 * std::log1p(-p));
+}
+
 // Longest step of a step generator in units of sqrt(6 D dt): Gaussian steps
 // have |n| <= sqrt(2 * 33.27) = 8.157 standard normals (24-bit uniforms),
 // i.e. at most 4.71 such units.
@@ -383,8 +391,8 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         H.rate[k + 1] = (s->T2[k] > 0 && std::isfinite(s->T2[k])) ? (float)(dt / s->T2[k]) : 0.0f;
         const double kap = s->kappa ? s->kappa[k] : 0.0;
         const double v_in = std::sqrt(6.0 * s->D[k] / dt);
-        H.thr_out[k] = kap > 0 ? (float)(4.0 * kap / v_in) : 0.0f;
-        H.thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
+        H.thr_out[k] = kap > 0 ? transmission_hazard(4.0 * kap / v_in) : 0.0f;
+        H.thr_in[k] = kap > 0 ? transmission_hazard(4.0 * kap / v_lat) : 0.0f;
         if (s->type[k] == AXX_PRIM_SPHERE) H.gen = true;
     }
     H.Lmax = Lmax;

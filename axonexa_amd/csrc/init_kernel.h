@@ -90,6 +90,8 @@ __device__ __forceinline__ void init_lane(const WalkArgs& A, const float4* geom,
     st[(size_t)F_NTRAN * N + w] = 0u;
     st[(size_t)F_NCAP * N + w] = 0u;
     is[(size_t)F_STEP * N + w] = 1;
+    // transmission budget for the walk's first transmission (ordinal 0)
+    fs[(size_t)F_BUD * N + w] = neg2log_unif(philox10(wid, 0u, kStreamColl, A.inst, A.k0, A.k1).x);
     for (int f = 0; f < n_mom; ++f) fs[(size_t)(F_MOM + f) * N + w] = 0.0f;
 }
 

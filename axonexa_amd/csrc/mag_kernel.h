@@ -46,7 +46,6 @@ inline int mag_samples(int integ, int s) {
 template <int MB, int INTEG>
 struct MagAcc {
     static constexpr bool kOneSite = true;
-    static constexpr bool kCacheColl = true;
     const_dptr tab;
     const_iptr off;
     const_iptr nsub;

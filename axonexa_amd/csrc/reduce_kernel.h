@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/axonexa.h"
+
 namespace axx {
 
 constexpr int kRedThreads = 256;
@@ -67,17 +69,17 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_i
     long long wend = wbeg + R.chunk;
     if (wend > N) wend = N;
     for (long long w = wbeg + threadIdx.x; w < wend; w += kRedThreads) {
-        const int comp = is[11 * N + w];
-        const int li = is[13 * N + w];
-        const double Ef = (double)fs[9 * N + w] +
-                          ((double)(R.n_steps - li) - (double)fs[10 * N + w]) * (double)R.rate[comp];
+        const int comp = is[AXX_F_COMP * N + w];
+        const int li = is[AXX_F_LI * N + w];
+        const double Ef = (double)fs[AXX_F_E * N + w] +
+                          ((double)(R.n_steps - li) - (double)fs[AXX_F_LF * N + w]) * (double)R.rate[comp];
         const double wt = R.relaxation ? exp(-Ef) : 1.0;
-        const bool intra = is[12 * N + w] != 0;
-        const double r0[3] = {(double)fs[6 * N + w], (double)fs[7 * N + w], (double)fs[8 * N + w]};
+        const bool intra = is[AXX_F_COMP0 * N + w] != 0;
+        const double r0[3] = {(double)fs[AXX_F_X0 * N + w], (double)fs[AXX_F_Y0 * N + w], (double)fs[AXX_F_Z0 * N + w]};
         if (gy == 0) {
-            cc[0] += R.state[14 * N + w];
-            cc[1] += R.state[15 * N + w];
-            cc[2] += R.state[16 * N + w];
+            cc[0] += R.state[AXX_F_NCOLL * N + w];
+            cc[1] += R.state[AXX_F_NTRAN * N + w];
+            cc[2] += R.state[AXX_F_NCAP * N + w];
             cc[3] += intra ? 1ull : 0ull;
         }
 #pragma unroll
@@ -90,7 +92,7 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_i
             for (int a = 0; a < 3; ++a) {
                 const double d = R.wf_dir[3 * j + a];
                 if (d == 0.0) continue;
-                ph += d * ((double)fs[(18 + 3 * p + a) * N + w] + r0[a] * R.S[p]);
+                ph += d * ((double)fs[(AXX_F_MOM + 3 * p + a) * N + w] + r0[a] * R.S[p]);
             }
             const double phi = R.wf_scale[j] * ph;
             double sv, cv;

@@ -35,8 +35,9 @@ struct WalkArgs {
     const int* __restrict__ g_meta;         // [n_img] compartment | kSphereBit | kPlaneBit
     const float4* __restrict__ g_cyl;       // [n_img] cx, cy, R, meta bits (cylinder-only substrates)
     const float* __restrict__ g_inv;        // [n_img] 1 / R (float division; 0 for planes)
-    const float2* __restrict__ g_thr;       // [n_img] 4 kappa / v_inside, 4 kappa / v_lattice of the
-                                            // image's primitive (0 for planes)
+    const float2* __restrict__ g_thr;       // [n_img] transmission hazards -2 ln(1 - p) of the image's
+                                            // primitive, p = 4 kappa / v_inside, 4 kappa / v_lattice
+                                            // (+inf for p >= 1; 0 for planes)
     const uint4* __restrict__ g_bins;       // [n_bins] {offset << 8 | count, sub-bin nibbles 0-7, 8-15, 0}
     const uint16_t* __restrict__ g_cand;    // [n_cand] image indices, ascending per bin
     const float* __restrict__ prof;         // [n_steps][Pt] unit profile samples
@@ -62,7 +63,7 @@ constexpr uint32_t kNibMany = 8u;
 // state field indices (include/axonexa.h)
 enum {
     F_X = 0, F_Y, F_Z, F_DX, F_DY, F_DZ, F_X0, F_Y0, F_Z0, F_E, F_LF,
-    F_COMP, F_COMP0, F_LI, F_NCOLL, F_NTRAN, F_NCAP, F_STEP, F_MOM
+    F_COMP, F_COMP0, F_LI, F_NCOLL, F_NTRAN, F_NCAP, F_STEP, F_BUD, F_MOM
 };
 
 // g = (cx, cy, cz, R); inside test against R^2 = R * R (cylinderXY.cuh:69-74,
@@ -116,7 +117,7 @@ __device__ __forceinline__ bool plane_hit(float4 g, float x0, float y0, float z0
 
 // Substrate tables in LDS: the azimuth table (2 KB), bins (16 B each), then one float4 per image
 // (cylinder-only: cx, cy, R, meta; otherwise cx, cy, cz, R plus a meta
-// array), the transmission thresholds of each image's primitive (float2:
+// array), the transmission hazards of each image's primitive (float2:
 // from inside, from outside), the inverse radii, then the u16 candidate
 // lists.
 struct LdsTables {
@@ -207,10 +208,6 @@ constexpr int walk_max_block(int P) { return P >= 32 ? 512 : 1024; }
 template <int P, int AX>
 struct PhaseAcc {
     static constexpr bool kOneSite = false;
-    // keep the last transmission-draw words in registers (four events per
-    // Philox call) unless the moments already fill the register file (96 or
-    // more accumulators): there recomputing the call per event is 1 % faster
-    static constexpr bool kCacheColl = P * ((AX & 1) + ((AX >> 1) & 1) + ((AX >> 2) & 1)) < 96;
     const_fptr prof;
     float m[P][3];
 
@@ -268,15 +265,11 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     comp = (is + F_COMP * N)[wo]; li = (is + F_LI * N)[wo];
     ncoll = (st + F_NCOLL * N)[wo]; ntran = (st + F_NTRAN * N)[wo];
     ncap = (st + F_NCAP * N)[wo];
+    float bud = (fs + F_BUD * N)[wo];
     acc.load(fs, N, wo);
 
     const int cap = A.cap;
     const bool permeable = A.mode == 1;
-    // transmission draws: boundary event n of the walk takes word n mod 4 of
-    // the Philox call at (wid, n / 4, kStreamColl, inst); the last call's
-    // words stay here so one call serves four events (block 0xFFFFFFFF: none)
-    uint32_t cblk = 0xFFFFFFFFu;
-    u32x4 cw = {0u, 0u, 0u, 0u};
     float Lc = A.compL[comp];
 
     // Surfaces within reach of the current start point: the bin's list,
@@ -431,17 +424,14 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                 const bool ins = bins_;
                 bool transmit = false;
                 if (permeable && !pl) {     // planes never transmit (plane.cuh:39)
-                    // transmission with p = 4 kappa / v (boundaryCheck.cuh:94)
-                    const uint32_t n = ncoll - 1u;   // this event's ordinal in the walk
-                    if (!Acc::kCacheColl || (n >> 2) != cblk) {
-                        cw = philox10(wid, n >> 2, kStreamColl, A.inst, A.k0, A.k1);
-                        cblk = n >> 2;
-                    }
-                    const uint32_t sel = n & 3u;
-                    const uint32_t wv = sel == 0 ? cw.x : sel == 1 ? cw.y : sel == 2 ? cw.z : cw.w;
-                    const float2 th = T.thr[bk];   // this surface's (thr_out, thr_in), from LDS
-                    const float thr = ins ? th.x : th.y;
-                    transmit = unif(wv) <= thr;
+                    // transmission with p = 4 kappa / v (boundaryCheck.cuh:94):
+                    // the budget, a 2 Exp(1) draw, pays this side's hazard
+                    // h = -2 ln(1 - p); the event transmits when the budget
+                    // runs out, which happens with probability p whatever
+                    // came before (the exponential has no memory)
+                    const float2 hz = T.thr[bk];   // this surface's (from inside, from outside), from LDS
+                    bud = bud - (ins ? hz.x : hz.y);
+                    transmit = bud <= 0.0f;
                 }
                 lastpl = pl ? bk : -1;
                 // left outward (outside reflection, transmission from inside,
@@ -459,6 +449,8 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     Lc = A.compL[nc];
                     ell = f * (gs * Lc);
                     ntran++;
+                    // the next budget: transmission ordinal ntran
+                    bud = neg2log_unif(philox10(wid, ntran, kStreamColl, A.inst, A.k0, A.k1).x);
                 } else {
                     // specular reflection about the surface normal (p - c) / R, or
                     // the plane's normal (boundaryCheck.cuh:59-62, :96-103,
@@ -599,6 +591,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
     (sto + F_NCOLL * N)[wo2] = ncoll; (sto + F_NTRAN * N)[wo2] = ntran;
     (sto + F_NCAP * N)[wo2] = ncap;
     (iso + F_STEP * N)[wo2] = A.i1;
+    (fso + F_BUD * N)[wo2] = bud;
     acc.store(fso, N, wo2);
 }
 

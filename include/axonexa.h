@@ -175,7 +175,10 @@ typedef struct axx_result {
 #define AXX_F_NTRAN 15   /* uint: transmissions                             */
 #define AXX_F_NCAP  16   /* uint: bounce-cap truncations                    */
 #define AXX_F_STEP  17   /* int: next step index to run                     */
-#define AXX_F_MOM   18
+#define AXX_F_BUD   18   /* float: transmission budget left: a 2 Exp(1) draw
+                          * less the hazards of the permeable boundary events
+                          * since the last transmission                     */
+#define AXX_F_MOM   19
 #define AXX_STATE_FIELDS(P) (AXX_F_MOM + 3 * (P))
 
 /* ---- context ---- */

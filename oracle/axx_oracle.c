@@ -92,6 +92,12 @@ static inline void draw(uint32_t wid, uint32_t idx, uint32_t stream_sub, uint32_
 }
 
 /* ------------------------------------------------------------------ */
+/* Hazard of a boundary event that transmits with probability p = 4 kappa / v
+ * (boundaryCheck.cuh:94): h = -2 ln(1 - p), so that a budget drawn as
+ * 2 Exp(1) = -2 ln u (u uniform) runs out at the event with probability p;
+ * p >= 1: always. */
+static float hazard(double p) { return p >= 1.0 ? INFINITY : (float)(-2.0 * log1p(-p)); }
+
 /* sin/cos of 2 pi v 2^-24 for a 24-bit integer v.  Quarter-turn range
  * reduction in integers, then Taylor polynomials of sin(pi/2 r) and
  * cos(pi/2 r) on r in [-1/2, 1/2] (truncation < 2e-9).  Builds the azimuth
@@ -409,8 +415,8 @@ typedef struct {
     int n_comp;
     float* L;      /* [n_comp] step length sqrt(6 D dt) */
     float* rate;   /* [n_comp] dt / T2 (0 if no decay)   */
-    float* thr_out;/* [n_prims] 4 kappa / v_inside          */
-    float* thr_in; /* [n_prims] 4 kappa / v_lattice         */
+    float* thr_out;/* [n_prims] hazard of p = 4 kappa / v_inside  */
+    float* thr_in; /* [n_prims] hazard of p = 4 kappa / v_lattice */
     float Lmax;
     float H2c, H2s; /* squared half of the smallest cell side (xy; xyz) */
     float plane_eps;/* minimum distance from a plane after reflecting from it */
@@ -683,8 +689,8 @@ static int oprep_build_view(oprep* P, const axx_substrate* s, const axx_acq* q, 
         P->rate[k + 1] = (s->T2[k] > 0 && isfinite(s->T2[k])) ? (float)(dt / s->T2[k]) : 0.0f;
         double kap = s->kappa ? s->kappa[k] : 0.0;
         double v_in = sqrt(6.0 * s->D[k] / dt);
-        P->thr_out[k] = kap > 0 ? (float)(4.0 * kap / v_in) : 0.0f;
-        P->thr_in[k] = kap > 0 ? (float)(4.0 * kap / v_lat) : 0.0f;
+        P->thr_out[k] = kap > 0 ? hazard(4.0 * kap / v_in) : 0.0f;
+        P->thr_in[k] = kap > 0 ? hazard(4.0 * kap / v_lat) : 0.0f;
     }
     P->Lmax = (float)Lmax;
     for (int k = 0; k < n; ++k) P->any_3d |= s->type[k] != AXX_PRIM_CYLINDER_XY;
@@ -1047,7 +1053,7 @@ static void* job_run(void* arg) {
 
     for (int64_t w = J->w_begin; w < J->w_end; ++w) {
         uint32_t wid = R->walker_offset + (uint32_t)w;
-        float x, y, z, dx, dy, dz, E, lf;
+        float x, y, z, dx, dy, dz, E, lf, bud;
         int comp, li;
         uint32_t ncoll, ntran, ncap;
         if (J->i0 == 0) {
@@ -1057,6 +1063,11 @@ static void* job_run(void* arg) {
             dx = dy = dz = 0.0f;
             E = 0.0f; lf = 0.0f; li = 1;
             ncoll = ntran = ncap = 0;
+            {   /* budget of the first transmission (ordinal 0) */
+                uint32_t ob[4];
+                draw(wid, 0u, STREAM_COLL << 24, inst, key, ob);
+                bud = neg2log_unif(ob[0]);
+            }
             memset(m, 0, sizeof(m));
             FS(AXX_F_X0)[w] = px0; FS(AXX_F_Y0)[w] = py0; FS(AXX_F_Z0)[w] = pz0;
             IS(AXX_F_COMP0)[w] = comp;
@@ -1074,6 +1085,7 @@ static void* job_run(void* arg) {
             comp = IS(AXX_F_COMP)[w]; li = IS(AXX_F_LI)[w];
             ncoll = (uint32_t)IS(AXX_F_NCOLL)[w]; ntran = (uint32_t)IS(AXX_F_NTRAN)[w];
             ncap = (uint32_t)IS(AXX_F_NCAP)[w];
+            bud = FS(AXX_F_BUD)[w];
             if (G) {
                 const uint32_t* U = J->st;
                 for (int j = 0; j < G->F; ++j) {
@@ -1188,20 +1200,18 @@ static void* job_run(void* arg) {
                     int inside = bins_;
                     int transmit = 0;
                     if (mode == AXX_BOUNDARY_PERMEABLE && pk->type != AXX_PRIM_PLANE) {
-                        /* boundary event n of the walk (n counts every event,
-                         * planes and earlier launches included): word n mod 4
-                         * of the call at counter (wid, n / 4, 1 << 24, inst) */
-                        uint32_t oc[4];
-                        const uint32_t n = ncoll - 1u;
-                        draw(wid, n >> 2, STREAM_COLL << 24, inst, key, oc);
-                        float U = unif(oc[n & 3u]);
-                        float thr = inside ? P->thr_out[pk->orig] : P->thr_in[pk->orig];
+                        /* the budget (a 2 Exp(1) draw) pays this side's hazard
+                         * h = -2 ln(1 - 4 kappa / v); the event transmits when
+                         * it runs out: probability 4 kappa / v, independent of
+                         * the events before (memoryless) */
+                        float h = inside ? P->thr_out[pk->orig] : P->thr_in[pk->orig];
 #if ORC_PLANT == 1
-                        thr = thr + thr;                 /* planted: twice 4 kappa / v */
+                        h = h + h;                       /* planted: twice the hazard (p -> 2p - p^2) */
 #elif ORC_PLANT == 3
-                        thr = P->thr_in[pk->orig];       /* planted: lattice speed on both sides */
+                        h = P->thr_in[pk->orig];         /* planted: lattice speed on both sides */
 #endif
-                        transmit = U <= thr;
+                        bud = bud - h;
+                        transmit = bud <= 0.0f;
                         J->ev[inside ? 0 : 2]++;
                         J->ev[inside ? 1 : 3] += transmit;
                     }
@@ -1223,6 +1233,11 @@ static void* job_run(void* arg) {
                         comp = nc;
                         ell = f * (gs * P->L[nc]);
                         ntran++;
+                        {   /* the next budget: transmission ordinal ntran */
+                            uint32_t ob[4];
+                            draw(wid, ntran, STREAM_COLL << 24, inst, key, ob);
+                            bud = neg2log_unif(ob[0]);
+                        }
                     } else {
                         float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
                         if (pk->type == AXX_PRIM_PLANE) {
@@ -1300,6 +1315,7 @@ static void* job_run(void* arg) {
         IS(AXX_F_NCOLL)[w] = (int32_t)ncoll; IS(AXX_F_NTRAN)[w] = (int32_t)ntran;
         IS(AXX_F_NCAP)[w] = (int32_t)ncap;
         IS(AXX_F_STEP)[w] = J->i1;
+        FS(AXX_F_BUD)[w] = bud;
         if (G) {
             uint32_t* U = J->st;
             for (int j = 0; j < G->F; ++j) {
