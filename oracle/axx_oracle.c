@@ -1233,11 +1233,19 @@ static void* job_run(void* arg) {
                         comp = nc;
                         ell = f * (gs * P->L[nc]);
                         ntran++;
+#if ORC_PLANT == 4
+                        /* planted: the budget is not redrawn after a transmission */
+#else
                         {   /* the next budget: transmission ordinal ntran */
                             uint32_t ob[4];
+#if ORC_PLANT == 5
+                            draw(wid, 0u, STREAM_COLL << 24, inst, key, ob);   /* planted: always budget 0 */
+#else
                             draw(wid, ntran, STREAM_COLL << 24, inst, key, ob);
+#endif
                             bud = neg2log_unif(ob[0]);
                         }
+#endif
                     } else {
                         float nx = px - pk->cx, ny = py - pk->cy, nz = 0.0f, kd;
                         if (pk->type == AXX_PRIM_PLANE) {

@@ -6,6 +6,10 @@ fix, checked against rules stated independently of the oracle's code:
   * a boundary event transmits with probability 4 kappa / v, v the speed on
     the side the walker comes from (boundaryCheck.cuh:76,94): counted per side
     with D_in != D_ex, within 4 binomial sigma;
+  * each event decides on its own (one uniform per event in the reference):
+    with the same probability p from both sides, every walker's
+    transmission count is binomial in its event count, so the counts'
+    dispersion about p times the events is that of independent trials;
   * after a transmission the walker goes on along the same direction with the
     far side's speed for the rest of dt (boundaryCheck.cuh:83,109): the end
     point of a one-event step sits where that rule puts it;
@@ -20,9 +24,10 @@ fix, checked against rules stated independently of the oracle's code:
     the equilibrium of the 4 kappa / v rule).
 
 Each check is also run on oracles rebuilt with a planted mistake (ORC_PLANT
-in oracle/axx_oracle.c: twice the probability; the reference's whole-step
-charge to the far side; the lattice speed on both sides) and must fail on
-each of them.
+in oracle/axx_oracle.c: twice the hazard; the reference's whole-step
+charge to the far side; the lattice speed on both sides; the transmission
+budget not redrawn after a transmission; every budget the walk's first)
+and must fail on each of them.
 """
 import math
 
@@ -61,6 +66,27 @@ def check_rate_per_side():
         res.append((n > 20000, abs(t / max(n, 1) - p) < 4 * sd, t / max(n, 1), p))
     ok = all(a and b for a, b, _, _ in res)
     return ok, res
+
+
+def check_transmissions_independent_per_event():
+    """Per-walker transmission counts against Binomial(events, p), p the
+    same from both sides (D_in = D_ex): the pooled fraction within 4 sigma
+    and the dispersion index sum (t - p n)^2 / sum p (1 - p) n within 4
+    sigma of 1 (about sqrt(2 / walkers))."""
+    p = 0.3
+    kappa = sb.kappa_for_probability(p, D_IN, DT)
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=D_IN, D_ex=D_IN, kappa=kappa)
+    run = abi.RunSpec(n_walkers=3000, seed=23, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    st = po.walk(sub, _acq(800), run, bin_width=1.0)
+    v = abi.state_view(st, run.n_walkers, 1)
+    n = v["ncoll"].astype(np.float64)
+    t = v["ntran"].astype(np.float64)
+    m = n > 0
+    frac = t[m].sum() / n[m].sum()
+    sd = math.sqrt(p * (1 - p) / n[m].sum())
+    disp = float(((t[m] - p * n[m]) ** 2).sum() / (p * (1 - p) * n[m]).sum())
+    ok = (n[m].sum() > 50000 and abs(frac - p) < 4 * sd and abs(disp - 1.0) < 4 * math.sqrt(2.0 / m.sum()))
+    return ok, (frac, disp, int(m.sum()))
 
 
 def _one_step_walk(sub, acq, run):
@@ -162,8 +188,8 @@ def check_mean_exponent_at_equilibrium():
     return ok, (float(x.mean()), want, sd)
 
 
-CHECKS = [check_rate_per_side, check_split_follows_geometry, check_equal_t2_exponent,
-          check_mean_exponent_at_equilibrium]
+CHECKS = [check_rate_per_side, check_transmissions_independent_per_event, check_split_follows_geometry,
+          check_equal_t2_exponent, check_mean_exponent_at_equilibrium]
 
 
 @pytest.mark.parametrize("check", CHECKS, ids=lambda c: c.__name__)
@@ -172,10 +198,11 @@ def test_oracle_permeable_physics(check):
     assert ok, detail
 
 
-@pytest.mark.parametrize("plant", [1, 2, 3])
+@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5])
 def test_planted_mistakes_fail_a_check(plant, tmp_path):
-    # 1: twice 4 kappa / v; 2: whole crossing step charged to the far side
-    # (D2); 3: the lattice speed in the threshold from both sides
+    # 1: twice the hazard; 2: whole crossing step charged to the far side
+    # (D2); 3: the lattice speed in the hazard from both sides; 4: no new
+    # budget after a transmission; 5: every budget the walk's first
     with po.planted(plant, str(tmp_path)):
         results = {c.__name__: c()[0] for c in CHECKS}
     assert not all(results.values()), results
