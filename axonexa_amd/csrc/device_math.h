@@ -261,8 +261,12 @@ __device__ __forceinline__ bool quad_hits(const Quad& Q, float ell) {
 __device__ __forceinline__ float quad_root(const Quad& Q, float ell) {
     const float sq = sqrtf(Q.disc > 0.0f ? Q.disc : 0.0f);
     const bool far_root = Q.inside && Q.B < 0.0f;
-    const float num = far_root ? sq - Q.B : (Q.inside ? -Q.Cp : Q.Cp);
-    const float den = far_root ? Q.A : (Q.inside ? Q.B + sq : sq - Q.B);
+    // both sums in every lane (the empty asm keeps the compiler from sinking
+    // them into exec-mask branches): the choices below stay v_cndmask selects
+    float smb = sq - Q.B, bps = Q.B + sq;
+    asm volatile("" : "+v"(smb), "+v"(bps));
+    const float num = far_root ? smb : (Q.inside ? -Q.Cp : Q.Cp);
+    const float den = far_root ? Q.A : (Q.inside ? bps : smb);
     const float t = den > 0.0f ? num / den : 0.0f;
     return t < ell ? t : ell;
 }
