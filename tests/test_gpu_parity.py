@@ -250,6 +250,18 @@ def test_walk_tasks_of_a_few_steps_equal_oracle(task_steps):
     assert sig.collisions > 100000 and sig.transmissions > 50000
 
 
+@pytest.mark.parametrize("prob", [1.0, 3.0])
+def test_certain_transmission(prob):
+    # 4 kappa / v >= 1: infinite hazard, every boundary event transmits
+    dt = 1e-2
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=1.0, D_ex=1.0,
+                         kappa=sb.kappa_for_probability(prob, 1.0, dt))
+    acq = _pgse(400, dt)
+    run = abi.RunSpec(n_walkers=1500, seed=31, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both(f"certain_transmission_{prob}", sub, acq, run)
+    assert sig.transmissions == sig.collisions > 1000
+
+
 def test_walk_tasks_inside_chunked_launches():
     # launches that begin on an even step: every task then starts inside a
     # step pair and takes the single-step path first

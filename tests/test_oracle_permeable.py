@@ -192,6 +192,19 @@ CHECKS = [check_rate_per_side, check_transmissions_independent_per_event, check_
           check_equal_t2_exponent, check_mean_exponent_at_equilibrium]
 
 
+@pytest.mark.parametrize("p", [1.0, 3.0])
+def test_probability_one_or_more_always_transmits(p):
+    # 4 kappa / v >= 1: the hazard is infinite and every event transmits
+    # (boundaryCheck.cuh:94: a uniform in (0, 1] is never above p)
+    kappa = sb.kappa_for_probability(p, D_IN, DT)
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=D_IN, D_ex=D_IN, kappa=kappa)
+    run = abi.RunSpec(n_walkers=500, seed=29, boundary_mode=abi.BOUNDARY_PERMEABLE)
+    st = po.walk(sub, _acq(300), run, bin_width=1.0)
+    v = abi.state_view(st, run.n_walkers, 1)
+    n, t = int(v["ncoll"].sum()), int(v["ntran"].sum())
+    assert n > 1000 and t == n
+
+
 @pytest.mark.parametrize("check", CHECKS, ids=lambda c: c.__name__)
 def test_oracle_permeable_physics(check):
     ok, detail = check()
