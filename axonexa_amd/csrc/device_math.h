@@ -225,9 +225,12 @@ __device__ __forceinline__ float sqrt_unit(float x) {
 struct Quad {
     float A, B, Cp, fe, disc;
     bool inside;
+    unsigned long long m_in;   // MASK quads: the wave's lane mask of inside
 };
 
-template <bool SPH>
+// MASK: also the lane mask of `inside` (for quad_hits_masked), made from the
+// two compares with ballots.
+template <bool SPH, bool MASK = false>
 __device__ __forceinline__ Quad quad_of(float ox, float oy, float oz, float ux, float uy, float uz,
                                         float A, float R2, bool same_comp, float H2, float ell) {
     Quad Q;
@@ -241,7 +244,13 @@ __device__ __forceinline__ Quad quad_of(float ox, float oy, float oz, float ux, 
     }
     const float C = q - R2;
     Q.A = A;
-    Q.inside = same_comp && q < H2;
+    if (MASK) {
+        Q.m_in = __builtin_amdgcn_ballot_w64(same_comp) & __builtin_amdgcn_ballot_w64(q < H2);
+        Q.inside = __builtin_amdgcn_inverse_ballot_w64(Q.m_in);
+    } else {
+        Q.m_in = 0ull;
+        Q.inside = same_comp && q < H2;
+    }
     Q.Cp = Q.inside ? (C < 0.0f ? C : 0.0f) : (C > 0.0f ? C : 0.0f);
     Q.fe = fmaf(fmaf(A, ell, Q.B + Q.B), ell, Q.Cp);
     Q.disc = fmaf(Q.B, Q.B, -(A * Q.Cp));
@@ -254,6 +263,24 @@ __device__ __forceinline__ bool quad_hits(const Quad& Q, float ell) {
     // them all avoids the exec-mask branches the && / || chain compiles to
     const bool hit_out = (Q.B < 0.0f) & ((Q.Cp == 0.0f) | (Q.fe < 0.0f) | (((-Q.B) < Q.A * ell) & (Q.disc > 0.0f)));
     return (Q.A > 0.0f) & (Q.inside ? hit_in : hit_out);
+}
+
+// quad_hits for a MASK quad when A > 0 holds in every lane (cylinder-only
+// walks: A = ux^2 + uy^2 of a drawn direction is at least about 2^-22 --
+// sin^2 theta of the isotropic draw, 2/3 for axis signs, r1^2 / |n|^2 for
+// Gaussian steps -- and a reflection about a normal with nz = 0 keeps it to
+// rounding).  Each compare is taken as the lane mask of its v_cmp (ballot),
+// the masks are combined by scalar instructions and the result read back as
+// the lane's condition (inverse ballot): the same decision as quad_hits
+// without turning booleans into 0/1 vector values.  valid: lanes whose
+// quad is a real candidate (others answer false).
+__device__ __forceinline__ bool quad_hits_masked(const Quad& Q, float ell, unsigned long long valid) {
+    typedef unsigned long long u64;
+    const u64 hit_in = __builtin_amdgcn_ballot_w64(Q.fe > 0.0f);
+    const u64 hit_out = __builtin_amdgcn_ballot_w64(Q.B < 0.0f) &
+                        (__builtin_amdgcn_ballot_w64(Q.Cp == 0.0f) | __builtin_amdgcn_ballot_w64(Q.fe < 0.0f) |
+                         (__builtin_amdgcn_ballot_w64((-Q.B) < Q.A * ell) & __builtin_amdgcn_ballot_w64(Q.disc > 0.0f)));
+    return __builtin_amdgcn_inverse_ballot_w64(valid & ((Q.m_in & hit_in) | (~Q.m_in & hit_out)));
 }
 
 // inside, B < 0: (sq - B) / A;  inside, B >= 0: -Cp / (B + sq);

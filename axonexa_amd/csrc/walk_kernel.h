@@ -392,9 +392,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                     const bool same = comp == (mt & kCompMask);
                     const Quad Q = sk ? quad_of<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w * g.w,
                                                       same, A.H2s, ell)
-                                      : quad_of<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w,
-                                                       same, A.H2c, ell);
-                    if (quad_hits(Q, ell)) {
+                                      : quad_of<false, !GEN>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w,
+                                                             same, A.H2c, ell);
+                    if (GEN ? quad_hits(Q, ell) : quad_hits_masked(Q, ell, ~0ull)) {
                         const float t = quad_root(Q, ell);
                         if (FIRST || t < best) {
                             best = t;
@@ -403,7 +403,25 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                         }
                     }
                 };
-                if (single) {
+                if (single && !GEN) {
+                    // cylinders, at most one candidate per lane: nrel == 1 in
+                    // every lane of this loop (live starts as nrel > 0), so
+                    // the candidate is tested in every lane without branches
+                    // (the surface just left outward masked out) and the pass
+                    // result is set once
+                    const int k = (int)cl[0];
+                    int mt;
+                    const float4 g = image_at<GEN>(T, k, mt);
+                    const bool same = comp == (mt & kCompMask);
+                    const Quad Q = quad_of<false, true>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w, same,
+                                                        A.H2c, ell);
+                    const bool hit = quad_hits_masked(Q, ell, __builtin_amdgcn_ballot_w64(k != excl));
+                    float t = ell;
+                    if (hit) t = quad_root(Q, ell);
+                    best = hit ? t : __builtin_inff();
+                    bk = hit ? k : -1;
+                    bins_ = hit && Q.inside;
+                } else if (single) {
                     if (nrel > 0 && (int)cl[0] != excl) test(cl[0], std::true_type{});
                 } else {
                     for (int r = 0; r < nrel; ++r) {
@@ -490,9 +508,9 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
                         const bool same2 = comp == pcomp;
                         const Quad Q2 = sk ? quad_of<true>(sx - g.x, sy - g.y, sz - g.z, ux, uy, uz, A3, g.w * g.w,
                                                            same2, A.H2s, ell)
-                                           : quad_of<false>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy, g.w * g.w,
-                                                            same2, A.H2c, ell);
-                        live = quad_hits(Q2, ell);
+                                           : quad_of<false, !GEN>(sx - g.x, sy - g.y, 0.0f, ux, uy, uz, Axy,
+                                                                  g.w * g.w, same2, A.H2c, ell);
+                        live = GEN ? quad_hits(Q2, ell) : quad_hits_masked(Q2, ell, ~0ull);
                     }
                 }
             }
