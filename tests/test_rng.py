@@ -68,6 +68,50 @@ def test_sincos_turn_whole_grid_against_libm():
     assert np.array_equal(s2, s[::4096]) and np.array_equal(c2, c[::4096])
 
 
+def test_cylinder_walk_directions_keep_an_xy_component():
+    """The cylinder-only crossing test takes its decision without the term
+    A > 0 (A = ux^2 + uy^2, the quadratic's leading coefficient): every step
+    generator's direction has A well above zero, over the whole grids of its
+    words, and a reflection about a normal with nz = 0 keeps A to rounding."""
+    # isotropic: cos(theta) on its whole grid of 2^23 values, sin theta the
+    # correctly rounded square root of fmaf(-ct, ct, 1) (exact in fp64 here)
+    m = np.arange(1 << 23, dtype=np.float64)
+    ct = m * 2.0 ** -22 + (2.0 ** -23 - 1.0)
+    sth = np.sqrt((1.0 - ct * ct).astype(np.float32)).astype(np.float64)
+    k = np.arange(1, 1 << 24, 2, dtype=np.uint64)
+    s, c = po.sincos_turn_n((k << 8).astype(np.uint32))
+    n2 = s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2
+    a_iso = sth.min() ** 2 * n2.min() * (1.0 - 2.0 ** -20)   # the products' roundings
+    assert a_iso > 2.0 ** -23, a_iso
+    # axis signs: two components of +-0.57735026
+    assert 2 * float(np.float32(0.57735026)) ** 2 > 0.66
+    # Gaussian: A = r1^2 (c1^2 + s1^2) / |n|^2 with |n|^2 <= r1^2 + r2^2
+    y = po.neg2log_unif((k << 8).astype(np.uint32)).astype(np.float64)
+    a_gauss = y.min() * n2.min() / (2.0 * y.max()) * (1.0 - 2.0 ** -18)
+    assert a_gauss > 1e-9, a_gauss
+    # reflection u' = u - 2 (u.n) n about n = (p - c) / R, nz = 0, in fp32,
+    # from directions at the isotropic minimum of A up to A = 1
+    rng = np.random.default_rng(11)
+    n = 200000
+    f32 = np.float32
+    R = rng.uniform(0.2, 8.0, n).astype(f32)
+    ang = rng.uniform(0.0, 2 * np.pi, n)
+    cx, cy = rng.uniform(-50, 50, n).astype(f32), rng.uniform(-50, 50, n).astype(f32)
+    px = (cx + R * np.cos(ang)).astype(f32)
+    py = (cy + R * np.sin(ang)).astype(f32)
+    inv = (f32(1.0) / R).astype(f32)
+    nx, ny = ((px - cx) * inv).astype(f32), ((py - cy) * inv).astype(f32)
+    st = np.where(rng.random(n) < 0.5, sth.min(), rng.uniform(sth.min(), 1.0, n)).astype(f32)
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    ux, uy = (st * np.cos(phi)).astype(f32), (st * np.sin(phi)).astype(f32)
+    a0 = (uy * uy + ux * ux).astype(np.float64)
+    kd = (ny * uy + nx * ux).astype(f32)
+    k2 = (kd + kd).astype(f32)
+    ux2, uy2 = (ux - k2 * nx).astype(f32), (uy - k2 * ny).astype(f32)
+    a1 = (uy2 * uy2 + ux2 * ux2).astype(np.float64)
+    assert np.abs(a1 / a0 - 1.0).max() < 1e-4 and a1.min() > 2.0 ** -23
+
+
 def test_azimuth_table_nodes():
     """Entry j is the quarter-turn polynomial at 2 pi (j + 1/2)/256, within
     2e-7 of libm; a table centred on j/256 instead (a plausible off-by-half)
