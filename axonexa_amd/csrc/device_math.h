@@ -269,7 +269,8 @@ __device__ __forceinline__ bool quad_hits(const Quad& Q, float ell) {
 // walks: A = ux^2 + uy^2 of a drawn direction is at least about 2^-22 --
 // sin^2 theta of the isotropic draw, 2/3 for axis signs, r1^2 / |n|^2 for
 // Gaussian steps -- and a reflection about a normal with nz = 0 keeps it to
-// rounding).  Each compare is taken as the lane mask of its v_cmp (ballot),
+// rounding; tests/test_rng.py checks the bound over the generators' whole
+// word grids).  Each compare is taken as the lane mask of its v_cmp (ballot),
 // the masks are combined by scalar instructions and the result read back as
 // the lane's condition (inverse ballot): the same decision as quad_hits
 // without turning booleans into 0/1 vector values.  valid: lanes whose
