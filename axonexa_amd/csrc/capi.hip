@@ -637,7 +637,7 @@ void plan_free(axx_plan* p) {
         return;
     }
     void* bufs[] = {p->d_geom, p->d_meta, p->d_cyl, p->d_inv, p->d_thr, p->d_bins, p->d_cand, p->d_L, p->d_rate,
-                    p->d_prof, p->d_state, p->d_err, p->d_wfp,
+                    p->d_prof, p->d_state, p->d_err, p->d_wfp, p->d_wford,
                     p->d_wfdir, p->d_wfscale, p->d_S, p->d_partial, p->d_out, p->d_cnt, p->d_queue};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -748,6 +748,10 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     for (int i = 0; i < q->n_steps; ++i)
         for (int k = 0; k < p->P; ++k) p->S[k] += (double)q->profiles[(size_t)i * p->P + k];
     p->wf_p.assign(q->wf_profile, q->wf_profile + p->M);
+    p->wf_ord.resize(p->M);
+    for (int j = 0; j < p->M; ++j) p->wf_ord[j] = j;
+    std::stable_sort(p->wf_ord.begin(), p->wf_ord.end(),
+                     [&](int a, int b) { return p->wf_p[a] < p->wf_p[b]; });
     p->wf_dir.assign(q->wf_dir, q->wf_dir + 3 * p->M);
     p->wf_scale.resize(p->M);
     for (int j = 0; j < p->M; ++j) p->wf_scale[j] = q->gamma * q->dt * q->wf_amplitude[j];
@@ -763,6 +767,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     if ((rc = dupload(ctx, ar, &p->d_rate, H.rate.data(), H.rate.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_prof, prof.data(), prof.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_wfp, p->wf_p.data(), p->wf_p.size()))) return rc;
+    if ((rc = dupload(ctx, ar, &p->d_wford, p->wf_ord.data(), p->wf_ord.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_wfdir, p->wf_dir.data(), p->wf_dir.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_wfscale, p->wf_scale.data(), p->wf_scale.size()))) return rc;
     if ((rc = dupload(ctx, ar, &p->d_S, p->S.data(), p->S.size()))) return rc;
@@ -988,6 +993,8 @@ RedArgs plan_red_args(const axx_plan* p) {
     R.n_chunks = p->n_chunks;
     R.rate = p->d_rate;
     R.wf_p = p->d_wfp;
+    R.wf_ord = p->d_wford;
+    R.ax = p->ax;
     R.wf_dir = p->d_wfdir;
     R.wf_scale = p->d_wfscale;
     R.S = p->d_S;

@@ -43,6 +43,7 @@ struct axx_plan {
     double dt = 0;
     std::vector<double> S, wf_scale, wf_dir;
     std::vector<int> wf_p;
+    std::vector<int> wf_ord;   // waveforms ordered by profile (stable): the reduction's grouping
     // geometry sizes
     int n_img = 0, n_bins = 0, n_cand = 0, nb[3] = {1, 1, 1};
     float ib[3] = {1, 1, 1}, ib4[2] = {4, 4};
@@ -68,6 +69,7 @@ struct axx_plan {
     uint32_t* d_state = nullptr;
     int* d_err = nullptr;
     int* d_wfp = nullptr;
+    int* d_wford = nullptr;
     double* d_wfdir = nullptr;
     double* d_wfscale = nullptr;
     double* d_S = nullptr;

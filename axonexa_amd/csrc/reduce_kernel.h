@@ -31,6 +31,9 @@ struct RedArgs {
     int n_chunks;
     const float* __restrict__ rate;      // [n_comp]
     const int* __restrict__ wf_p;        // [M]
+    const int* __restrict__ wf_ord;      // [M] waveforms in profile order: group gy reduces
+                                         // wf_ord[8 gy .. 8 gy + 7]
+    int ax;                              // moment axes stored (bit a: axis a)
     const double* __restrict__ wf_dir;   // [M][3]
     const double* __restrict__ wf_scale; // [M] gamma*dt*amplitude
     const double* __restrict__ S;        // [P] sum_i profile_p(t_i)
@@ -51,7 +54,10 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // One block of kRedThreads reduces walker chunk chunk_id for waveform group
-// gy (waveforms gy*kRedJB ...); group 0 also adds the counters.
+// gy (waveforms wf_ord[gy*kRedJB ...]); group 0 also adds the counters.  The
+// waveforms of a group are taken in profile order, so a walker's moments are
+// loaded once per profile of the group rather than once per waveform; each
+// waveform's phase, sum and partial slot are the same whatever the grouping.
 __device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_id, const int gy) {
     __shared__ double sh[kRedThreads / 64][kRedJB * kRedVals];
     __shared__ unsigned long long shc[kRedThreads / 64][4];
@@ -82,17 +88,26 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_i
             cc[2] += R.state[AXX_F_NCAP * N + w];
             cc[3] += intra ? 1ull : 0ull;
         }
+        // the moments of profile pm (axes not stored are zero in the state)
+        int pm = -1;
+        float mo[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int jj = 0; jj < kRedJB; ++jj) {
-            const int j = j0 + jj;
-            if (j >= R.M) break;
+            if (j0 + jj >= R.M) break;
+            const int j = R.wf_ord[j0 + jj];
             const int p = R.wf_p[j];
+            if (p != pm) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if ((R.ax >> a) & 1) mo[a] = fs[(AXX_F_MOM + 3 * p + a) * N + w];
+                pm = p;
+            }
             double ph = 0.0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const double d = R.wf_dir[3 * j + a];
                 if (d == 0.0) continue;
-                ph += d * ((double)fs[(AXX_F_MOM + 3 * p + a) * N + w] + r0[a] * R.S[p]);
+                ph += d * ((double)mo[a] + r0[a] * R.S[p]);
             }
             const double phi = R.wf_scale[j] * ph;
             double sv, cv;
@@ -126,8 +141,8 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& R, const int chunk_i
     __syncthreads();
     if (threadIdx.x < kRedJB * kRedVals) {
         const int jj = threadIdx.x / kRedVals, v = threadIdx.x % kRedVals;
-        const int j = j0 + jj;
-        if (j < R.M) {
+        if (j0 + jj < R.M) {
+            const int j = R.wf_ord[j0 + jj];
             double s = 0.0;
             for (int wv = 0; wv < kRedThreads / 64; ++wv) s += sh[wv][threadIdx.x];
             R.partial[((size_t)j * R.n_chunks + chunk_id) * kRedVals + v] = s;
