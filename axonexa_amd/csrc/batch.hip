@@ -50,13 +50,24 @@ struct BatchWalkArgs {
     unsigned int* __restrict__ next;     // this kernel's tile counter, zeroed before the launch
 };
 
-template <int PT, bool GEN, bool REJ, bool ALT>
-__device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
+template <int PT, bool GEN, bool REJ, bool ALT, int GM>
+__device__ __forceinline__ void batch_lanes(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
     for (int w = w0 + (int)threadIdx.x; w < w1; w += (int)blockDim.x) {
         PhaseAcc<PT, 7> acc;
         acc.prof = (const_fptr)A.prof;
-        walk_lane<PhaseAcc<PT, 7>, GEN, REJ, ALT>(A, T, acc, (uint32_t)w);
+        walk_lane<PhaseAcc<PT, 7>, GEN, REJ, ALT, GM>(A, T, acc, (uint32_t)w);
     }
+}
+
+// The instance's lookup group as a compile-time constant (a wave-uniform
+// branch): spheres and planes (GEN) always take groups of two, cylinder
+// instances two or four.
+template <int PT, bool GEN, bool REJ, bool ALT>
+__device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
+    if (!GEN && A.gmask == 3)
+        batch_lanes<PT, false, REJ, ALT, 3>(A, T, w0, w1);
+    else
+        batch_lanes<PT, GEN, REJ, ALT, 1>(A, T, w0, w1);
 }
 
 // At most 128 VGPRs (four waves per SIMD, four 256-thread workgroups per CU

@@ -90,6 +90,7 @@ extern "C" void axx_destroy(axx_ctx* ctx) {
 // Substrate preparation
 
 struct HostPrep {
+    int group = 2;               // steps per cell-bin lookup (lookup_group)
     std::vector<float4> geom;
     std::vector<int> meta;
     std::vector<int> orig;
@@ -254,8 +255,8 @@ static void whole_cell_view(const axx_substrate* s, GeoView& V) {
 }
 
 // The unit-cell view when the substrate repeats (no planes) and the unit cell
-// is at least twice every radius and twice the reach of a step pair (reach),
-// else the whole cell.
+// is at least twice every radius and twice the reach of a lookup group
+// (reach), else the whole cell.
 static void unit_cell_view(const axx_substrate* s, bool allow, double reach, GeoView& V) {
     whole_cell_view(s, V);
     const int n = s->n_prims;
@@ -362,6 +363,32 @@ static float transmission_hazard(double p) {
 // i.e. at most 4.71 such units.
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
 
+// Steps per cell-bin lookup; positions wrap into the cell after the last
+// step of each group.  Four when steps have a fixed length and every surface
+// is a cylinder at least eight step lengths from every other (periodic
+// images included): a four-step reach then keeps the candidate lists short,
+// and one lookup serves two pairs.  Two otherwise (close surfaces, spheres,
+// planes or Gaussian steps, where the longer lists cost more than the saved
+// lookups).  The oracle applies the same rule with the same arithmetic.
+static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor) {
+    if (reach_factor != 1.0) return 2;
+    const int n = s->n_prims;
+    for (int k = 0; k < n; ++k)
+        if (s->type[k] != AXX_PRIM_CYLINDER_XY) return 2;
+    const double lim = 8.0 * Lmax;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j)
+            for (int sy = -1; sy <= 1; ++sy)
+                for (int sx = -1; sx <= 1; ++sx) {
+                    if (i == j && sx == 0 && sy == 0) continue;
+                    const double dx = s->center[3 * j] + sx * s->a - s->center[3 * i];
+                    const double dy = s->center[3 * j + 1] + sy * s->b - s->center[3 * i + 1];
+                    const double r = lim + s->radius[i] + s->radius[j];
+                    if (dx * dx + dy * dy < r * r) return 2;
+                }
+    return 4;
+}
+
 static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_t lds_budget,
                    double bin_width_req, int step_mode, bool allow_unit_cell, HostPrep& H) {
     const int n = s->n_prims;
@@ -396,10 +423,12 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
         if (s->type[k] == AXX_PRIM_SPHERE) H.gen = true;
     }
     H.Lmax = Lmax;
-    // reach of a pair of steps (one lookup serves both; the first may end up
-    // to one step outside the cell); a plane counts as hit within plane_eps
-    // of it, so its reach is longer
-    const double reach = 2.0 * step_reach_factor(step_mode) * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
+    H.group = lookup_group(s, Lmax, step_reach_factor(step_mode));
+    // reach of a lookup group (positions wrap at the group's end, so a
+    // lookup's start may lie up to group - 1 steps outside the cell, with the
+    // rest of the group still to walk); a plane counts as hit within
+    // plane_eps of it, so its reach is longer
+    const double reach = H.group * step_reach_factor(step_mode) * Lmax * 1.001 + 1e-6 * std::max(s->a, s->b) +
                          (any_plane ? 4.0 * H.plane_eps : 0.0);
     GeoView V;
     unit_cell_view(s, allow_unit_cell, reach, V);
@@ -543,15 +572,19 @@ static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_
 // ---------------------------------------------------------------------------
 // Kernel dispatch over (profile width, axes, general geometry, rejection) instantiations
 
-static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej, bool alt) {
-    switch (Pt) {
-    case 1: return pick_walk_p1(ax, gen, rej, alt);
-    case 2: return pick_walk_p2(ax, gen, rej, alt);
-    case 4: return pick_walk_p4(ax, gen, rej, alt);
-    case 8: return pick_walk_p8(ax, gen, rej, alt);
-    case 16: return pick_walk_p16(ax, gen, rej, alt);
-    case 32: return pick_walk_p32(ax, gen, rej, alt);
-    default: return pick_walk_p64(ax, gen, rej, alt);
+// (gen implies groups of two: lookup_group gives four to cylinders only)
+static walk_fn pick_walk(const axx_plan* p) {
+    const int ax = p->ax, gm = p->gmask;
+    const bool gen = p->gen, rej = p->run.boundary_mode == AXX_BOUNDARY_REJECTION;
+    const bool alt = p->run.step_mode != AXX_STEP_ISOTROPIC;
+    switch (p->Pt) {
+    case 1: return pick_walk_p1(ax, gen, rej, alt, gm);
+    case 2: return pick_walk_p2(ax, gen, rej, alt, gm);
+    case 4: return pick_walk_p4(ax, gen, rej, alt, gm);
+    case 8: return pick_walk_p8(ax, gen, rej, alt, gm);
+    case 16: return pick_walk_p16(ax, gen, rej, alt, gm);
+    case 32: return pick_walk_p32(ax, gen, rej, alt, gm);
+    default: return pick_walk_p64(ax, gen, rej, alt, gm);
     }
 }
 
@@ -562,8 +595,7 @@ static walk_fn pick_walk(int Pt, int ax, bool gen, bool rej, bool alt) {
 // cylinder hexagonal workload in its unit cell: 256 (6 waves per SIMD),
 // 2.18e11 walker-steps/s against 1.97e11 at 1024 (4 waves).
 static int default_block(const axx_plan* p) {
-    const walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION,
-                                 p->run.step_mode != AXX_STEP_ISOTROPIC);
+    const walk_fn fn = pick_walk(p);
     int best = 256, best_waves = -1;
     for (int blk : {256, 512, 1024}) {
         if (blk > walk_max_block(p->Pt)) continue;
@@ -714,6 +746,7 @@ int plan_create_impl(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, con
     p->ax = axes_mask(q);
     p->gen = H.gen;
     p->plane_eps = H.plane_eps;
+    p->gmask = H.group - 1;
     p->cell[0] = (float)H.cell[0]; p->cell[1] = (float)H.cell[1]; p->cell[2] = (float)H.cell[2];
     p->whole[0] = (float)s->a; p->whole[1] = (float)s->b;
     p->units = H.units;
@@ -832,6 +865,7 @@ WalkArgs plan_walk_args(const axx_plan* p, int i0, int i1) {
     A.a = p->cell[0]; A.b = p->cell[1]; A.c = p->cell[2];
     A.ca = p->whole[0]; A.cb = p->whole[1];
     A.H2c = p->H2c; A.H2s = p->H2s; A.plane_eps = p->plane_eps;
+    A.gmask = p->gmask;
     A.nbx = p->nb[0]; A.nby = p->nb[1]; A.nbz = p->nb[2];
     A.ibz = p->ib[2];
     A.ibx4 = p->ib4[0]; A.iby4 = p->ib4[1];
@@ -915,8 +949,7 @@ int plan_tasks(axx_plan* p, const void* fn, int block, int i0, int i1, WalkQueue
 static int launch_walk(axx_plan* p, int i0, int i1) {
     axx_ctx* ctx = p->ctx;
     WalkArgs A = plan_walk_args(p, i0, i1);
-    walk_fn fn = pick_walk(p->Pt, p->ax, p->gen, p->run.boundary_mode == AXX_BOUNDARY_REJECTION,
-                           p->run.step_mode != AXX_STEP_ISOTROPIC);
+    walk_fn fn = pick_walk(p);
     const int block = std::min(p->block, walk_max_block(p->Pt));
     const long long grid = (p->N + block - 1) / block;
     if (grid > 0x7fffffff) return fail(ctx, AXX_ERR_LIMIT, "grid too large");

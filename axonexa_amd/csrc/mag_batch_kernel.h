@@ -68,13 +68,15 @@ void mag_batch_walk_kernel(MagBatchArgs B) {
             MagAcc<MB, INTEG> acc;
             acc.init(G);
             if constexpr (CLS >= 0) {
-                walk_lane<MagAcc<MB, INTEG>, (CLS & 1) != 0, (CLS & 2) != 0, false>(A, T, acc, (uint32_t)w);
+                // spheres and planes (class bit 0) take lookup groups of two
+                walk_lane<MagAcc<MB, INTEG>, (CLS & 1) != 0, (CLS & 2) != 0, false, (CLS & 1) != 0 ? 1 : 0>(
+                    A, T, acc, (uint32_t)w);
             } else {
                 switch (c) {
-                case 0: walk_lane<MagAcc<MB, INTEG>, false, false, true>(A, T, acc, (uint32_t)w); break;
-                case 1: walk_lane<MagAcc<MB, INTEG>, true, false, true>(A, T, acc, (uint32_t)w); break;
-                case 2: walk_lane<MagAcc<MB, INTEG>, false, true, true>(A, T, acc, (uint32_t)w); break;
-                default: walk_lane<MagAcc<MB, INTEG>, true, true, true>(A, T, acc, (uint32_t)w); break;
+                case 0: walk_lane<MagAcc<MB, INTEG>, false, false, true, 0>(A, T, acc, (uint32_t)w); break;
+                case 1: walk_lane<MagAcc<MB, INTEG>, true, false, true, 1>(A, T, acc, (uint32_t)w); break;
+                case 2: walk_lane<MagAcc<MB, INTEG>, false, true, true, 0>(A, T, acc, (uint32_t)w); break;
+                default: walk_lane<MagAcc<MB, INTEG>, true, true, true, 1>(A, T, acc, (uint32_t)w); break;
                 }
             }
         }

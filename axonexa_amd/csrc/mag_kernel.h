@@ -160,7 +160,8 @@ template <int MB, int INTEG, bool GEN, bool REJ, bool ALT>
 __global__ __launch_bounds__(kMagBlock) __attribute__((amdgpu_waves_per_eu(mag_waves<MB, INTEG>()))) void
 mag_walk_kernel(WalkArgs A, MagArgs G, WalkQueue Q) {
     extern __shared__ __align__(16) unsigned char lds[];
-    walk_tasks<MagAcc<MB, INTEG>, GEN, REJ, ALT>(A, Q, lds, [&]() {
+    // spheres and planes (GEN) always take lookup groups of two
+    walk_tasks<MagAcc<MB, INTEG>, GEN, REJ, ALT, GEN ? 1 : 0>(A, Q, lds, [&]() {
         MagAcc<MB, INTEG> acc;
         acc.init(G);
         return acc;

@@ -49,6 +49,8 @@ struct WalkArgs {
     int mode, init_mode, init_arg, init_img;
     int step_mode;                          // 0 isotropic, 1 axis signs, 2 Gaussian
     int i0, i1, cap;
+    int gmask;                              // steps per cell-bin lookup - 1 (1 or 3): positions wrap
+                                            // into the cell after step i when (i - 1) & gmask == gmask
     int* __restrict__ err;
 };
 
@@ -190,13 +192,13 @@ struct WalkQueue {
 constexpr int kErrWaitLimit = 2;    // walk error bit: a task gave up waiting for its group
 
 typedef void (*walk_fn)(WalkArgs, WalkQueue, const float*);
-walk_fn pick_walk_p1(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p2(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p4(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p8(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p16(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p32(int ax, bool gen, bool rej, bool alt);
-walk_fn pick_walk_p64(int ax, bool gen, bool rej, bool alt);
+walk_fn pick_walk_p1(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p2(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p4(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p8(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p16(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p32(int ax, bool gen, bool rej, bool alt, int gmask);
+walk_fn pick_walk_p64(int ax, bool gen, bool rej, bool alt, int gmask);
 
 // Wide profile sets keep up to 192 moments per lane: those instantiations are
 // limited to 512 threads per block so that each lane may use 256 VGPRs.
@@ -247,8 +249,10 @@ struct PhaseAcc {
 // per-image meta words); otherwise cylinders only.  ALT: the axis-sign and
 // Gaussian step generators are compiled in (selected by A.step_mode);
 // otherwise isotropic steps only.
-template <class Acc, bool GEN, bool REJ, bool ALT>
+// GM: the lookup-group mask when fixed at compile time (1 or 3), 0 = A.gmask
+template <class Acc, bool GEN, bool REJ, bool ALT, int GM = 0>
 __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T, Acc& acc, const uint32_t wo) {
+    const int gm = GM > 0 ? GM : A.gmask;
     const long long N = A.N;
     uint32_t* st = A.state;
     float* fs = reinterpret_cast<float*>(st);
@@ -274,9 +278,12 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     // Surfaces within reach of the current start point: the bin's list,
     // narrowed by the sub-bin nibble to one entry (or the whole list when
-    // several).  The lists cover two step lengths, so one lookup serves both
-    // steps of a pair; the start may lie up to one step outside the cell
-    // (clamped to the edge sub-bin, still within reach).
+    // several).  The lists cover a lookup group of gmask + 1 steps (two or
+    // four): one lookup serves the group (steps g (gmask + 1) + 1 ..; the
+    // position wraps into the cell after its last step), and a task's first
+    // steps inside a group take a lookup of their own; such a start may lie
+    // up to gmask steps outside the cell (clamped to the edge sub-bin, still
+    // within reach of the steps left in the group).
     struct Cands {
         int nrel;
         const uint16_t* cl;
@@ -298,7 +305,7 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     // One step i: direction from the two Philox words, boundary handling,
     // displacement, phase moments; the position is wrapped into the cell
-    // after the second step of each pair (i even) (kernelLattice.cuh:46-99).
+    // after the last step of each lookup group (kernelLattice.cuh:46-99).
     auto step = [&](const int i, const uint32_t wa, const uint32_t wb, const uint32_t wc, const uint32_t wd,
                     const Cands C, const bool wrap) {
         const int nrel = C.nrel;
@@ -564,23 +571,28 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
             if (!second || i == ifirst) {
                 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
                 g = gauss(i);
-                c = lookup();
+                if (GM == 1) c = lookup();
             }
-            step(i, second ? o.z : o.x, second ? o.w : o.y, second ? g.z : g.x, second ? g.w : g.y, c, second);
+            if (GM != 1 && (((i - 1) & gm) == 0 || i == ifirst)) c = lookup();
+            step(i, second ? o.z : o.x, second ? o.w : o.y, second ? g.z : g.x, second ? g.w : g.y, c,
+                 ((i - 1) & gm) == gm);
         }
     } else {
         if (((i - 1) & 1) && i < A.i1) {
             const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
             const u32x4 g = gauss(i);
-            step(i, o.z, o.w, g.z, g.w, lookup(), true);
+            step(i, o.z, o.w, g.z, g.w, lookup(), GM == 1 || ((i - 1) & gm) == gm);
             ++i;
         }
+        // pairs: a lookup at each group's first pair and at the loop's first
+        Cands c{0, T.cand};
+        const int ipair = i;
         for (; i + 1 < A.i1; i += 2) {
             const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
             const u32x4 g = gauss(i);
-            const Cands c = lookup();
+            if (GM == 1 || ((i - 1) & gm) == 0 || i == ipair) c = lookup();
             step(i, o.x, o.y, g.x, g.y, c, false);
-            step(i + 1, o.z, o.w, g.z, g.w, c, true);
+            step(i + 1, o.z, o.w, g.z, g.w, c, GM == 1 || (i & gm) == gm);
         }
         if (i < A.i1) {
             const u32x4 o = philox10(wid, (uint32_t)(i - 1) >> 1, kStreamStep, A.inst, A.k0, A.k1);
@@ -651,7 +663,7 @@ __device__ __forceinline__ bool wait_group(const WalkQueue& Q, uint32_t g, uint3
 // budget is already full.
 constexpr int kWalkStaticLds = 64 + 4 * 16;   // static LDS of walk_tasks (bytes)
 
-template <class Acc, bool GEN, bool REJ, bool ALT, class MakeAcc>
+template <class Acc, bool GEN, bool REJ, bool ALT, int GM, class MakeAcc>
 __device__ __forceinline__ void walk_tasks(const WalkArgs& A, const WalkQueue& Q, unsigned char* lds,
                                            MakeAcc make_acc) {
     __shared__ WalkQueue s_q;
@@ -693,7 +705,7 @@ __device__ __forceinline__ void walk_tasks(const WalkArgs& A, const WalkQueue& Q
         const uint32_t w = g * 64u + lane;
         if ((long long)w < A.N) {
             Acc acc = make_acc();
-            walk_lane<Acc, GEN, REJ, ALT>(B, T, acc, w);
+            walk_lane<Acc, GEN, REJ, ALT, GM>(B, T, acc, w);
         }
         asm volatile("" ::: "memory");
         const uint32_t ng = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_q.n_groups);
@@ -709,11 +721,13 @@ __device__ __forceinline__ void walk_tasks(const WalkArgs& A, const WalkQueue& Q
     }
 }
 
-template <int P, int AX, bool GEN, bool REJ, bool ALT>
+// GM: lookup-group mask fixed at compile time (1 or 3), or 0 to read it
+// from the arguments (the axis-sign and Gaussian walks, where either occurs)
+template <int P, int AX, bool GEN, bool REJ, bool ALT, int GM>
 __global__ __launch_bounds__(walk_max_block(P)) __attribute__((amdgpu_waves_per_eu(walk_waves(P, AX, ALT)))) void
 walk_kernel(WalkArgs A, WalkQueue Q, const float* __restrict__ prof_g) {
     extern __shared__ __align__(16) unsigned char lds[];
-    walk_tasks<PhaseAcc<P, AX>, GEN, REJ, ALT>(A, Q, lds, [&]() {
+    walk_tasks<PhaseAcc<P, AX>, GEN, REJ, ALT, GM>(A, Q, lds, [&]() {
         PhaseAcc<P, AX> acc;
         acc.prof = (const_fptr)prof_g;
         return acc;

@@ -14,27 +14,32 @@
 
 namespace axx {
 
+// Lookup-group mask: spheres and planes (GEN) always take groups of two;
+// isotropic cylinder walks take the group as a compile-time constant; the
+// axis-sign and Gaussian generators (ALT) read it from the arguments.
 template <int AX, bool GEN, bool REJ>
-static walk_fn pick_alt(bool alt) {
-    return alt ? (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, true>
-               : (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, false>;
+static walk_fn pick_alt(bool alt, int gmask) {
+    if (alt) return (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, true, 0>;
+    if (GEN || gmask == 1) return (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, false, 1>;
+    return (walk_fn)walk_kernel<AXX_WALK_P, AX, false, REJ, false, 3>;
 }
 template <int AX, bool GEN>
-static walk_fn pick_rej(bool rej, bool alt) {
-    return rej ? pick_alt<AX, GEN, true>(alt) : pick_alt<AX, GEN, false>(alt);
+static walk_fn pick_rej(bool rej, bool alt, int gmask) {
+    return rej ? pick_alt<AX, GEN, true>(alt, gmask) : pick_alt<AX, GEN, false>(alt, gmask);
 }
 template <int AX>
-static walk_fn pick_gen(bool gen, bool rej, bool alt) {
-    return gen ? pick_rej<AX, true>(rej, alt) : pick_rej<AX, false>(rej, alt);
+static walk_fn pick_gen(bool gen, bool rej, bool alt, int gmask) {
+    return gen ? pick_rej<AX, true>(rej, alt, gmask) : pick_rej<AX, false>(rej, alt, gmask);
 }
 
-// ax: 1 = x, 3 = x and y, 5 = x and z, otherwise all three axes
-walk_fn AXX_PICK_NAME(AXX_WALK_P)(int ax, bool gen, bool rej, bool alt) {
+// ax: 1 = x, 3 = x and y, 5 = x and z, otherwise all three axes;
+// gmask: steps per lookup - 1 (1 or 3; 1 whenever gen)
+walk_fn AXX_PICK_NAME(AXX_WALK_P)(int ax, bool gen, bool rej, bool alt, int gmask) {
     switch (ax) {
-    case 1: return pick_gen<1>(gen, rej, alt);
-    case 3: return pick_gen<3>(gen, rej, alt);
-    case 5: return pick_gen<5>(gen, rej, alt);
-    default: return pick_gen<7>(gen, rej, alt);
+    case 1: return pick_gen<1>(gen, rej, alt, gmask);
+    case 3: return pick_gen<3>(gen, rej, alt, gmask);
+    case 5: return pick_gen<5>(gen, rej, alt, gmask);
+    default: return pick_gen<7>(gen, rej, alt, gmask);
     }
 }
 

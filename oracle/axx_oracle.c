@@ -420,6 +420,7 @@ typedef struct {
     float Lmax;
     float H2c, H2s; /* squared half of the smallest cell side (xy; xyz) */
     float plane_eps;/* minimum distance from a plane after reflecting from it */
+    int gmask;      /* steps per lookup group - 1: positions wrap after step i when i & gmask == 0 */
     int any_3d;     /* a sphere or plane is present (z matters) */
     /* optional cell bins (binned variant of the scan) */
     int nb[3];
@@ -547,6 +548,32 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
  * |n| = sqrt(2 * 33.27) = 8.157 standard normals long (24-bit uniforms),
  * 4.71 such units. */
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
+
+/* Steps between two wraps of the position into the cell (the product's
+ * lookup group, capi.hip lookup_group, same rule and arithmetic): four when
+ * steps have a fixed length and every surface is a cylinder at least eight
+ * step lengths from every other, periodic images of the whole cell included;
+ * two otherwise.  The reference wraps after every step
+ * (src/kernels/kernelLattice.cuh:96-99); where a position wraps changes no
+ * geometry, only the fp32 rounding of the coordinates. */
+static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor) {
+    if (reach_factor != 1.0) return 2;
+    int n = s->n_prims;
+    for (int k = 0; k < n; ++k)
+        if (s->type[k] != AXX_PRIM_CYLINDER_XY) return 2;
+    double lim = 8.0 * Lmax;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j)
+            for (int sy = -1; sy <= 1; ++sy)
+                for (int sx = -1; sx <= 1; ++sx) {
+                    if (i == j && sx == 0 && sy == 0) continue;
+                    double dx = s->center[3 * j] + sx * s->a - s->center[3 * i];
+                    double dy = s->center[3 * j + 1] + sy * s->b - s->center[3 * i + 1];
+                    double r = lim + s->radius[i] + s->radius[j];
+                    if (dx * dx + dy * dy < r * r) return 2;
+                }
+    return 4;
+}
 
 /* Walk a repeating substrate in its unit cell unless this is cleared (tests
  * and the reference-faithful CPU timing walk the whole cell). */
@@ -696,10 +723,13 @@ static int oprep_build_view(oprep* P, const axx_substrate* s, const axx_acq* q, 
     for (int k = 0; k < n; ++k) P->any_3d |= s->type[k] != AXX_PRIM_CYLINDER_XY;
     /* periodic images: every shift in {-1,0,1}^2 (x{-1,0,1} for spheres)
      * whose grown shape touches the cell, in (original, shift) order */
-    /* positions wrap after every second step, so a step may start up to one
-     * step length outside the cell: images within two step lengths; a plane
-     * counts as hit within plane_eps of it, so its reach is longer */
-    double reach = 2.0 * reach_factor * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) +
+    /* positions wrap after every G-th step (G = lookup_group), so a step may
+     * start up to G - 1 step lengths outside the cell: images within G step
+     * lengths; a plane counts as hit within plane_eps of it, so its reach is
+     * longer */
+    int G = lookup_group(s, Lmax, reach_factor);
+    P->gmask = G - 1;
+    double reach = G * reach_factor * Lmax * 1.001 + 1e-6 * fmax(s->a, s->b) +
                    (any_plane ? 4.0 * P->plane_eps : 0.0);
     oview V;
     oview_build(s, allow_unit_cell, reach, &V);
@@ -1292,8 +1322,8 @@ static void* job_run(void* arg) {
             dx = dx + (ex - x);
             dy = dy + (ey - y);
             dz = dz + (ez - z);
-            /* wrap after the second step of each pair (i even) */
-            if ((i & 1) == 0) {
+            /* wrap after the last step of each lookup group */
+            if ((i & P->gmask) == 0) {
                 x = wrap1(ex, P->a);
                 y = wrap1(ey, P->b);
                 /* with cylinders only, z enters no geometry: it is not wrapped */
