@@ -61,11 +61,13 @@ __device__ __forceinline__ void batch_lanes(const WalkArgs& A, const LdsTables& 
 
 // The instance's lookup group as a compile-time constant (a wave-uniform
 // branch): spheres and planes (GEN) always take groups of two, cylinder
-// instances two or four.
+// instances two, four or eight.
 template <int PT, bool GEN, bool REJ, bool ALT>
 __device__ __forceinline__ void batch_tile(const WalkArgs& A, const LdsTables& T, int w0, int w1) {
     if (!GEN && A.gmask == 3)
         batch_lanes<PT, false, REJ, ALT, 3>(A, T, w0, w1);
+    else if (!GEN && A.gmask == 7)
+        batch_lanes<PT, false, REJ, ALT, 7>(A, T, w0, w1);
     else
         batch_lanes<PT, GEN, REJ, ALT, 1>(A, T, w0, w1);
 }

@@ -364,18 +364,20 @@ static float transmission_hazard(double p) {
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
 
 // Steps per cell-bin lookup; positions wrap into the cell after the last
-// step of each group.  Four when steps have a fixed length and every surface
-// is a cylinder at least eight step lengths from every other (periodic
-// images included): a four-step reach then keeps the candidate lists short,
-// and one lookup serves two pairs.  Two otherwise (close surfaces, spheres,
-// planes or Gaussian steps, where the longer lists cost more than the saved
-// lookups).  The oracle applies the same rule with the same arithmetic.
+// step of each group.  When steps have a fixed length and every surface is a
+// cylinder: eight if every cylinder is at least sixteen step lengths from
+// every other (periodic images included), four if at least eight -- the
+// longer reach then keeps the candidate lists short, and one lookup serves
+// four or two pairs.  Two otherwise (close surfaces, spheres, planes or
+// Gaussian steps, where the longer lists cost more than the saved lookups).
+// The oracle applies the same rule with the same arithmetic.
 static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor) {
     if (reach_factor != 1.0) return 2;
     const int n = s->n_prims;
     for (int k = 0; k < n; ++k)
         if (s->type[k] != AXX_PRIM_CYLINDER_XY) return 2;
-    const double lim = 8.0 * Lmax;
+    const double lim4 = 8.0 * Lmax, lim8 = 16.0 * Lmax;
+    int g = 8;
     for (int i = 0; i < n; ++i)
         for (int j = i; j < n; ++j)
             for (int sy = -1; sy <= 1; ++sy)
@@ -383,10 +385,13 @@ static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor
                     if (i == j && sx == 0 && sy == 0) continue;
                     const double dx = s->center[3 * j] + sx * s->a - s->center[3 * i];
                     const double dy = s->center[3 * j + 1] + sy * s->b - s->center[3 * i + 1];
-                    const double r = lim + s->radius[i] + s->radius[j];
-                    if (dx * dx + dy * dy < r * r) return 2;
+                    const double d2 = dx * dx + dy * dy;
+                    const double r4 = lim4 + s->radius[i] + s->radius[j];
+                    const double r8 = lim8 + s->radius[i] + s->radius[j];
+                    if (d2 < r4 * r4) return 2;
+                    if (d2 < r8 * r8) g = 4;
                 }
-    return 4;
+    return g;
 }
 
 static int prepare(axx_ctx* ctx, const axx_substrate* s, const axx_acq* q, size_t lds_budget,

@@ -86,7 +86,7 @@ struct axx_plan {
     float whole[2] = {0, 0};        // the substrate's cell along x and y (placement)
     int units = 1;                  // unit cells in the substrate's cell
     float H2c = 0, H2s = 0, plane_eps = 0;
-    int gmask = 1;          // steps per cell-bin lookup - 1 (1 or 3)
+    int gmask = 1;          // steps per cell-bin lookup - 1 (1, 3 or 7)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

@@ -49,7 +49,7 @@ struct WalkArgs {
     int mode, init_mode, init_arg, init_img;
     int step_mode;                          // 0 isotropic, 1 axis signs, 2 Gaussian
     int i0, i1, cap;
-    int gmask;                              // steps per cell-bin lookup - 1 (1 or 3): positions wrap
+    int gmask;                              // steps per cell-bin lookup - 1 (1, 3 or 7): positions wrap
                                             // into the cell after step i when (i - 1) & gmask == gmask
     int* __restrict__ err;
 };
@@ -249,7 +249,7 @@ struct PhaseAcc {
 // per-image meta words); otherwise cylinders only.  ALT: the axis-sign and
 // Gaussian step generators are compiled in (selected by A.step_mode);
 // otherwise isotropic steps only.
-// GM: the lookup-group mask when fixed at compile time (1 or 3), 0 = A.gmask
+// GM: the lookup-group mask when fixed at compile time (1, 3 or 7), 0 = A.gmask
 template <class Acc, bool GEN, bool REJ, bool ALT, int GM = 0>
 __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T, Acc& acc, const uint32_t wo) {
     const int gm = GM > 0 ? GM : A.gmask;
@@ -278,8 +278,8 @@ __device__ __forceinline__ void walk_lane(const WalkArgs& A, const LdsTables& T,
 
     // Surfaces within reach of the current start point: the bin's list,
     // narrowed by the sub-bin nibble to one entry (or the whole list when
-    // several).  The lists cover a lookup group of gmask + 1 steps (two or
-    // four): one lookup serves the group (steps g (gmask + 1) + 1 ..; the
+    // several).  The lists cover a lookup group of gmask + 1 steps (two,
+    // four or eight): one lookup serves the group (steps g (gmask + 1) + 1 ..; the
     // position wraps into the cell after its last step), and a task's first
     // steps inside a group take a lookup of their own; such a start may lie
     // up to gmask steps outside the cell (clamped to the edge sub-bin, still
@@ -721,7 +721,7 @@ __device__ __forceinline__ void walk_tasks(const WalkArgs& A, const WalkQueue& Q
     }
 }
 
-// GM: lookup-group mask fixed at compile time (1 or 3), or 0 to read it
+// GM: lookup-group mask fixed at compile time (1, 3 or 7), or 0 to read it
 // from the arguments (the axis-sign and Gaussian walks, where either occurs)
 template <int P, int AX, bool GEN, bool REJ, bool ALT, int GM>
 __global__ __launch_bounds__(walk_max_block(P)) __attribute__((amdgpu_waves_per_eu(walk_waves(P, AX, ALT)))) void

@@ -21,7 +21,8 @@ template <int AX, bool GEN, bool REJ>
 static walk_fn pick_alt(bool alt, int gmask) {
     if (alt) return (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, true, 0>;
     if (GEN || gmask == 1) return (walk_fn)walk_kernel<AXX_WALK_P, AX, GEN, REJ, false, 1>;
-    return (walk_fn)walk_kernel<AXX_WALK_P, AX, false, REJ, false, 3>;
+    if (gmask == 3) return (walk_fn)walk_kernel<AXX_WALK_P, AX, false, REJ, false, 3>;
+    return (walk_fn)walk_kernel<AXX_WALK_P, AX, false, REJ, false, 7>;
 }
 template <int AX, bool GEN>
 static walk_fn pick_rej(bool rej, bool alt, int gmask) {
@@ -33,7 +34,7 @@ static walk_fn pick_gen(bool gen, bool rej, bool alt, int gmask) {
 }
 
 // ax: 1 = x, 3 = x and y, 5 = x and z, otherwise all three axes;
-// gmask: steps per lookup - 1 (1 or 3; 1 whenever gen)
+// gmask: steps per lookup - 1 (1, 3 or 7; 1 whenever gen)
 walk_fn AXX_PICK_NAME(AXX_WALK_P)(int ax, bool gen, bool rej, bool alt, int gmask) {
     switch (ax) {
     case 1: return pick_gen<1>(gen, rej, alt, gmask);

@@ -550,10 +550,11 @@ int orc_validate(const axx_substrate* s, const axx_acq* q) {
 static double step_reach_factor(int step_mode) { return step_mode == AXX_STEP_GAUSSIAN ? 4.75 : 1.0; }
 
 /* Steps between two wraps of the position into the cell (the product's
- * lookup group, capi.hip lookup_group, same rule and arithmetic): four when
- * steps have a fixed length and every surface is a cylinder at least eight
- * step lengths from every other, periodic images of the whole cell included;
- * two otherwise.  The reference wraps after every step
+ * lookup group, capi.hip lookup_group, same rule and arithmetic): when steps
+ * have a fixed length and every surface is a cylinder, eight if every
+ * cylinder is at least sixteen step lengths from every other (periodic
+ * images of the whole cell included), four if at least eight; two
+ * otherwise.  The reference wraps after every step
  * (src/kernels/kernelLattice.cuh:96-99); where a position wraps changes no
  * geometry, only the fp32 rounding of the coordinates. */
 static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor) {
@@ -561,7 +562,8 @@ static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor
     int n = s->n_prims;
     for (int k = 0; k < n; ++k)
         if (s->type[k] != AXX_PRIM_CYLINDER_XY) return 2;
-    double lim = 8.0 * Lmax;
+    double lim4 = 8.0 * Lmax, lim8 = 16.0 * Lmax;
+    int g = 8;
     for (int i = 0; i < n; ++i)
         for (int j = i; j < n; ++j)
             for (int sy = -1; sy <= 1; ++sy)
@@ -569,10 +571,13 @@ static int lookup_group(const axx_substrate* s, double Lmax, double reach_factor
                     if (i == j && sx == 0 && sy == 0) continue;
                     double dx = s->center[3 * j] + sx * s->a - s->center[3 * i];
                     double dy = s->center[3 * j + 1] + sy * s->b - s->center[3 * i + 1];
-                    double r = lim + s->radius[i] + s->radius[j];
-                    if (dx * dx + dy * dy < r * r) return 2;
+                    double d2 = dx * dx + dy * dy;
+                    double r4 = lim4 + s->radius[i] + s->radius[j];
+                    double r8 = lim8 + s->radius[i] + s->radius[j];
+                    if (d2 < r4 * r4) return 2;
+                    if (d2 < r8 * r8) g = 4;
                 }
-    return 4;
+    return g;
 }
 
 /* Walk a repeating substrate in its unit cell unless this is cleared (tests

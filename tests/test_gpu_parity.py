@@ -262,6 +262,23 @@ def test_certain_transmission(prob):
     assert sig.transmissions == sig.collisions > 1000
 
 
+@pytest.mark.parametrize("dt,group", [(1e-2, 2), (1.6667e-3, 4), (4.1667e-4, 8)])
+def test_lookup_groups_equal_oracle(dt, group):
+    # a hexagonal lattice with 1 um gaps walked with steps of 0.245, 0.1 and
+    # 0.05 um: lookup groups of two, four and eight steps (every cylinder at
+    # least 8 or 16 step lengths from every other); tasks of 10 steps and
+    # launches that start inside a group
+    sub = sb.hex_lattice(2.5, 6.0, 2, 1, D_in=1.0, D_ex=1.0,
+                         kappa=sb.kappa_for_probability(0.3, 1.0, dt))
+    L = math.sqrt(6.0 * 1.0 * dt)
+    gap = 6.0 - 2 * 2.5
+    assert {gap < 8 * L: 2, 8 * L <= gap < 16 * L: 4, 16 * L <= gap: 8}[True] == group
+    acq = _pgse(600, dt)
+    run = abi.RunSpec(n_walkers=2048, seed=17, boundary_mode=abi.BOUNDARY_PERMEABLE, relaxation=1)
+    sig, _ = _run_both(f"lookup_group_{group}", sub, acq, run, chunks=[1, 3, 6, 307], schedule=9)
+    assert sig.collisions > 1000 and sig.transmissions > 100
+
+
 def test_walk_tasks_inside_chunked_launches():
     # launches that begin on an even step: every task then starts inside a
     # step pair and takes the single-step path first
